@@ -1,0 +1,189 @@
+/*
+ * dna_amd.h — C ABI of the MI355X Geant4-DNA chain (transport + prechemistry,
+ * IRT chemistry, SSB/DSB scoring, spatial-hash binning, chemistry surrogate).
+ *
+ * Plain C types only: pointers, sizes, PODs.  Every entry point returns an int
+ * status (DNA_OK = 0); on failure dna_last_error() returns a message for the calling
+ * thread.  A context owns device buffers and host (pinned) output buffers; output
+ * pointers handed back in *_out structs stay valid until the next call on the same
+ * context, and the caller copies what it keeps (the reference hands out
+ * `.slice(0)` copies of mapped ranges, src/gpu/dispatch.ts:122,157,175,237).
+ * Calls on one context are serialised by the caller; use one context per thread.
+ *
+ * Reference interfaces replaced (allaunthefox/webgpu-dna):
+ *   dna_ctx_create/destroy   initGPU + allocateBuffers + createPipelines
+ *                            (src/gpu/device.ts:11, src/gpu/buffers.ts:48,
+ *                             src/gpu/pipelines.ts:40; ensurePipelines src/app.ts:65)
+ *   dna_run_transport        Phase A + Phase B of runAtEnergy (src/gpu/dispatch.ts:85-192),
+ *                            kernels src/shaders/primary.wgsl:114 and secondary.wgsl:83
+ *   dna_run_irt              runChemistryWorker (src/chemistry/worker.ts:32) ->
+ *                            public/irt-worker.js:370-699
+ *   dna_run_irt_batch        many independent worker invocations in one launch
+ *   dna_score                scoreDirectSSB_events + combineHits + clusterDSB
+ *                            (src/scoring/ssb-dsb.ts:135,413,348) with the LCG of
+ *                            makeSsbRng (ssb-dsb.ts:421) carried in/out
+ *   dna_hash_cells           spatial-hash cell assignment of src/shaders/chemistry.wgsl:66-80,
+ *                            110-122 (cell = floor(p / cell), Teschner key mod 2^bits)
+ *   dna_surrogate_*          no reference counterpart (learned IRT surrogate)
+ */
+#ifndef DNA_AMD_H
+#define DNA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNA_OK              0
+#define DNA_ERR_ARG         1   /* invalid argument                          */
+#define DNA_ERR_NO_DEVICE   2   /* no usable GPU (reference: initGPU -> null) */
+#define DNA_ERR_HIP         3   /* HIP runtime error                         */
+#define DNA_ERR_TABLES      4   /* cross-section table file missing/corrupt  */
+#define DNA_ERR_CAPACITY    5   /* a size exceeds a documented maximum        */
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct dna_ctx dna_ctx;   /* opaque context */
+
+/* Transport parameters: the uniform struct P of primary.wgsl:1-7 as packed by
+ * writePrimaryParams (src/gpu/dispatch.ts:22-41), plus three fields the reference
+ * fixes elsewhere (seed: dispatch.ts:98 passes floor(E_eV); max_sec_steps:
+ * MAX_SEC_STEPS=2000, constants.ts:66; dir_mode replaces P._pad3). */
+typedef struct dna_params {
+    uint32_t n;            /* number of primaries                                  */
+    float    box;          /* world half-width (nm)                                */
+    float    ce;           /* tracking cut / solvation threshold (eV)              */
+    uint32_t ms;           /* max primary steps (reference 65536)                  */
+    float    be;           /* beam energy (eV), used when energies == NULL         */
+    uint32_t max_sec;      /* secondary capacity, 0 = sized automatically          */
+    uint32_t vc;           /* dose voxels per axis (reference 128)                 */
+    uint32_t max_rad;      /* radical capacity, 0 = sized automatically            */
+    float    start_half;   /* start positions uniform in +-start_half (0 = origin) */
+    uint32_t dna_enable;   /* kernel-level DNA proximity counter (counters[4])     */
+    uint32_t dna_grid_n;   /* fibers per grid side                                 */
+    uint32_t dir_mode;     /* 0 isotropic start direction (reference), 1 = +z      */
+    float    dna_rise;     /* nm per bp                                            */
+    float    dna_spacing;  /* fiber spacing (nm)                                   */
+    float    dna_x0;       /* x of bp 0                                            */
+    float    dna_r_bb;     /* backbone radius (nm)                                 */
+    uint32_t seed;         /* RNG seed (reference: floor(E_eV))                    */
+    uint32_t max_sec_steps;/* secondary step cap (reference 2000)                  */
+} dna_params;
+
+/* Per-primary result, struct R of primary.wgsl:9 (32 bytes). */
+typedef struct dna_result {
+    float    path;   /* total path length (nm)                 */
+    float    prod;   /* path while E > 12 eV (nm)              */
+    float    fE;     /* final energy (eV)                      */
+    uint32_t ni;     /* ionisations                            */
+    uint32_t nx;     /* electronic excitations                 */
+    uint32_t esc;    /* 1 if the primary left the world        */
+    float    mx;     /* max distance from the start point (nm) */
+    uint32_t _pad;
+} dna_result;
+
+/* Copy flags for dna_run_transport: which large arrays to copy to host. */
+#define DNA_COPY_RESULTS  1u
+#define DNA_COPY_DOSE     2u
+#define DNA_COPY_RAD      4u
+#define DNA_COPY_ALL      7u
+
+typedef struct dna_transport_out {
+    const dna_result *results;   /* [n]            (DNA_COPY_RESULTS)                    */
+    const uint32_t   *dose;      /* [vc^3], 100 units per eV (DNA_COPY_DOSE)             */
+    const float      *rad_buf;   /* [4*rad_n_stored] (x,y,z, pid*8+code) (DNA_COPY_RAD)  */
+    uint32_t counters[8];        /* slot map primary.wgsl:24-35                          */
+    uint32_t sec_stats[8];       /* [1] cutoff, [2] bounds, [3] steps, [4] tertiary ions */
+    uint32_t dbg[8];             /* [0] sum ni, [1] sum nx, [2] escaped, [3] primaries,
+                                    [4] sum(u32(W_transfer*10)) over primary ionisations */
+    uint64_t dose_sum;           /* sum of dose[] (exact)                                */
+    uint64_t rad_n_raw, rad_n_stored, rad_dropped;
+    uint64_t sec_n_raw, sec_n, sec_dropped;
+    double   t_prim_ms, t_sec_ms, t_total_ms;   /* device time of the two phases        */
+} dna_transport_out;
+
+/* IRT chemistry output for one instance (irt-worker.js:670-698). */
+#define DNA_MAX_CKPT 32
+#define DNA_N_RXN 9
+typedef struct dna_chem_out {
+    int      n_ckpt;
+    double   t_ckpt[DNA_MAX_CKPT];
+    /* species tallies per checkpoint, exact integers: 0 OH, 1 e-aq, 2 H, 3 H2O2, 4 H2 */
+    uint64_t tally[DNA_MAX_CKPT][5];
+    double   G[DNA_MAX_CKPT][5];       /* tally / (n_therm * E / 100)        */
+    uint64_t rxn_counts[DNA_N_RXN];
+    uint64_t n_reacted;
+    uint64_t chem_n;                   /* rad entries handed in              */
+    uint64_t n_tracks;                 /* primaries with at least one radical */
+    uint64_t n_events;                 /* candidate reaction events sampled  */
+    double   t_ms;                     /* device time                        */
+} dna_chem_out;
+
+/* DNA fiber target, DNATarget of src/physics/types.ts:23-37 / buildDNATarget
+ * (src/physics/dna-geometry.ts:16-68).  Arrays are f32 as in the reference. */
+typedef struct dna_dna_target {
+    double rise, r_bb, spacing_nm, x0, L_nm;
+    uint32_t n_bp_per, n_fibers, n_bp, grid_N;
+    const float *fy, *fz;      /* [n_fibers]     */
+    const float *rbb0, *rbb1;  /* [2 * n_bp_per] */
+} dna_dna_target;
+
+typedef struct dna_score_out {
+    uint64_t ssb_count, candidates, in_reach;   /* DirectSSBResult (types.ts:131-136)   */
+    uint64_t dsb, ssb0, ssb1;                    /* DSBClusterResult (types.ts:139-144)  */
+    double   dsb_expected;
+    const uint8_t *hits;                         /* [2*n_bp] combined hits, if requested */
+    double   t_ms;
+} dna_score_out;
+
+/* ------------------------------------------------------------ entry points */
+
+/* Create a context on HIP device `device`, loading the cross-section table file
+ * (NULL = the cross_sections.bin shipped next to the library). */
+int  dna_ctx_create(int device, const char *table_path, struct dna_ctx **out);
+void dna_ctx_destroy(struct dna_ctx *ctx);
+const char *dna_last_error(void);
+const char *dna_version(void);
+
+/* Phase A + B: transport n primaries (energies[i] per primary, or params->be for all
+ * when energies == NULL) with in-kernel prechemistry.  The radical buffer is left on
+ * the device in canonical order (by primary id, then by emitting particle — the
+ * primary first, then its secondaries in emission order — then emission order) for
+ * the chained dna_run_irt / dna_score calls that pass rad == NULL. */
+int dna_run_transport(struct dna_ctx *ctx, const dna_params *p, const float *energies,
+                      uint32_t copy_flags, dna_transport_out *out);
+
+/* Phase C: IRT to the last checkpoint (reference 1000 ns) for every primary found in
+ * rad (host pointer, 4*rad_n floats) or, when rad == NULL, in the device-resident
+ * radical buffer of the last dna_run_transport.  seed keys the counter-based RNG. */
+int dna_run_irt(struct dna_ctx *ctx, const float *rad, uint64_t rad_n, uint32_t n_therm,
+                float E_eV, const double *t_ckpt, int n_ckpt, uint64_t seed,
+                dna_chem_out *out);
+
+/* Many independent IRT instances in one launch: instance i owns radical entries
+ * [inst_off[i], inst_off[i+1]) of rad (host pointer, n_inst+1 offsets). */
+int dna_run_irt_batch(struct dna_ctx *ctx, const float *rad, const uint64_t *inst_off,
+                      uint32_t n_inst, const uint32_t *n_therm, const float *E_eV,
+                      const double *t_ckpt, int n_ckpt, uint64_t seed,
+                      dna_chem_out *out /* [n_inst] */);
+
+/* Direct-event SSB scoring + DSB clustering on tgt.  rad == NULL uses the
+ * device-resident radical buffer.  *lcg_state is the makeSsbRng state before the call
+ * and is advanced by the number of draws consumed.  copy_hits != 0 copies the
+ * combined per-strand hit array to out->hits. */
+int dna_score(struct dna_ctx *ctx, const dna_dna_target *tgt, const float *rad,
+              uint64_t rad_n, uint32_t *lcg_state, int copy_hits, dna_score_out *out);
+
+/* Spatial-hash binning: for n points (x,y,z,w float4, host pointer) compute
+ * cell = floor(p / cell_nm) (f32 division) and key = Teschner hash mod 2^hash_bits;
+ * then counting-sort the points by key: cell_start[2^hash_bits + 1] (CSR offsets) and
+ * order[n] (point indices, stable within a bucket).  Outputs are host pointers. */
+int dna_hash_cells(struct dna_ctx *ctx, const float *pos4, uint64_t n, float cell_nm,
+                   uint32_t hash_bits, uint32_t *key_out, uint32_t *cell_start_out,
+                   uint32_t *order_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNA_AMD_H */
