@@ -1,0 +1,60 @@
+// dna_ctx.h — the opaque context behind the C ABI.
+#pragma once
+#include "dna_internal.h"
+
+struct IrtBuffers {
+    void   *scratch = nullptr;  size_t cap_scratch = 0;   // per-track particle arrays
+    void   *events = nullptr;   size_t cap_events = 0;    // per-track candidate events
+    void   *tracks = nullptr;   size_t cap_tracks = 0;    // track descriptors
+    void   *part = nullptr;     size_t cap_part = 0;      // grouping scratch
+    void   *rad_in = nullptr;   size_t cap_rad_in = 0;    // host-supplied radicals
+    void   *tally = nullptr;    size_t cap_tally = 0;     // per-instance tallies
+};
+
+struct ScoreBuffers {
+    void *hitbits = nullptr;  size_t cap_hitbits = 0;
+    void *flags = nullptr;    size_t cap_flags = 0;
+    void *rank = nullptr;     size_t cap_rank = 0;
+    void *hits_u8 = nullptr;  size_t cap_hits_u8 = 0;
+    void *rad_in = nullptr;   size_t cap_rad_in = 0;
+    void *tgt = nullptr;      size_t cap_tgt = 0;
+    uint8_t *h_hits = nullptr; size_t hcap_hits = 0;
+};
+
+struct HashBuffers {
+    void *buf = nullptr; size_t cap = 0;
+};
+
+struct dna_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    dna_tables h_tab;
+    dna_tables *d_tab = nullptr;
+    uint32_t *d_ctr = nullptr, *h_ctr = nullptr;
+    unsigned long long *d_u64 = nullptr;
+    unsigned long long h_u64_tmp = 0;
+    hipEvent_t ev[8] = {};
+    // transport
+    dna_result *d_results = nullptr;  size_t cap_n_results = 0;
+    uint32_t *d_prim_nsec = nullptr;   size_t cap_prim_nsec = 0;
+    uint32_t *d_prim_nrad = nullptr;   size_t cap_prim_nrad = 0;
+    float    *d_energies = nullptr;    size_t cap_energies = 0;
+    uint32_t *d_base_p = nullptr;      size_t cap_base_p = 0;
+    uint32_t *d_dose = nullptr;        size_t cap_dose = 0;
+    float4   *d_rad_raw = nullptr;     size_t cap_rad_raw = 0;
+    uint64_t *d_rad_key = nullptr;     size_t cap_rad_key = 0;
+    float4   *d_rad = nullptr;         size_t cap_rad = 0;      // canonical radicals
+    dna::SecRec *d_sec = nullptr;      size_t cap_sec = 0;
+    uint32_t *d_sec_nrad = nullptr;    size_t cap_sec_nrad = 0;
+    uint32_t *d_cnt = nullptr;         size_t cap_cnt = 0;
+    uint32_t *d_block = nullptr;       size_t cap_block = 0;
+    uint64_t rad_n = 0;                // canonical radicals resident on the device
+    uint32_t last_n = 0;
+    // host outputs (pinned)
+    dna_result *h_results = nullptr;   size_t hcap_results = 0;
+    uint32_t *h_dose = nullptr;        size_t hcap_dose = 0;
+    float *h_rad = nullptr;            size_t hcap_rad = 0;
+    IrtBuffers irt;
+    ScoreBuffers sc;
+    HashBuffers hs;
+};

@@ -1,0 +1,64 @@
+// dna_internal.h — declarations shared by the HIP translation units of libdna_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include "../../include/dna_amd.h"
+#include "../../include/dna_tables.h"
+
+#define DNA_HIP(expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) { \
+    dna::set_error(std::string(#expr) + ": " + hipGetErrorString(e_)); return DNA_ERR_HIP; } } while (0)
+
+namespace dna {
+
+void set_error(const std::string &msg);
+
+// Secondary-electron record of the device work queue (64 bytes).
+struct SecRec {
+    float4 posE;      // x, y, z, E
+    float4 dir;       // dx, dy, dz, unused
+    uint4  rng;       // xoshiro128++ state
+    uint32_t pid;     // parent primary
+    uint32_t k;       // 1-based emission index within the parent
+    uint32_t pad0, pad1;
+};
+
+// Device buffers and counters used by one transport run.
+struct TransportDev {
+    const dna_tables *tab;      // device copy of the tables
+    dna_params p;
+    const float *energies;      // [n] or null
+    dna_result *results;        // [n]
+    uint32_t *dose;             // [vc^3]
+    float4   *rad_raw;          // [rad_cap] unordered emissions
+    uint64_t *rad_key;          // [rad_cap] pid<<40 | k<<20 | seq
+    uint32_t rad_cap;
+    SecRec   *sec;              // [sec_cap]
+    uint32_t sec_cap;
+    uint32_t *prim_nsec;        // [n] secondaries emitted by each primary
+    uint32_t *prim_nrad;        // [n] radical entries emitted by each primary
+    uint32_t *sec_nrad;         // [sec_cap] radical entries emitted by each secondary
+    uint32_t *ctr;              // [32] global counters, layout below
+};
+// ctr layout
+enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, C_RAD = 7,
+       C_SST = 8 /* 8..15 sec_stats */, C_DBG = 16 /* 16..23 dbg */, C_PNEXT = 24, C_SNEXT = 25,
+       C_NCTR = 32 };
+
+hipError_t launch_phase_a(const TransportDev &d, hipStream_t st);
+hipError_t launch_phase_b(const TransportDev &d, uint32_t sec_n, hipStream_t st);
+
+// canonical ordering: returns the device canonical buffer through out pointers
+struct CanonScratch {
+    uint32_t *base_p;     // [n+1]
+    uint32_t *cnt;        // [P+1]
+    uint32_t *block;      // scan partials
+    size_t    block_cap;
+};
+hipError_t launch_canonicalize(const TransportDev &d, uint32_t n, uint32_t sec_n, uint32_t rad_n,
+                               CanonScratch &cs, float4 *rad_out, hipStream_t st);
+
+// exclusive scan of u32 (n elements) in place; out[n] = total. tmp holds >= ceil(n/1024)+1 words.
+hipError_t exclusive_scan_u32(uint32_t *data, uint32_t n, uint32_t *tmp, hipStream_t st);
+
+}  // namespace dna

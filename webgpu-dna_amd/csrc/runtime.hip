@@ -1,0 +1,222 @@
+// runtime.hip — context, device memory and the transport entry point of the C ABI
+// (include/dna_amd.h).  Replaces initGPU/allocateBuffers/createPipelines and the Phase A/B
+// part of runAtEnergy (src/gpu/device.ts:11, buffers.ts:48, pipelines.ts:40,
+// dispatch.ts:85-192).
+#include "dna_internal.h"
+#include "dna_ctx.h"
+#include <dlfcn.h>
+#include <cstdio>
+#include <cstring>
+#include <cstdlib>
+#include <vector>
+
+namespace dna {
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+}  // namespace dna
+
+using namespace dna;
+
+extern "C" const char *dna_last_error(void) { return g_err.c_str(); }
+extern "C" const char *dna_version(void) { return "dna_amd 0.1 (gfx950)"; }
+
+static std::string default_table_path()
+{
+    Dl_info info;
+    if (dladdr((void *)&default_table_path, &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t s = p.rfind('/');
+        std::string dir = (s == std::string::npos) ? "." : p.substr(0, s);
+        return dir + "/../data/cross_sections.bin";
+    }
+    return "cross_sections.bin";
+}
+
+template <class T> static hipError_t grow(T *&ptr, size_t &cap, size_t need)
+{
+    if (need <= cap && ptr) return hipSuccess;
+    if (ptr) { hipError_t e = hipFree(ptr); if (e != hipSuccess) return e; ptr = nullptr; }
+    size_t n = need < 16 ? 16 : need;
+    hipError_t e = hipMalloc((void **)&ptr, n * sizeof(T));
+    if (e == hipSuccess) cap = n; else cap = 0;
+    return e;
+}
+
+template <class T> static hipError_t grow_host(T *&ptr, size_t &cap, size_t need)
+{
+    if (need <= cap && ptr) return hipSuccess;
+    if (ptr) { hipHostFree(ptr); ptr = nullptr; }
+    size_t n = need < 16 ? 16 : need;
+    hipError_t e = hipHostMalloc((void **)&ptr, n * sizeof(T), hipHostMallocDefault);
+    if (e == hipSuccess) cap = n; else cap = 0;
+    return e;
+}
+
+extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
+{
+    if (!out) { set_error("dna_ctx_create: out is NULL"); return DNA_ERR_ARG; }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return DNA_ERR_NO_DEVICE; }
+    if (device < 0 || device >= ndev) { set_error("device index out of range"); return DNA_ERR_ARG; }
+    std::string path = table_path ? std::string(table_path) : default_table_path();
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) { set_error("cannot open table file " + path); return DNA_ERR_TABLES; }
+    std::vector<char> buf(sizeof(dna_tables) + 1);
+    size_t got = fread(buf.data(), 1, buf.size(), f);
+    fclose(f);
+    if (got != sizeof(dna_tables) || sizeof(dna_tables) != DNA_TABLES_BYTES) {
+        set_error("table file " + path + " has the wrong size"); return DNA_ERR_TABLES;
+    }
+    DNA_HIP(hipSetDevice(device));
+    dna_ctx *c = new dna_ctx();
+    c->device = device;
+    memcpy(&c->h_tab, buf.data(), sizeof(dna_tables));
+    DNA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    DNA_HIP(hipMalloc((void **)&c->d_tab, sizeof(dna_tables)));
+    DNA_HIP(hipMemcpy(c->d_tab, &c->h_tab, sizeof(dna_tables), hipMemcpyHostToDevice));
+    DNA_HIP(hipMalloc((void **)&c->d_ctr, C_NCTR * sizeof(uint32_t)));
+    DNA_HIP(hipHostMalloc((void **)&c->h_ctr, C_NCTR * sizeof(uint32_t), hipHostMallocDefault));
+    DNA_HIP(hipMalloc((void **)&c->d_u64, 64 * sizeof(unsigned long long)));
+    for (int i = 0; i < 8; i++) DNA_HIP(hipEventCreate(&c->ev[i]));
+    *out = c;
+    return DNA_OK;
+}
+
+extern "C" void dna_ctx_destroy(dna_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    void *dev[] = {c->d_tab, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
+                   c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
+                   c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
+                   c->irt.part, c->irt.rad_in, c->irt.tally, c->sc.hitbits, c->sc.flags, c->sc.rank,
+                   c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf};
+    for (void *p : dev) if (p) hipFree(p);
+    void *host[] = {c->h_ctr, c->h_results, c->h_dose, c->h_rad, c->sc.h_hits};
+    for (void *p : host) if (p) hipHostFree(p);
+    for (int i = 0; i < 8; i++) hipEventDestroy(c->ev[i]);
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+__global__ void k_dose_sum(const uint32_t *dose, size_t n, unsigned long long *out)
+{
+    unsigned long long s = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += dose[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(out, s);
+}
+
+static int ensure_transport(dna_ctx *c, uint32_t n, size_t rad_cap, size_t sec_cap, uint32_t vc)
+{
+    size_t v3 = (size_t)vc * vc * vc;
+    DNA_HIP(grow(c->d_results, c->cap_n_results, n));
+    DNA_HIP(grow(c->d_prim_nsec, c->cap_prim_nsec, n));
+    DNA_HIP(grow(c->d_prim_nrad, c->cap_prim_nrad, n));
+    DNA_HIP(grow(c->d_energies, c->cap_energies, n));
+    DNA_HIP(grow(c->d_base_p, c->cap_base_p, (size_t)n + 1));
+    DNA_HIP(grow(c->d_dose, c->cap_dose, v3));
+    DNA_HIP(grow(c->d_rad_raw, c->cap_rad_raw, rad_cap));
+    DNA_HIP(grow(c->d_rad_key, c->cap_rad_key, rad_cap));
+    DNA_HIP(grow(c->d_rad, c->cap_rad, rad_cap));
+    DNA_HIP(grow(c->d_sec, c->cap_sec, sec_cap));
+    DNA_HIP(grow(c->d_sec_nrad, c->cap_sec_nrad, sec_cap));
+    DNA_HIP(grow(c->d_cnt, c->cap_cnt, (size_t)n + sec_cap + 1));
+    size_t nb = ((size_t)n + sec_cap + 1 + 1023) / 1024 + 2;
+    DNA_HIP(grow(c->d_block, c->cap_block, nb));
+    return DNA_OK;
+}
+
+extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float *energies,
+                                 uint32_t copy_flags, dna_transport_out *out)
+{
+    if (!c || !p_in || !out) { set_error("dna_run_transport: NULL argument"); return DNA_ERR_ARG; }
+    dna_params p = *p_in;
+    if (p.n == 0 || p.vc == 0 || !(p.box > 0.0f) || p.ms == 0) { set_error("dna_run_transport: bad params"); return DNA_ERR_ARG; }
+    if (p.n >= (1u << 24)) { set_error("dna_run_transport: at most 2^24-1 primaries"); return DNA_ERR_CAPACITY; }
+    if ((size_t)p.vc * p.vc * p.vc > (1ull << 31)) { set_error("dna_run_transport: vc too large"); return DNA_ERR_CAPACITY; }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    double sumE = 0.0;
+    if (energies) for (uint32_t i = 0; i < p.n; i++) sumE += energies[i] > 0 ? energies[i] : 0.0;
+    else sumE = (double)p.n * (p.be > 0 ? p.be : 0.0);
+    // capacity: reference yields are ~0.13 radical entries and ~0.0145 secondaries per eV;
+    // grow and repeat the (deterministic) run if a count overflows
+    size_t rad_cap = p.max_rad ? p.max_rad : (size_t)(0.2 * sumE) + 65536;
+    size_t sec_cap = p.max_sec ? p.max_sec : (size_t)(0.025 * sumE) + 4096;
+    const size_t HARD = (size_t)0xFFFFFFF0u;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        if (rad_cap > HARD) rad_cap = HARD;
+        if (sec_cap > HARD) sec_cap = HARD;
+        int rc = ensure_transport(c, p.n, rad_cap, sec_cap, p.vc);
+        if (rc) return rc;
+        if (energies) DNA_HIP(hipMemcpyAsync(c->d_energies, energies, p.n * sizeof(float), hipMemcpyHostToDevice, st));
+        size_t v3 = (size_t)p.vc * p.vc * p.vc;
+        DNA_HIP(hipMemsetAsync(c->d_ctr, 0, C_NCTR * sizeof(uint32_t), st));
+        DNA_HIP(hipMemsetAsync(c->d_dose, 0, v3 * sizeof(uint32_t), st));
+        DNA_HIP(hipMemsetAsync(c->d_u64, 0, 64 * sizeof(unsigned long long), st));
+        TransportDev d;
+        d.tab = c->d_tab; d.p = p; d.energies = energies ? c->d_energies : nullptr;
+        d.results = c->d_results; d.dose = c->d_dose; d.rad_raw = c->d_rad_raw; d.rad_key = c->d_rad_key;
+        d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
+        d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
+        DNA_HIP(hipEventRecord(c->ev[0], st));
+        DNA_HIP(launch_phase_a(d, st));
+        DNA_HIP(hipEventRecord(c->ev[1], st));
+        DNA_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, C_NCTR * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipStreamSynchronize(st));
+        uint32_t sec_raw = c->h_ctr[C_SEC];
+        if (sec_raw > sec_cap) { sec_cap = (size_t)sec_raw * 5 / 4 + 4096; continue; }
+        uint32_t sec_n = sec_raw;
+        DNA_HIP(hipEventRecord(c->ev[2], st));
+        DNA_HIP(launch_phase_b(d, sec_n, st));
+        DNA_HIP(hipEventRecord(c->ev[3], st));
+        DNA_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, C_NCTR * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipStreamSynchronize(st));
+        uint32_t rad_raw = c->h_ctr[C_RAD];
+        if (rad_raw > rad_cap) { rad_cap = (size_t)rad_raw * 5 / 4 + 65536; continue; }
+        CanonScratch cs; cs.base_p = c->d_base_p; cs.cnt = c->d_cnt; cs.block = c->d_block; cs.block_cap = c->cap_block;
+        DNA_HIP(launch_canonicalize(d, p.n, sec_n, rad_raw, cs, c->d_rad, st));
+        hipLaunchKernelGGL(k_dose_sum, dim3(1024), dim3(256), 0, st, c->d_dose, v3, c->d_u64);
+        DNA_HIP(hipGetLastError());
+        DNA_HIP(hipEventRecord(c->ev[4], st));
+        // copies
+        unsigned long long dsum = 0;
+        DNA_HIP(hipMemcpyAsync(&c->h_u64_tmp, c->d_u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        out->results = nullptr; out->dose = nullptr; out->rad_buf = nullptr;
+        if (copy_flags & DNA_COPY_RESULTS) {
+            DNA_HIP(grow_host(c->h_results, c->hcap_results, p.n));
+            DNA_HIP(hipMemcpyAsync(c->h_results, c->d_results, p.n * sizeof(dna_result), hipMemcpyDeviceToHost, st));
+            out->results = c->h_results;
+        }
+        if (copy_flags & DNA_COPY_DOSE) {
+            DNA_HIP(grow_host(c->h_dose, c->hcap_dose, v3));
+            DNA_HIP(hipMemcpyAsync(c->h_dose, c->d_dose, v3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+            out->dose = c->h_dose;
+        }
+        if ((copy_flags & DNA_COPY_RAD) && rad_raw) {
+            DNA_HIP(grow_host(c->h_rad, c->hcap_rad, (size_t)rad_raw * 4));
+            DNA_HIP(hipMemcpyAsync(c->h_rad, c->d_rad, (size_t)rad_raw * 16, hipMemcpyDeviceToHost, st));
+            out->rad_buf = c->h_rad;
+        }
+        DNA_HIP(hipStreamSynchronize(st));
+        dsum = c->h_u64_tmp;
+        for (int i = 0; i < 8; i++) { out->counters[i] = c->h_ctr[i]; out->sec_stats[i] = c->h_ctr[C_SST + i]; out->dbg[i] = c->h_ctr[C_DBG + i]; }
+        out->dose_sum = dsum;
+        out->rad_n_raw = rad_raw; out->rad_n_stored = rad_raw; out->rad_dropped = 0;
+        out->sec_n_raw = sec_raw; out->sec_n = sec_n; out->sec_dropped = 0;
+        float ta = 0, tb = 0, tt = 0;
+        hipEventElapsedTime(&ta, c->ev[0], c->ev[1]);
+        hipEventElapsedTime(&tb, c->ev[2], c->ev[3]);
+        hipEventElapsedTime(&tt, c->ev[0], c->ev[4]);
+        out->t_prim_ms = ta; out->t_sec_ms = tb; out->t_total_ms = tt;
+        c->rad_n = rad_raw;
+        c->last_n = p.n;
+        return DNA_OK;
+    }
+    set_error("dna_run_transport: capacity estimate kept overflowing");
+    return DNA_ERR_CAPACITY;
+}

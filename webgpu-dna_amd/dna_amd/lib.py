@@ -1,0 +1,158 @@
+"""Loader and thin object wrapper of libdna_amd.so (the HIP C ABI of include/dna_amd.h).
+
+There is no fallback: if the library or a GPU is missing, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (DNA_COPY_ALL, DnaChemOut, DnaParams, DnaResult, DnaScoreOut, DnaTarget,
+                  DnaTransportOut)
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libdna_amd.so")
+TABLE_PATH = os.path.join(PKG, "data", "cross_sections.bin")
+
+_lib = None
+
+
+class DnaError(RuntimeError):
+    def __init__(self, fn, rc, msg):
+        super().__init__(f"{fn} failed (status {rc}): {msg}")
+        self.rc = rc
+
+
+def lib():
+    """Load libdna_amd.so (built by webgpu-dna_amd/build.py); raise if it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: run webgpu-dna_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+    L.dna_last_error.restype = C.c_char_p
+    L.dna_version.restype = C.c_char_p
+    L.dna_ctx_create.argtypes = [C.c_int, C.c_char_p, C.POINTER(vp)]
+    L.dna_ctx_destroy.argtypes = [vp]
+    L.dna_run_transport.argtypes = [vp, C.POINTER(DnaParams), vp, u32, C.POINTER(DnaTransportOut)]
+    L.dna_run_irt.argtypes = [vp, vp, u64, u32, C.c_float, vp, C.c_int, u64, C.POINTER(DnaChemOut)]
+    L.dna_run_irt_batch.argtypes = [vp, vp, vp, u32, vp, vp, vp, C.c_int, u64, vp]
+    L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
+                            C.POINTER(DnaScoreOut)]
+    L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
+    _lib = L
+    return L
+
+
+EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_version",
+                    "dna_run_transport", "dna_run_irt", "dna_run_irt_batch", "dna_score",
+                    "dna_hash_cells"]
+
+
+def _check(fn, rc):
+    if rc != 0:
+        raise DnaError(fn, rc, lib().dna_last_error().decode())
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+class Context:
+    """One dna_ctx (one GPU).  Output arrays returned by methods are copies."""
+
+    def __init__(self, device=0, table_path=TABLE_PATH):
+        L = lib()
+        h = C.c_void_p()
+        _check("dna_ctx_create", L.dna_ctx_create(device, table_path.encode(), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().dna_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- Phase A + B
+    def transport(self, params, energies=None, copy=DNA_COPY_ALL):
+        e = None if energies is None else np.ascontiguousarray(energies, np.float32)
+        o = DnaTransportOut()
+        _check("dna_run_transport", lib().dna_run_transport(self.h, C.byref(params), _ptr(e), copy,
+                                                            C.byref(o)))
+        out = dict(counters=np.array(o.counters[:], np.uint32),
+                   sec_stats=np.array(o.sec_stats[:], np.uint32),
+                   dbg=np.array(o.dbg[:], np.uint32), dose_sum=int(o.dose_sum),
+                   rad_n=int(o.rad_n_stored), rad_n_raw=int(o.rad_n_raw),
+                   rad_dropped=int(o.rad_dropped), sec_n=int(o.sec_n),
+                   sec_n_raw=int(o.sec_n_raw), sec_dropped=int(o.sec_dropped),
+                   t_prim_ms=o.t_prim_ms, t_sec_ms=o.t_sec_ms, t_total_ms=o.t_total_ms)
+        n = params.n
+        if o.results:
+            out["results"] = np.ctypeslib.as_array(C.cast(o.results, C.POINTER(DnaResult)), (n,)).copy()
+        if o.dose:
+            nv = params.vc ** 3
+            out["dose"] = np.ctypeslib.as_array(C.cast(o.dose, C.POINTER(C.c_uint32)), (nv,)).copy()
+        if o.rad_buf:
+            m = out["rad_n"]
+            out["rad"] = np.ctypeslib.as_array(C.cast(o.rad_buf, C.POINTER(C.c_float)),
+                                               (m * 4,)).copy().reshape(m, 4)
+        return out
+
+    # ---- Phase C
+    def irt(self, rad, rad_n, n_therm, E_eV, t_ckpt, seed=1):
+        """rad: (n,4) float32 array, or None for the device-resident radicals."""
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        tc = np.ascontiguousarray(t_ckpt, np.float64)
+        o = DnaChemOut()
+        n = rad_n if r is None else r.shape[0]
+        _check("dna_run_irt", lib().dna_run_irt(self.h, _ptr(r), n, n_therm, E_eV, _ptr(tc),
+                                                tc.size, seed, C.byref(o)))
+        return o
+
+    def irt_batch(self, rad, inst_off, n_therm, E, t_ckpt, seed=1):
+        r = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        off = np.ascontiguousarray(inst_off, np.uint64)
+        nt = np.ascontiguousarray(n_therm, np.uint32)
+        Ee = np.ascontiguousarray(E, np.float32)
+        tc = np.ascontiguousarray(t_ckpt, np.float64)
+        n_inst = off.size - 1
+        outs = (DnaChemOut * n_inst)()
+        _check("dna_run_irt_batch", lib().dna_run_irt_batch(self.h, _ptr(r), _ptr(off), n_inst,
+                                                            _ptr(nt), _ptr(Ee), _ptr(tc), tc.size,
+                                                            seed, outs))
+        return list(outs)
+
+    # ---- scoring
+    def score(self, target, rad=None, rad_n=0, lcg_state=0x12345678, want_hits=False):
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        t = target.to_c()
+        st = C.c_uint32(lcg_state)
+        o = DnaScoreOut()
+        n = rad_n if r is None else r.shape[0]
+        _check("dna_score", lib().dna_score(self.h, C.byref(t), _ptr(r), n, C.byref(st),
+                                            1 if want_hits else 0, C.byref(o)))
+        d = dict(ssb_count=int(o.ssb_count), candidates=int(o.candidates),
+                 in_reach=int(o.in_reach), dsb=int(o.dsb), ssb0=int(o.ssb0), ssb1=int(o.ssb1),
+                 dsb_expected=o.dsb_expected, lcg_state=st.value, t_ms=o.t_ms)
+        if want_hits and o.hits:
+            d["hits"] = np.ctypeslib.as_array(C.cast(o.hits, C.POINTER(C.c_uint8)),
+                                              (2 * target.n_bp,)).copy()
+        return d
+
+    # ---- spatial hash
+    def hash_cells(self, pos4, cell_nm=1.5, bits=23):
+        p = np.ascontiguousarray(pos4, np.float32).reshape(-1, 4)
+        n = p.shape[0]
+        key = np.empty(n, np.uint32)
+        start = np.empty((1 << bits) + 1, np.uint32)
+        order = np.empty(n, np.uint32)
+        _check("dna_hash_cells", lib().dna_hash_cells(self.h, _ptr(p), n, cell_nm, bits, _ptr(key),
+                                                      _ptr(start), _ptr(order)))
+        return key, start, order
