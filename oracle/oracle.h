@@ -78,6 +78,7 @@ int oracle_score(const dna_dna_target *tgt, const float *rad, uint64_t rad_n,
                  uint32_t *lcg_state, uint8_t *hits_out /* [2*n_bp] or NULL */,
                  dna_score_out *out);
 
+void oracle_cluster(const dna_dna_target *tgt, const uint8_t *hits, dna_score_out *out);
 /* Spatial hash (chemistry.wgsl:66-80,110-122) */
 void oracle_hash_cells(const float *pos4, uint64_t n, float cell_nm, uint32_t hash_bits,
                        uint32_t *key_out, uint32_t *cell_start_out, uint32_t *order_out);

@@ -67,6 +67,7 @@ def lib():
         L.oracle_irt_sample.restype = C.c_double
         L.oracle_score.argtypes = [C.POINTER(DnaTarget), C.c_void_p, C.c_uint64,
                                    C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(DnaScoreOut)]
+        L.oracle_cluster.argtypes = [C.POINTER(DnaTarget), C.c_void_p, C.POINTER(DnaScoreOut)]
         L.oracle_hash_cells.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_uint32,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
         rc = L.oracle_load_tables(TABLES.encode())
@@ -198,3 +199,11 @@ def hash_cells(pos4, cell_nm=1.5, bits=23):
     order = np.empty(n, np.uint32)
     lib().oracle_hash_cells(_ptr(pos4), n, cell_nm, bits, _ptr(key), _ptr(start), _ptr(order))
     return key, start, order
+
+
+def cluster(target, hits):
+    hits = np.ascontiguousarray(hits, np.uint8)
+    t = target.to_c()
+    o = DnaScoreOut()
+    lib().oracle_cluster(C.byref(t), _ptr(hits), C.byref(o))
+    return dict(dsb=o.dsb, ssb0=o.ssb0, ssb1=o.ssb1, dsb_expected=o.dsb_expected)

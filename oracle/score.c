@@ -134,3 +134,10 @@ void oracle_hash_cells(const float *pos4, uint64_t n, float cell_nm, uint32_t bi
     for (uint64_t i = 0; i < n; i++) order[fill[key[i]]++] = (uint32_t)i;
     free(fill);
 }
+
+/* clusterDSB alone on a caller-supplied hit array (test hook) */
+void oracle_cluster(const dna_dna_target *tgt, const uint8_t *hits, dna_score_out *out)
+{
+    memset(out, 0, sizeof(*out));
+    cluster(tgt, hits, out);
+}

@@ -1,4 +1,753 @@
+// irt.hip — Phase C, independent-reaction-time chemistry (public/irt-worker.js:370-699)
+// for many tracks (primaries) and many instances (worker invocations) in one launch.
+//
+// Pipeline (all on the device; the host only sizes buffers from small per-primary
+// count arrays):
+//   decode   w -> (species code, primary id) per radical entry (irt-worker.js:382-394);
+//            code 7 entries count initial H2, code 4 entries are skipped
+//   group    entries of one (instance, primary) become one track, in input order
+//            (the worker's per-pid list, irt-worker.js:391-393)
+//   k_irt    one workgroup per track:
+//            1. product displacement of the initial species (irt-worker.js:475-537)
+//            2. all initial pairs i<j, enumerated circularly so every lane gets the same
+//               count: pair (i, i+d mod n) for d = 1..(n-1)/2 (+ n/2 for i < n/2, n even);
+//               reaction-time sampling in FP64 (irt-worker.js:541-558, 116-184)
+//            3. event loop: the earliest event whose two reactants are alive is found by
+//               a workgroup min-reduction over the event list, ordered by (t, a, b); the
+//               reaction, its products and checkpoint snapshots are applied by one lane;
+//               each product is re-paired with every alive particle by the whole
+//               workgroup (irt-worker.js:566-645, 430-465)
+//            4. per-instance tallies with 64-bit atomics
+// Random numbers come from the counter-based streams of include/dna_irt_math.h, so the
+// result is independent of thread scheduling and identical to the CPU restatement.
 #include "dna_internal.h"
 #include "dna_ctx.h"
-extern "C" int dna_run_irt(dna_ctx *, const float *, uint64_t, uint32_t, float, const double *, int, uint64_t, dna_chem_out *) { dna::set_error("not built"); return DNA_ERR_ARG; }
-extern "C" int dna_run_irt_batch(dna_ctx *, const float *, const uint64_t *, uint32_t, const uint32_t *, const float *, const double *, int, uint64_t, dna_chem_out *) { dna::set_error("not built"); return DNA_ERR_ARG; }
+#include "../../include/dna_irt_math.h"
+#include <vector>
+#include <algorithm>
+#include <cstring>
+
+namespace dna {
+
+struct IrtConst {
+    double sigma[9], sigma_eff[9], rc[9], kdif[9], kact[9], prob[9], kobs[9];
+    double D[6], sqrtD[6];
+    double R_CUT2, sig_oh, sig_h, sig_eaq;
+    int rxnMap[36];
+    int type[9], nprod[9], prods[9][2];
+};
+__constant__ IrtConst c_irt;
+
+struct IrtTrack {
+    uint32_t inst, pid, start, n;     // bucket range in idx[]
+    uint32_t h2;                      // initial H2 markers
+    uint32_t cap;                     // particle capacity of this track (products included)
+    uint32_t cap_global;              // the worker's CAP = 2*maxN + 512 of the instance
+    uint32_t ev_cap;
+    uint64_t part_off, ev_off;
+};
+
+struct IrtArgs {
+    const float4 *rad;
+    const uint32_t *idx;          // bucketed entry indices
+    const IrtTrack *tracks;
+    float4 *pos;                  // (x, y, z, species) per particle slot
+    uint8_t *alive;
+    double *tb;
+    double *ev_t;
+    uint2 *ev_ab;
+    unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
+    uint32_t *overflow;
+    double tc[DNA_MAX_CKPT];
+    int nc;
+    uint64_t seed;
+    int sort_tracks;              // 1: bucket order is not input order -> sort per track
+};
+#define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 2)
+
+// ------------------------------------------------------------ samplers (FP64)
+__device__ double sample_pdc(double a, double b, uint64_t ps)
+{
+    double p = 2.0 * sqrt(2.0 * b / a);
+    double q = 2.0 / sqrt(2.0 * b / a);
+    double M1 = 1.0 / (a * a), M2 = 3.0 * b / a;
+    double M = M1 > M2 ? M1 : M2;
+    for (int trial = 0; trial < 10000; trial++) {
+        double U = dna_u01(ps, 7u + 2u * (uint32_t)trial);
+        double X, v;
+        if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
+        else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
+        U = dna_u01(ps, 8u + 2u * (uint32_t)trial);
+        double sqX = sqrt(X);
+        double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
+        if ((X <= 2.0 * b / a && U <= lam) || (X >= 2.0 * b / a && U * M / X <= lam)) return X;
+    }
+    return -1.0;
+}
+
+__device__ double sample_irt(double r0, int ri, double D, uint64_t ps)
+{
+    const IrtConst &C = c_irt;
+    if (C.type[ri] == 0) {
+        double sigma = C.sigma[ri], rc = C.rc[ri];
+        if (sigma <= 0 || D <= 0) return -1;
+        if (r0 <= sigma) return 0;
+        double r0e = r0;
+        if (rc != 0) r0e = -rc / (1 - dna_exp(rc / r0));
+        if (r0e <= sigma) return 0;
+        double Winf = sigma / r0e;
+        double U = dna_u01(ps, 6u);
+        if (U <= 0 || U >= Winf) return -1;
+        double ei = dna_erfcinv(r0e * U / sigma);
+        if (fabs(ei) < 1e-10) return -1;
+        double dr = r0e - sigma;
+        return 0.25 * dr * dr / (D * ei * ei);
+    }
+    double sc = C.sigma[ri], rc = C.rc[ri], kobs = C.kobs[ri], kdif = C.kdif[ri], kact = C.kact[ri];
+    double sig_chk = sc, r0_chk = r0;
+    if (rc != 0) { r0_chk = -rc / (1 - dna_exp(rc / r0)); sig_chk = C.sigma_eff[ri]; }
+    if (sig_chk > r0_chk) return (C.prob[ri] > dna_u01(ps, 6u)) ? 0.0 : -1.0;
+    double Winf = sig_chk / r0_chk * kobs / kdif;
+    if (!(Winf > dna_u01(ps, 6u))) return -1;
+    double a, b;
+    if (rc == 0) { a = (1.0 / sc) * kact / kobs; b = (r0 - sc) / 2; }
+    else {
+        double v = kact / (4 * DNA_PI * sc * sc * dna_exp(-rc / sc));
+        double alpha = v + rc * D / (sc * sc * (1 - dna_exp(-rc / sc)));
+        double sh, ch; dna_sinhcosh(rc / (2 * sc), &sh, &ch);
+        a = 4 * sc * sc * alpha / (D * rc * rc) * sh * sh;
+        double sh0, ch0; dna_sinhcosh(rc / (2 * r0), &sh0, &ch0);
+        b = rc / 4 * (ch0 / sh0 - ch / sh);
+    }
+    double X = sample_pdc(a, b, ps);
+    if (X > 0) return X / D;
+    return -1;
+}
+
+// ------------------------------------------------------------ grouping kernels
+__device__ __forceinline__ bool decode_w(float wf, uint32_t &pid, int &sp)
+{
+    double w = floor((double)wf + 0.5);
+    double spd = fmod(w, 8.0);
+    if (spd < 0) return false;
+    sp = (int)spd;
+    double pd = floor(w / 8.0);
+    if (pd >= 4294967295.0) return false;
+    pid = (uint32_t)pd;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t find_inst(const uint64_t *off, uint32_t n_inst, uint64_t i)
+{
+    uint32_t lo = 0, hi = n_inst;       // off[lo] <= i < off[hi]
+    while (hi - lo > 1) { uint32_t m = (lo + hi) >> 1; if (off[m] <= i) lo = m; else hi = m; }
+    return lo;
+}
+
+__global__ void k_pidmax(const float4 *rad, uint64_t n, uint32_t *pidmax)
+{
+    uint32_t m = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t pid; int sp;
+        if (decode_w(rad[i].w, pid, sp) && pid > m) m = pid;
+    }
+    for (int o = 32; o > 0; o >>= 1) { uint32_t t = __shfl_xor(m, o, 64); m = t > m ? t : m; }
+    if ((threadIdx.x & 63) == 0) atomicMax(pidmax, m);
+}
+
+// kept[i] = 1 for a particle entry; key[i] = inst*(pidmax+1) + pid; counts and markers
+__global__ void k_count(const float4 *rad, uint64_t n, const uint64_t *off, uint32_t n_inst, uint32_t K1,
+                        uint32_t *kept, uint32_t *key, uint32_t *cnt, uint32_t *h2)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t pid; int sp;
+    uint32_t kk = 0, kf = 0;
+    if (decode_w(rad[i].w, pid, sp) && sp != 4) {
+        uint32_t inst = find_inst(off, n_inst, i);
+        kk = inst * K1 + pid;
+        if (sp == 7) atomicAdd(&h2[kk], 1u);
+        else { kf = 1; atomicAdd(&cnt[kk], 1u); }
+    }
+    kept[i] = kf;
+    key[i] = kk;
+}
+
+__global__ void k_first(const uint32_t *kept, const uint32_t *S, const uint32_t *key, uint64_t n,
+                        uint32_t *ck, uint32_t *firstj)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !kept[i]) return;
+    uint32_t j = S[i];
+    ck[j] = key[i];
+    atomicMin(&firstj[key[i]], j);
+}
+
+__global__ void k_changes(const uint32_t *ck, uint32_t m, uint32_t *changes)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t c = (j > 0 && j < m && ck[j] != ck[j - 1]) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(changes, c);
+}
+
+__global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t *key, uint64_t n,
+                         const uint32_t *bstart, const uint32_t *firstj, uint32_t *cursor, int contiguous,
+                         uint32_t *idx)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n || !kept[i]) return;
+    uint32_t k = key[i];
+    uint32_t r = contiguous ? S[i] - firstj[k] : atomicAdd(&cursor[k], 1u);
+    idx[bstart[k] + r] = (uint32_t)i;
+}
+
+// ------------------------------------------------------------ per-track kernel
+#define IRT_BLOCK 256
+#define SORT_MAX 8192
+
+__device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint64_t ab2)
+{
+    return t1 < t2 || (t1 == t2 && ab1 < ab2);
+}
+
+struct TrackShared {
+    uint32_t n_ev;
+    uint32_t n_total;
+    double   win_t;
+    uint64_t win_ab;
+    int      np;
+    uint32_t prod_idx[2];
+    double   wt[IRT_BLOCK / 64];
+    uint64_t wab[IRT_BLOCK / 64];
+    unsigned long long cnt[5];   // running OH, e-aq, H, H2O2, H2
+    unsigned long long rxc[9];   // reaction counts
+    unsigned long long n_re;
+};
+
+// pair particle `idx` (a new product born at t_new) with all alive particles < n_total
+__device__ void pair_with_alive(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t idx,
+                                uint32_t n_total, double t_new, TrackShared &sh)
+{
+    const IrtConst &C = c_irt;
+    const float4 *P = A.pos + T.part_off;
+    const uint8_t *al = A.alive + T.part_off;
+    const double *tb = A.tb + T.part_off;
+    float4 pi = P[idx];
+    int si = (int)pi.w;
+    double Di = C.D[si];
+    for (uint32_t jj = threadIdx.x; jj < n_total; jj += IRT_BLOCK) {
+        if (jj == idx || !al[jj]) continue;
+        float4 pj = P[jj];
+        int sj = (int)pj.w;
+        int ri = C.rxnMap[si * 6 + sj];
+        if (ri < 0) continue;
+        double Dj = C.D[sj];
+        uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
+        uint64_t ps = dna_pair_state(tk, a, b);
+        double dt = t_new - tb[jj];
+        double xj = pj.x, yj = pj.y, zj = pj.z;
+        if (dt > 0) {
+            double sig = sqrt(2.0 * Dj * dt);
+            double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
+            double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
+            double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
+            xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
+        }
+        double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
+        double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (r2 > C.R_CUT2) continue;
+        double t = sample_irt(sqrt(r2), ri, Di + Dj, ps);
+        if (t >= 0) {
+            double ta = t_new + t;
+            if (ta < DNA_T_END_NS) {
+                uint32_t e = atomicAdd(&sh.n_ev, 1u);
+                if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = ta; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
+                else atomicOr(A.overflow, 2u);
+            }
+        }
+    }
+}
+
+__device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh, int c,
+                         unsigned long long *loc)
+{
+    for (int s = 0; s < 5; s++) loc[c * 5 + s] += sh.cnt[s];
+}
+
+template <bool SORT>
+__global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
+{
+    const IrtConst &C = c_irt;
+    __shared__ TrackShared sh;
+    __shared__ uint32_t s_sort[SORT ? SORT_MAX : 1];
+    __shared__ unsigned long long s_loc[DNA_MAX_CKPT * 5];
+    const IrtTrack T = A.tracks[blockIdx.x];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = T.n;
+    const uint32_t *idx = A.idx + T.start;
+    float4 *P = A.pos + T.part_off;
+    uint8_t *al = A.alive + T.part_off;
+    double *tb = A.tb + T.part_off;
+    const uint64_t tk = dna_track_key(A.seed, T.inst, T.pid);
+
+    // input order of this track's entries (bucket order is already input order unless
+    // the input interleaved primaries; then sort the <= SORT_MAX indices)
+    const uint32_t *ord = idx;
+    if (SORT) {
+        uint32_t np2 = 1; while (np2 < n) np2 <<= 1;
+        for (uint32_t i = tid; i < np2; i += IRT_BLOCK) s_sort[i] = i < n ? idx[i] : 0xFFFFFFFFu;
+        __syncthreads();
+        for (uint32_t k = 2; k <= np2; k <<= 1)
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < np2; i += IRT_BLOCK) {
+                    uint32_t l = i ^ j;
+                    if (l > i) {
+                        uint32_t x = s_sort[i], y = s_sort[l];
+                        bool up = (i & k) == 0;
+                        if ((x > y) == up) { s_sort[i] = y; s_sort[l] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        ord = s_sort;
+    }
+    if (tid == 0) { sh.n_ev = 0; sh.n_total = n; sh.n_re = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
+    for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_BLOCK) s_loc[i] = 0;
+    __syncthreads();
+
+    // 1. load + displacement (irt-worker.js:475-537)
+    unsigned long long c0 = 0, c1 = 0, c2 = 0;
+    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+        float4 e = A.rad[ord[k]];
+        double x = e.x, y = e.y, z = e.z;
+        int s = (int)fmod(floor((double)e.w + 0.5), 8.0);
+        uint64_t ps = dna_pair_state(tk, k, k);
+        uint32_t dn = 0;
+        double sig = 0.0;
+        int spc; uint8_t alv = 1;
+        if (s == 5) spc = 1;
+        else if (s == 0 || s == 3) { spc = s; if (dna_u01(ps, dn++) < 0.5) sig = C.sig_oh; }
+        else if (s == 1) { spc = 1; sig = C.sig_eaq; }
+        else if (s == 2) { spc = 2; sig = C.sig_h; }
+        else if (s == 6) spc = 5;
+        else { spc = s; alv = 0; }
+        if (sig != 0.0) {
+            double v;
+            v = dna_u01(ps, dn++); for (int q = 1; q < 6; q++) v += dna_u01(ps, dn++);
+            x += sig * ((v - 3) * 1.4142135623730951);
+            v = dna_u01(ps, dn++); for (int q = 1; q < 6; q++) v += dna_u01(ps, dn++);
+            y += sig * ((v - 3) * 1.4142135623730951);
+            v = dna_u01(ps, dn++); for (int q = 1; q < 6; q++) v += dna_u01(ps, dn++);
+            z += sig * ((v - 3) * 1.4142135623730951);
+        }
+        P[k] = make_float4((float)x, (float)y, (float)z, (float)spc);
+        al[k] = alv;
+        tb[k] = 0.0;
+        if (alv) { if (spc == 0) c0++; else if (spc == 1) c1++; else if (spc == 2) c2++; }
+    }
+    for (int o = 32; o > 0; o >>= 1) { c0 += __shfl_xor(c0, o, 64); c1 += __shfl_xor(c1, o, 64); c2 += __shfl_xor(c2, o, 64); }
+    if ((tid & 63) == 0) { atomicAdd(&sh.cnt[0], c0); atomicAdd(&sh.cnt[1], c1); atomicAdd(&sh.cnt[2], c2); }
+    if (tid == 0) sh.cnt[4] = T.h2;
+    __syncthreads();
+
+    // 2. initial pairs (irt-worker.js:541-558), circular enumeration
+    if (n > 1) {
+        uint32_t h = (n - 1) / 2;
+        bool even = (n & 1u) == 0;
+        for (uint32_t i = tid; i < n; i += IRT_BLOCK) {
+            if (!al[i]) continue;
+            float4 pi = P[i];
+            int si = (int)pi.w;
+            double Di = C.D[si];
+            uint32_t dmax = h + ((even && i < n / 2) ? 1u : 0u);
+            uint32_t j = i;
+            for (uint32_t d = 1; d <= dmax; d++) {
+                j = j + 1 == n ? 0 : j + 1;
+                if (!al[j]) continue;
+                float4 pj = P[j];
+                int sj = (int)pj.w;
+                int ri = C.rxnMap[si * 6 + sj];
+                if (ri < 0) continue;
+                double ddx = (double)pi.x - (double)pj.x;
+                double ddy = (double)pi.y - (double)pj.y;
+                double ddz = (double)pi.z - (double)pj.z;
+                double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (r2 > C.R_CUT2) continue;
+                uint32_t a = i < j ? i : j, b = i < j ? j : i;
+                double t = sample_irt(sqrt(r2), ri, Di + C.D[sj], dna_pair_state(tk, a, b));
+                if (t >= 0 && t < DNA_T_END_NS) {
+                    uint32_t e = atomicAdd(&sh.n_ev, 1u);
+                    if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
+                    else atomicOr(A.overflow, 2u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long n_init_ev = sh.n_ev;
+    int cp = 0;
+
+    // 3. event loop
+    for (;;) {
+        uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
+        double bt = 1e300; uint64_t bab = ~0ull;
+        for (uint32_t e = tid; e < ne; e += IRT_BLOCK) {
+            uint2 ab = A.ev_ab[T.ev_off + e];
+            if (!al[ab.x] || !al[ab.y]) continue;
+            double t = A.ev_t[T.ev_off + e];
+            uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
+            if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            double ot = __shfl_xor(bt, o, 64);
+            uint64_t oab = __shfl_xor(bab, o, 64);
+            if (ev_less(ot, oab, bt, bab)) { bt = ot; bab = oab; }
+        }
+        if ((tid & 63) == 0) { sh.wt[tid >> 6] = bt; sh.wab[tid >> 6] = bab; }
+        __syncthreads();
+        if (tid == 0) {
+            double t0 = sh.wt[0]; uint64_t ab0 = sh.wab[0];
+            for (int w = 1; w < IRT_BLOCK / 64; w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
+            sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
+            if (t0 < 1e300) {
+                while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
+                uint32_t a = (uint32_t)(ab0 >> 32), b = (uint32_t)ab0;
+                int si = (int)P[a].w, sj = (int)P[b].w;
+                int ri = C.rxnMap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
+                al[a] = 0; al[b] = 0;
+                sh.n_re++; sh.rxc[ri]++;
+                if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
+                if (sj == 0) sh.cnt[0]--; else if (sj == 1) sh.cnt[1]--; else if (sj == 2) sh.cnt[2]--; else if (sj == 4) sh.cnt[3]--;
+                if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
+                int np = C.nprod[ri];
+                if (np > 0) {
+                    float4 pa = P[a], pb = P[b];
+                    double sqDi = C.sqrtD[si], sqDj = C.sqrtD[sj];
+                    double wsum = sqDi + sqDj;
+                    double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
+                    double my = (sqDj * pa.y + sqDi * pb.y) / wsum;
+                    double mz = (sqDj * pa.z + sqDi * pb.z) / wsum;
+                    uint32_t nt = sh.n_total;
+                    int made = 0;
+                    for (int pp = 0; pp < np; pp++) {
+                        uint32_t id = nt;
+                        if (id >= T.cap_global) break;
+                        if (id >= T.cap) { atomicOr(A.overflow, 1u); break; }
+                        int ps = C.prods[ri][pp];
+                        float4 q;
+                        if (np == 1) q = make_float4((float)mx, (float)my, (float)mz, (float)ps);
+                        else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
+                        else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
+                        P[id] = q; al[id] = 1; tb[id] = t0;
+                        if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
+                        sh.prod_idx[made++] = id;
+                        nt++;
+                    }
+                    sh.np = made;
+                    sh.n_total = nt;
+                }
+            }
+        }
+        __syncthreads();
+        if (!(sh.win_t < 1e300)) break;
+        int np = sh.np;
+        uint32_t base_total = sh.n_total - (uint32_t)np;
+        for (int pp = 0; pp < np; pp++) {
+            // product pp is paired with everything alive up to and including itself
+            pair_with_alive(A, T, tk, sh.prod_idx[pp], base_total + (uint32_t)pp + 1u, sh.win_t, sh);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        while (cp < A.nc) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
+        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE;
+        for (int i = 0; i < A.nc * 5; i++) if (s_loc[i]) atomicAdd(&g[i], s_loc[i]);
+        for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
+        atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
+        atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
+    }
+}
+
+// ------------------------------------------------------------ host side
+static bool g_const_ready[64] = {false};
+
+static void build_const(IrtConst &C)
+{
+    static const double IRT_D[6] = {2.2, 4.9, 7.0, 9.46, 2.3, 5.3};
+    static const double VDW_R[6] = {0.22, 0.50, 0.19, 0.25, 0.21, 0.33};
+    static const double CHARGE[6] = {0, -1, 0, 1, 0, -1};
+    static const int RA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, RB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+    static const double RK[9] = {0.55e10, 2.95e10, 1.55e10, 0.636e10, 2.50e10, 2.11e10, 0.503e10, 1.10e10, 1.13e11};
+    static const int TY[9] = {1, 1, 1, 0, 0, 1, 0, 1, 0};
+    static const int NP[9] = {1, 1, 0, 2, 1, 1, 0, 2, 0};
+    static const int PR[9][2] = {{4, 0}, {5, 0}, {0, 0}, {5, 5}, {5, 0}, {2, 0}, {0, 0}, {5, 0}, {0, 0}};
+    const double ONSAGER = 1.44 / (0.02527 * 80.1), Rs = 0.29, K_CONV = 1e24 / (6.022e23 * 1e9);
+    for (int s = 0; s < 6; s++) { C.D[s] = IRT_D[s]; C.sqrtD[s] = sqrt(IRT_D[s]); }
+    for (int r = 0; r < 9; r++) {
+        int a = RA[r], b = RB[r];
+        double D_sum = IRT_D[a] + IRT_D[b];
+        bool self = a == b;
+        double rc = CHARGE[a] * CHARGE[b] * ONSAGER;
+        double k_nm = RK[r] * K_CONV;
+        C.rc[r] = rc; C.kobs[r] = k_nm; C.type[r] = TY[r]; C.nprod[r] = NP[r];
+        C.prods[r][0] = PR[r][0]; C.prods[r][1] = PR[r][1];
+        if (TY[r] == 0) {
+            double sig = k_nm / (4 * DNA_PI * D_sum);
+            if (self) sig *= 2;
+            C.sigma[r] = sig; C.sigma_eff[r] = sig; C.kdif[r] = 0; C.kact[r] = 0; C.prob[r] = 1;
+        } else {
+            double sc = VDW_R[a] + VDW_R[b];
+            C.sigma[r] = sc;
+            double se, kd;
+            if (rc == 0) { se = sc; kd = 4 * DNA_PI * D_sum * sc; }
+            else { se = -rc / (1 - dna_exp(rc / sc)); kd = 4 * DNA_PI * D_sum * se; }
+            if (self) kd /= 2;
+            C.sigma_eff[r] = se; C.kdif[r] = kd;
+            double ka = kd * k_nm / (kd - k_nm);
+            C.kact[r] = ka;
+            C.prob[r] = Rs / (Rs + (kd / ka) * (se + Rs));
+        }
+    }
+    for (int i = 0; i < 36; i++) C.rxnMap[i] = -1;
+    for (int r = 0; r < 9; r++) { C.rxnMap[RA[r] * 6 + RB[r]] = r; C.rxnMap[RB[r] * 6 + RA[r]] = r; }
+    double R_CUT = 1.45 + 2 * sqrt(8 * 9.46 * 1000);
+    C.R_CUT2 = R_CUT * R_CUT;
+    static const double g[13] = {-4.06217193e-08, 3.06848412e-06, -9.93217814e-05, 1.80172797e-03,
+        -2.01135480e-02, 1.42939448e-01, -6.48348714e-01, 1.85227848e+00, -3.36450378e+00,
+        4.37785068e+00, -4.20557339e+00, 3.81679083e+00, -2.34069784e-01};
+    double rm = 0, pw[13];
+    pw[0] = 1.0; for (int i = 1; i < 13; i++) pw[i] = pw[i - 1] * 1.7;
+    for (int i = 12; i >= 0; i--) rm += g[12 - i] * pw[i];
+    C.sig_eaq = rm * 0.62665706865775006;
+    C.sig_oh = 0.8 / sqrt(3.0);
+    C.sig_h = (17.0 / 18.0) * 2.4 / sqrt(3.0);
+}
+
+template <class T> static hipError_t dgrow(void *&ptr, size_t &cap, size_t need_elems)
+{
+    size_t need = need_elems * sizeof(T);
+    if (need <= cap && ptr) return hipSuccess;
+    if (ptr) { hipError_t e = hipFree(ptr); if (e != hipSuccess) return e; ptr = nullptr; }
+    size_t n = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&ptr, n);
+    cap = (e == hipSuccess) ? n : 0;
+    return e;
+}
+
+}  // namespace dna
+
+using namespace dna;
+
+// Scratch of the grouping stage, carved out of one allocation.
+struct GroupScratch {
+    uint32_t *kept, *key, *S, *ck, *cnt, *h2, *firstj, *bstart, *cursor, *idx, *misc;
+};
+
+static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
+                          const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
+                          dna_chem_out *outs)
+{
+    hipStream_t st = c->stream;
+    uint32_t n_inst = (uint32_t)off.size() - 1;
+    if (!g_const_ready[c->device]) {
+        IrtConst C; build_const(C);
+        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_irt), &C, sizeof(C)));
+        g_const_ready[c->device] = true;
+    }
+    for (uint32_t i = 0; i < n_inst; i++) {
+        memset(&outs[i], 0, sizeof(dna_chem_out));
+        outs[i].n_ckpt = nc;
+        for (int k = 0; k < nc; k++) outs[i].t_ckpt[k] = t_ckpt[k];
+        outs[i].chem_n = off[i + 1] - off[i];
+    }
+    if (rad_n == 0) return DNA_OK;
+    if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_run_irt: too many radical entries"); return DNA_ERR_CAPACITY; }
+    DNA_HIP(hipEventRecord(c->ev[5], st));
+    // instance offsets + pid range
+    uint64_t *d_off = nullptr;
+    std::vector<char> dummy;
+    size_t misc_words = 64 + 2 * (n_inst + 1);
+    // stage 1 scratch: kept, key, S (rad_n + 1 each), ck (rad_n), misc
+    size_t need1 = 4 * ((size_t)rad_n + 1) + misc_words;
+    DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, need1));
+    uint32_t *base = (uint32_t *)c->irt.part;
+    GroupScratch g;
+    g.kept = base; g.key = g.kept + rad_n + 1; g.S = g.key + rad_n + 1; g.ck = g.S + rad_n + 1;
+    g.misc = g.ck + rad_n + 1;
+    d_off = (uint64_t *)(((uintptr_t)(g.misc + 16) + 7) & ~(uintptr_t)7);
+    DNA_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemsetAsync(g.misc, 0, 16 * 4, st));
+    const uint32_t B = 256;
+    uint32_t gb = (uint32_t)((rad_n + B - 1) / B);
+    hipLaunchKernelGGL(k_pidmax, dim3(gb < 4096 ? gb : 4096), dim3(B), 0, st, d_rad, rad_n, g.misc);
+    uint32_t h_misc[16];
+    DNA_HIP(hipMemcpyAsync(h_misc, g.misc, 16 * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    uint64_t K1 = (uint64_t)h_misc[0] + 1, K = K1 * n_inst;
+    if (K > (1ull << 28)) { set_error("dna_run_irt: primary-id range too large"); return DNA_ERR_CAPACITY; }
+    // stage 2 scratch: per-key arrays cnt, h2, firstj, bstart, cursor
+    DNA_HIP(dgrow<uint32_t>(c->irt.tracks, c->irt.cap_tracks, 5 * (K + 1)));
+    uint32_t *kb = (uint32_t *)c->irt.tracks;
+    g.cnt = kb; g.h2 = kb + (K + 1); g.firstj = kb + 2 * (K + 1); g.bstart = kb + 3 * (K + 1); g.cursor = kb + 4 * (K + 1);
+    DNA_HIP(hipMemsetAsync(g.cnt, 0, 2 * (K + 1) * 4, st));
+    DNA_HIP(hipMemsetAsync(g.firstj, 0xFF, (K + 1) * 4, st));
+    DNA_HIP(hipMemsetAsync(g.cursor, 0, (K + 1) * 4, st));
+    hipLaunchKernelGGL(k_count, dim3(gb), dim3(B), 0, st, d_rad, rad_n, d_off, n_inst, (uint32_t)K1,
+                       g.kept, g.key, g.cnt, g.h2);
+    DNA_HIP(hipGetLastError());
+    // exclusive scan of kept flags -> S (the scan helper scans in place)
+    DNA_HIP(hipMemcpyAsync(g.S, g.kept, rad_n * 4, hipMemcpyDeviceToDevice, st));
+    DNA_HIP(dgrow<uint32_t>(c->irt.scratch, c->irt.cap_scratch, rad_n / 1024 + 16));
+    DNA_HIP(exclusive_scan_u32(g.S, (uint32_t)rad_n, (uint32_t *)c->irt.scratch, st));
+    hipLaunchKernelGGL(k_first, dim3(gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, g.ck, g.firstj);
+    uint32_t m_kept = 0;
+    DNA_HIP(hipMemcpyAsync(&m_kept, g.S + rad_n, 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    DNA_HIP(hipMemsetAsync(g.misc + 1, 0, 4, st));
+    if (m_kept > 1)
+        hipLaunchKernelGGL(k_changes, dim3((m_kept + B - 1) / B), dim3(B), 0, st, g.ck, m_kept, g.misc + 1);
+    std::vector<uint32_t> cnt(K), h2(K);
+    DNA_HIP(hipMemcpyAsync(cnt.data(), g.cnt, K * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(h2.data(), g.h2, K * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(h_misc, g.misc, 16 * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    // tracks
+    std::vector<uint32_t> maxN(n_inst, 0);
+    std::vector<IrtTrack> tr;
+    std::vector<uint32_t> bstart(K + 1, 0);
+    uint32_t run = 0;
+    for (uint64_t k = 0; k < K; k++) {
+        bstart[k] = run;
+        if (cnt[k]) {
+            IrtTrack t; memset(&t, 0, sizeof(t));
+            t.inst = (uint32_t)(k / K1); t.pid = (uint32_t)(k % K1); t.start = run; t.n = cnt[k]; t.h2 = h2[k];
+            tr.push_back(t);
+            if (cnt[k] > maxN[t.inst]) maxN[t.inst] = cnt[k];
+            run += cnt[k];
+        }
+    }
+    bstart[K] = run;
+    uint32_t ntr = (uint32_t)tr.size();
+    bool contiguous = ntr == 0 || h_misc[1] == ntr - 1;
+    uint32_t maxn_all = 0;
+    for (auto &t : tr) maxn_all = t.n > maxn_all ? t.n : maxn_all;
+    if (!contiguous && maxn_all > SORT_MAX) {
+        set_error("dna_run_irt: radicals of a primary are interleaved with others and the primary has more than 8192 entries; group the input by primary");
+        return DNA_ERR_CAPACITY;
+    }
+    for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
+    for (auto &t : tr) outs[t.inst].n_tracks++;
+    if (ntr == 0) return DNA_OK;
+    int ev_mult = 8;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        uint64_t part = 0, evs = 0;
+        for (auto &t : tr) {
+            uint32_t capg = 2 * maxN[t.inst] + 512;
+            uint32_t capt = 2 * t.n + 512;
+            if (attempt > 0) capt = capg;           // after an overflow use the worker's CAP
+            t.cap_global = capg; t.cap = capt < capg ? capt : capg;
+            t.part_off = part; part += t.cap;
+            t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
+            t.ev_off = evs; evs += t.ev_cap;
+        }
+        size_t bytes_part = part * (16 + 1 + 8);
+        DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch,
+                            std::max<size_t>(bytes_part + 256, rad_n / 1024 * 4 + 64)));
+        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 16 + 256));
+        // layout of the track/bucket block inside irt.tally
+        size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
+        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, tally_bytes + ntr * sizeof(IrtTrack) + (K + 1) * 4 + (size_t)run * 4 + 1024));
+        char *tbase = (char *)c->irt.tally;
+        unsigned long long *d_tally = (unsigned long long *)tbase;
+        IrtTrack *d_tr = (IrtTrack *)(tbase + ((tally_bytes + 255) & ~(size_t)255));
+        uint32_t *d_bst = (uint32_t *)((char *)d_tr + ((ntr * sizeof(IrtTrack) + 255) & ~(size_t)255));
+        uint32_t *d_idx = d_bst + ((K + 1 + 63) & ~(uint64_t)63);
+        DNA_HIP(hipMemsetAsync(d_tally, 0, tally_bytes, st));
+        DNA_HIP(hipMemcpyAsync(d_tr, tr.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
+        DNA_HIP(hipMemcpyAsync(d_bst, bstart.data(), (K + 1) * 4, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_bucket, dim3(gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, d_bst, g.firstj, g.cursor,
+                           contiguous ? 1 : 0, d_idx);
+        DNA_HIP(hipMemsetAsync(g.misc + 2, 0, 4, st));
+        IrtArgs A;
+        char *sp = (char *)c->irt.scratch;
+        A.rad = d_rad; A.idx = d_idx; A.tracks = d_tr;
+        A.pos = (float4 *)sp; A.tb = (double *)(sp + part * 16); A.alive = (uint8_t *)(sp + part * 24);
+        A.ev_t = (double *)c->irt.events; A.ev_ab = (uint2 *)((char *)c->irt.events + evs * 8);
+        A.tally = d_tally; A.overflow = g.misc + 2;
+        for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
+        A.nc = nc; A.seed = seed; A.sort_tracks = contiguous ? 0 : 1;
+        if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(ntr), dim3(IRT_BLOCK), 0, st, A);
+        else hipLaunchKernelGGL(k_irt<true>, dim3(ntr), dim3(IRT_BLOCK), 0, st, A);
+        DNA_HIP(hipGetLastError());
+        uint32_t ovf = 0;
+        DNA_HIP(hipMemcpyAsync(&ovf, g.misc + 2, 4, hipMemcpyDeviceToHost, st));
+        std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE);
+        DNA_HIP(hipMemcpyAsync(ht.data(), d_tally, tally_bytes, hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipEventRecord(c->ev[6], st));
+        DNA_HIP(hipStreamSynchronize(st));
+        if (ovf & 2u) { ev_mult *= 4; continue; }
+        if (ovf & 1u) { continue; }
+        float ms = 0; hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+        for (uint32_t i = 0; i < n_inst; i++) {
+            const unsigned long long *g2 = &ht[(size_t)i * TALLY_STRIDE];
+            dna_chem_out &o = outs[i];
+            double per100 = (double)n_therm[i] * (double)E[i] / 100.0;
+            for (int k = 0; k < nc; k++) for (int s = 0; s < 5; s++) {
+                o.tally[k][s] = g2[k * 5 + s];
+                o.G[k][s] = per100 > 0 ? (double)o.tally[k][s] / per100 : 0.0;
+            }
+            for (int r = 0; r < 9; r++) o.rxn_counts[r] = g2[DNA_MAX_CKPT * 5 + r];
+            o.n_reacted = g2[DNA_MAX_CKPT * 5 + 9];
+            o.n_events = g2[DNA_MAX_CKPT * 5 + 10];
+            o.t_ms = ms;
+        }
+        return DNA_OK;
+    }
+    set_error("dna_run_irt: event/particle capacity kept overflowing");
+    return DNA_ERR_CAPACITY;
+}
+
+static int stage_rad(dna_ctx *c, const float *rad, uint64_t rad_n, const float4 **d_rad)
+{
+    if (rad) {
+        DNA_HIP(dgrow<float4>(c->irt.rad_in, c->irt.cap_rad_in, rad_n + 1));
+        DNA_HIP(hipMemcpyAsync(c->irt.rad_in, rad, rad_n * 16, hipMemcpyHostToDevice, c->stream));
+        *d_rad = (const float4 *)c->irt.rad_in;
+    } else {
+        if (rad_n > c->rad_n) { set_error("dna_run_irt: rad == NULL but fewer radicals are resident"); return DNA_ERR_ARG; }
+        *d_rad = c->d_rad;
+    }
+    return DNA_OK;
+}
+
+extern "C" int dna_run_irt(dna_ctx *c, const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                           const double *t_ckpt, int n_ckpt, uint64_t seed, dna_chem_out *out)
+{
+    if (!c || !out || !t_ckpt || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) { set_error("dna_run_irt: bad arguments"); return DNA_ERR_ARG; }
+    DNA_HIP(hipSetDevice(c->device));
+    // the host buffer (if any) is staged into irt.rad_in; the bucket/track block lives in irt.tally
+    const float4 *d_rad = nullptr;
+    int rc = stage_rad(c, rad, rad_n, &d_rad);
+    if (rc) return rc;
+    std::vector<uint64_t> off = {0, rad_n};
+    return run_irt_device(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, seed, out);
+}
+
+extern "C" int dna_run_irt_batch(dna_ctx *c, const float *rad, const uint64_t *inst_off, uint32_t n_inst,
+                                 const uint32_t *n_therm, const float *E_eV, const double *t_ckpt, int n_ckpt,
+                                 uint64_t seed, dna_chem_out *out)
+{
+    if (!c || !out || !rad || !inst_off || !n_therm || !E_eV || !t_ckpt || n_inst == 0 || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) {
+        set_error("dna_run_irt_batch: bad arguments"); return DNA_ERR_ARG;
+    }
+    std::vector<uint64_t> off(inst_off, inst_off + n_inst + 1);
+    for (uint32_t i = 0; i < n_inst; i++) if (off[i + 1] < off[i]) { set_error("dna_run_irt_batch: offsets must be non-decreasing"); return DNA_ERR_ARG; }
+    DNA_HIP(hipSetDevice(c->device));
+    uint64_t rad_n = off[n_inst] - off[0];
+    for (auto &o : off) o -= inst_off[0];
+    const float4 *d_rad = nullptr;
+    int rc = stage_rad(c, rad + 4 * inst_off[0], rad_n, &d_rad);
+    if (rc) return rc;
+    return run_irt_device(c, d_rad, rad_n, off, n_therm, E_eV, t_ckpt, n_ckpt, seed, out);
+}

@@ -1,4 +1,375 @@
+// score.hip — SSB/DSB scoring (src/scoring/ssb-dsb.ts) and spatial-hash binning
+// (src/shaders/chemistry.wgsl:66-80,110-122) on gfx950.
+//
+// scoreDirectSSB_events (ssb-dsb.ts:135-217) is sequential in two ways; both are made
+// parallel without changing the result:
+//  * adjacent-duplicate skipping: inside a maximal run of equal (x,y,z) entries starting
+//    at s, the loop visits exactly s, s+2, s+4, ...; each entry finds its run start by
+//    walking back, so visited(i) = (i - s) even;
+//  * the LCG (makeSsbRng, ssb-dsb.ts:421-427) is consumed once per in-reach event in
+//    index order: an exclusive scan gives each in-reach event its rank r, and the draw
+//    is state_{r+1} = A^(r+1) s0 + C(r+1) mod 2^32, composed from a table of the
+//    affine maps for 2^k steps.
+// Geometry uses doubles on the f32 inputs with Math.round = floor(x + 0.5), as JS does.
+// Hits are a bitmap; a hit counts when atomicOr finds its bit clear (the number of
+// distinct bits, independent of order).  clusterDSB (ssb-dsb.ts:348-407) runs one
+// workgroup per fiber.
 #include "dna_internal.h"
 #include "dna_ctx.h"
-extern "C" int dna_score(dna_ctx *, const dna_dna_target *, const float *, uint64_t, uint32_t *, int, dna_score_out *) { dna::set_error("not built"); return DNA_ERR_ARG; }
-extern "C" int dna_hash_cells(dna_ctx *, const float *, uint64_t, float, uint32_t, uint32_t *, uint32_t *, uint32_t *) { dna::set_error("not built"); return DNA_ERR_ARG; }
+#include <vector>
+#include <cmath>
+#include <cstring>
+
+namespace dna {
+
+struct ScoreGeom {
+    double x_half, rise_inv, grid_off, inv_spacing, r_bb, x0, rise;
+    uint32_t grid_N, n_bp_per, n_bp, n_fibers;
+};
+
+__constant__ uint32_t c_lcg_a[32], c_lcg_c[32];   // affine map of 2^k LCG steps
+
+__device__ __forceinline__ double js_round(double x) { return floor(x + 0.5); }
+
+__device__ __forceinline__ bool same_xyz(float4 a, float4 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// per entry: 1 = candidate, 2 = in reach (then *hit = bit index)
+__device__ int classify(const ScoreGeom &G, const float *fy, const float *fz, const float *rbb0,
+                        const float *rbb1, float4 e, uint64_t *hit)
+{
+    const double r_direct = 0.29, r2d = r_direct * r_direct;
+    double x = e.x, y = e.y, z = e.z;
+    if (x < -G.x_half - r_direct || x > G.x_half + r_direct) return 0;
+    double fi = js_round((y - G.grid_off) * G.inv_spacing);
+    double fj = js_round((z - G.grid_off) * G.inv_spacing);
+    if (fi < 0 || fi >= G.grid_N || fj < 0 || fj >= G.grid_N) return 0;
+    int64_t fiber = (int64_t)fi * G.grid_N + (int64_t)fj;
+    double y_rel = y - fy[fiber], z_rel = z - fz[fiber];
+    double r2 = y_rel * y_rel + z_rel * z_rel;
+    double outer = G.r_bb + r_direct;
+    if (r2 > outer * outer) return 0;
+    double bp_est = js_round((x + G.x_half) * G.rise_inv);
+    int64_t b0 = (int64_t)(bp_est - 2 > 0 ? bp_est - 2 : 0);
+    int64_t b1 = (int64_t)(bp_est + 2 < G.n_bp_per - 1 ? bp_est + 2 : G.n_bp_per - 1);
+    double best = 1e300; int64_t best_bp = -1; int best_strand = -1;
+    for (int64_t b = b0; b <= b1; b++) {
+        double dx = x - (G.x0 + b * G.rise);
+        double dy0 = y_rel - rbb0[2 * b], dz0 = z_rel - rbb0[2 * b + 1];
+        double d20 = dx * dx + dy0 * dy0 + dz0 * dz0;
+        if (d20 < best) { best = d20; best_bp = b; best_strand = 0; }
+        double dy1 = y_rel - rbb1[2 * b], dz1 = z_rel - rbb1[2 * b + 1];
+        double d21 = dx * dx + dy1 * dy1 + dz1 * dz1;
+        if (d21 < best) { best = d21; best_bp = b; best_strand = 1; }
+    }
+    if (!(best < r2d)) return 1;
+    *hit = (uint64_t)(fiber * G.n_bp_per + best_bp) + (uint64_t)best_strand * G.n_bp;
+    return 2;
+}
+
+// pass 1: visited/classify; flag[i] = 1 if an in-reach event; hitidx[i] = bit; counts
+__global__ void k_ssb_classify(const float4 *rad, uint64_t n, ScoreGeom G, const float *fy, const float *fz,
+                               const float *rbb0, const float *rbb1, uint32_t *flag, uint64_t *hitidx,
+                               unsigned long long *cnt /* [0] candidates, [1] in reach */)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    unsigned long long cand = 0, reach = 0;
+    if (i < n) {
+        float4 e = rad[i];
+        uint64_t s = i;
+        while (s > 0 && same_xyz(rad[s - 1], rad[s])) s--;
+        uint32_t f = 0;
+        if (((i - s) & 1ull) == 0) {
+            uint64_t h = 0;
+            int c = classify(G, fy, fz, rbb0, rbb1, e, &h);
+            if (c >= 1) cand = 1;
+            if (c == 2) { reach = 1; f = 1; hitidx[i] = h; }
+        }
+        flag[i] = f;
+    }
+    for (int o = 32; o > 0; o >>= 1) { cand += __shfl_xor(cand, o, 64); reach += __shfl_xor(reach, o, 64); }
+    if ((threadIdx.x & 63) == 0) {
+        if (cand) atomicAdd(&cnt[0], cand);
+        if (reach) atomicAdd(&cnt[1], reach);
+    }
+}
+
+__device__ __forceinline__ uint32_t lcg_jump(uint32_t s0, uint32_t m)
+{
+    // apply m LCG steps: compose the 2^k maps for the set bits of m
+    uint32_t s = s0;
+    for (int k = 0; k < 32; k++) if ((m >> k) & 1u) s = c_lcg_a[k] * s + c_lcg_c[k];
+    return s;
+}
+
+// pass 2: draws for in-reach events (rank from the exclusive scan of flag)
+__global__ void k_ssb_hits(const uint32_t *flag, const uint32_t *rank, const uint64_t *hitidx, uint64_t n,
+                           uint32_t s0, uint32_t *bits, unsigned long long *cnt /* [2] ssb */)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    unsigned long long add = 0;
+    if (i < n && flag[i]) {
+        uint32_t s = lcg_jump(s0, rank[i] + 1u);
+        double u = (double)s / 4294967296.0;
+        if (u < 0.15) {
+            uint64_t h = hitidx[i];
+            uint32_t bit = 1u << (h & 31u);
+            uint32_t old = atomicOr(&bits[h >> 5], bit);
+            if (!(old & bit)) add = 1;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
+    if ((threadIdx.x & 63) == 0 && add) atomicAdd(&cnt[2], add);
+}
+
+__device__ __forceinline__ bool getbit(const uint32_t *bits, uint64_t i) { return (bits[i >> 5] >> (i & 31u)) & 1u; }
+
+// clusterDSB per fiber: strand hit lists in bp order, greedy pairing within +-10 bp
+#define CL_BLOCK 256
+#define CL_MAXK 4096
+__global__ void __launch_bounds__(CL_BLOCK) k_cluster(const uint32_t *bits, ScoreGeom G, uint32_t *fiber_k,
+                                                      unsigned long long *cnt /* [3] dsb, [4] ssb0, [5] ssb1 */,
+                                                      uint32_t *overflow)
+{
+    __shared__ uint32_t s0[CL_MAXK], s1[CL_MAXK];
+    __shared__ uint32_t c0w[CL_BLOCK], c1w[CL_BLOCK];
+    __shared__ uint8_t used[CL_MAXK];
+    const uint32_t fi = blockIdx.x, tid = threadIdx.x, np = G.n_bp_per;
+    const uint64_t base = (uint64_t)fi * np;
+    uint32_t per = (np + CL_BLOCK - 1) / CL_BLOCK;
+    uint32_t lo = tid * per, hi = lo + per < np ? lo + per : np;
+    uint32_t a = 0, b = 0;
+    for (uint32_t q = lo; q < hi; q++) { a += getbit(bits, base + q); b += getbit(bits, base + q + G.n_bp); }
+    c0w[tid] = a; c1w[tid] = b;
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t r0 = 0, r1 = 0;
+        for (int t = 0; t < CL_BLOCK; t++) { uint32_t x = c0w[t], y = c1w[t]; c0w[t] = r0; c1w[t] = r1; r0 += x; r1 += y; }
+        fiber_k[2 * fi] = r0; fiber_k[2 * fi + 1] = r1;
+    }
+    __syncthreads();
+    uint32_t k0 = fiber_k[2 * fi], k1 = fiber_k[2 * fi + 1];
+    if (k0 > CL_MAXK || k1 > CL_MAXK) { if (tid == 0) atomicOr(overflow, 1u); return; }
+    uint32_t o0 = c0w[tid], o1 = c1w[tid];
+    for (uint32_t q = lo; q < hi; q++) {
+        if (getbit(bits, base + q)) s0[o0++] = q;
+        if (getbit(bits, base + q + G.n_bp)) s1[o1++] = q;
+    }
+    for (uint32_t j = tid; j < k1; j += CL_BLOCK) used[j] = 0;
+    __syncthreads();
+    if (tid == 0) {
+        const int64_t W = 10;
+        unsigned long long dsb = 0;
+        uint32_t jlo = 0;
+        for (uint32_t i = 0; i < k0; i++) {
+            int64_t b0 = s0[i];
+            while (jlo < k1 && (int64_t)s1[jlo] < b0 - W) jlo++;
+            for (uint32_t j = jlo; j < k1 && (int64_t)s1[j] <= b0 + W; j++)
+                if (!used[j]) { used[j] = 1; dsb++; break; }
+        }
+        if (dsb) atomicAdd(&cnt[3], dsb);
+        if (k0) atomicAdd(&cnt[4], (unsigned long long)k0);
+        if (k1) atomicAdd(&cnt[5], (unsigned long long)k1);
+    }
+}
+
+__global__ void k_bits_to_u8(const uint32_t *bits, uint64_t n, uint8_t *out)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint8_t)getbit(bits, i);
+}
+
+// ------------------------------------------------------------ spatial hash
+__global__ void k_hash_key(const float4 *pos, uint64_t n, float cell, uint32_t mask, uint32_t *key, uint32_t *count)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    int32_t ix = (int32_t)floorf(p.x / cell);
+    int32_t iy = (int32_t)floorf(p.y / cell);
+    int32_t iz = (int32_t)floorf(p.z / cell);
+    uint32_t h = ((uint32_t)ix * 73856093u) ^ ((uint32_t)iy * 19349663u) ^ ((uint32_t)iz * 83492791u);
+    h &= mask;
+    key[i] = h;
+    atomicAdd(&count[h], 1u);
+}
+
+__global__ void k_hash_scatter(const uint32_t *key, uint64_t n, const uint32_t *start, uint32_t *cursor, uint32_t *order)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k = key[i];
+    order[start[k] + atomicAdd(&cursor[k], 1u)] = (uint32_t)i;
+}
+
+// buckets are tiny: each lane sorts one bucket by insertion (stable order = by index)
+__global__ void k_hash_sort_buckets(const uint32_t *start, uint32_t nb, uint32_t *order)
+{
+    uint32_t bkt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bkt >= nb) return;
+    uint32_t a = start[bkt], b = start[bkt + 1];
+    for (uint32_t i = a + 1; i < b; i++) {
+        uint32_t v = order[i];
+        uint32_t j = i;
+        while (j > a && order[j - 1] > v) { order[j] = order[j - 1]; j--; }
+        order[j] = v;
+    }
+}
+
+template <class T> static hipError_t sgrow(void *&ptr, size_t &cap, size_t need_elems)
+{
+    size_t need = need_elems * sizeof(T);
+    if (need <= cap && ptr) return hipSuccess;
+    if (ptr) { hipError_t e = hipFree(ptr); if (e != hipSuccess) return e; ptr = nullptr; }
+    size_t n = need < 256 ? 256 : need;
+    hipError_t e = hipMalloc(&ptr, n);
+    cap = (e == hipSuccess) ? n : 0;
+    return e;
+}
+
+static bool g_lcg_ready[64] = {false};
+
+}  // namespace dna
+
+using namespace dna;
+
+extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, uint64_t rad_n,
+                         uint32_t *lcg_state, int copy_hits, dna_score_out *out)
+{
+    if (!c || !t || !lcg_state || !out) { set_error("dna_score: NULL argument"); return DNA_ERR_ARG; }
+    if (!t->fy || !t->fz || !t->rbb0 || !t->rbb1 || t->n_bp_per == 0 || t->grid_N == 0 ||
+        (uint64_t)t->n_fibers * t->n_bp_per != t->n_bp || t->n_fibers != t->grid_N * t->grid_N) {
+        set_error("dna_score: inconsistent DNA target"); return DNA_ERR_ARG;
+    }
+    if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_score: too many radical entries"); return DNA_ERR_CAPACITY; }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    memset(out, 0, sizeof(*out));
+    if (!g_lcg_ready[c->device]) {
+        uint32_t A[32], Cc[32];
+        uint32_t a = 1664525u, cc = 1013904223u;
+        for (int k = 0; k < 32; k++) { A[k] = a; Cc[k] = cc; cc = a * cc + cc; a = a * a; }
+        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_a), A, sizeof(A)));
+        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_c), Cc, sizeof(Cc)));
+        g_lcg_ready[c->device] = true;
+    }
+    ScoreGeom G;
+    G.x_half = (t->n_bp_per - 1) * t->rise * 0.5;
+    G.rise_inv = 1 / t->rise;
+    G.grid_off = -((t->grid_N - 1) * t->spacing_nm) * 0.5;
+    G.inv_spacing = 1 / t->spacing_nm;
+    G.r_bb = t->r_bb; G.x0 = t->x0; G.rise = t->rise;
+    G.grid_N = t->grid_N; G.n_bp_per = t->n_bp_per; G.n_bp = t->n_bp; G.n_fibers = t->n_fibers;
+    // target arrays
+    size_t ntg = 2 * (size_t)t->n_fibers + 4 * (size_t)t->n_bp_per;
+    DNA_HIP(sgrow<float>(c->sc.tgt, c->sc.cap_tgt, ntg));
+    float *d_fy = (float *)c->sc.tgt, *d_fz = d_fy + t->n_fibers, *d_r0 = d_fz + t->n_fibers, *d_r1 = d_r0 + 2 * (size_t)t->n_bp_per;
+    DNA_HIP(hipMemcpyAsync(d_fy, t->fy, t->n_fibers * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_fz, t->fz, t->n_fibers * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_r0, t->rbb0, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_r1, t->rbb1, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
+    const float4 *d_rad;
+    if (rad) {
+        DNA_HIP(sgrow<float4>(c->sc.rad_in, c->sc.cap_rad_in, rad_n + 1));
+        DNA_HIP(hipMemcpyAsync(c->sc.rad_in, rad, rad_n * 16, hipMemcpyHostToDevice, st));
+        d_rad = (const float4 *)c->sc.rad_in;
+    } else {
+        if (rad_n > c->rad_n) { set_error("dna_score: rad == NULL but fewer radicals are resident"); return DNA_ERR_ARG; }
+        d_rad = c->d_rad;
+    }
+    uint64_t nwords = (2 * (uint64_t)t->n_bp + 31) / 32;
+    DNA_HIP(sgrow<uint32_t>(c->sc.hitbits, c->sc.cap_hitbits, nwords + 64 + 2 * (size_t)t->n_fibers));
+    uint32_t *bits = (uint32_t *)c->sc.hitbits;
+    unsigned long long *cnt = (unsigned long long *)(bits + nwords + (nwords & 1));
+    uint32_t *fiber_k = (uint32_t *)(cnt + 8);
+    DNA_HIP(sgrow<uint32_t>(c->sc.flags, c->sc.cap_flags, 3 * (rad_n + 1) + 16));
+    uint32_t *flag = (uint32_t *)c->sc.flags, *rank = flag + rad_n + 1;
+    DNA_HIP(sgrow<uint64_t>(c->sc.rank, c->sc.cap_rank, rad_n + 1 + rad_n / 1024 + 16));
+    uint64_t *hitidx = (uint64_t *)c->sc.rank;
+    uint32_t *scan_tmp = (uint32_t *)(hitidx + rad_n + 1);
+    DNA_HIP(hipEventRecord(c->ev[5], st));
+    DNA_HIP(hipMemsetAsync(bits, 0, (nwords + (nwords & 1)) * 4 + 8 * 8, st));
+    const uint32_t B = 256;
+    uint32_t gb = (uint32_t)((rad_n + B - 1) / B);
+    if (rad_n) {
+        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, rad_n, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx, cnt);
+        DNA_HIP(hipMemcpyAsync(rank, flag, rad_n * 4, hipMemcpyDeviceToDevice, st));
+        DNA_HIP(exclusive_scan_u32(rank, (uint32_t)rad_n, scan_tmp, st));
+        hipLaunchKernelGGL(k_ssb_hits, dim3(gb), dim3(B), 0, st, flag, rank, hitidx, rad_n, *lcg_state, bits, cnt);
+    }
+    uint32_t *ovf = fiber_k + 2 * t->n_fibers;
+    DNA_HIP(hipMemsetAsync(ovf, 0, 4, st));
+    hipLaunchKernelGGL(k_cluster, dim3(t->n_fibers), dim3(CL_BLOCK), 0, st, bits, G, fiber_k, cnt, ovf);
+    DNA_HIP(hipGetLastError());
+    unsigned long long h_cnt[8];
+    std::vector<uint32_t> fk(2 * (size_t)t->n_fibers + 1);
+    DNA_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof(h_cnt), hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(fk.data(), fiber_k, (2 * (size_t)t->n_fibers + 1) * 4, hipMemcpyDeviceToHost, st));
+    out->hits = nullptr;
+    if (copy_hits) {
+        uint64_t nh = 2 * (uint64_t)t->n_bp;
+        DNA_HIP(sgrow<uint8_t>(c->sc.hits_u8, c->sc.cap_hits_u8, nh));
+        if (nh > c->sc.hcap_hits) {
+            if (c->sc.h_hits) hipHostFree(c->sc.h_hits);
+            DNA_HIP(hipHostMalloc((void **)&c->sc.h_hits, nh, hipHostMallocDefault));
+            c->sc.hcap_hits = nh;
+        }
+        hipLaunchKernelGGL(k_bits_to_u8, dim3((uint32_t)((nh + B - 1) / B)), dim3(B), 0, st, bits, nh, (uint8_t *)c->sc.hits_u8);
+        DNA_HIP(hipMemcpyAsync(c->sc.h_hits, c->sc.hits_u8, nh, hipMemcpyDeviceToHost, st));
+        out->hits = c->sc.h_hits;
+    }
+    DNA_HIP(hipEventRecord(c->ev[6], st));
+    DNA_HIP(hipStreamSynchronize(st));
+    if (fk[2 * t->n_fibers]) { set_error("dna_score: more than 4096 hits on one strand of one fiber"); return DNA_ERR_CAPACITY; }
+    out->candidates = h_cnt[0]; out->in_reach = h_cnt[1]; out->ssb_count = h_cnt[2];
+    out->dsb = h_cnt[3]; out->ssb0 = h_cnt[4]; out->ssb1 = h_cnt[5];
+    // closed-form expected DSB (ssb-dsb.ts:398-403), summed over fibers in order
+    double p_pair = 21.0 / (double)t->n_bp_per;
+    if (p_pair > 1) p_pair = 1;
+    double ex = 0;
+    for (uint32_t f = 0; f < t->n_fibers; f++) {
+        uint64_t k0 = fk[2 * f], k1 = fk[2 * f + 1];
+        if (k0 > 0 && k1 > 0) ex += 1 - std::pow(1 - p_pair, (double)(k0 * k1));
+    }
+    out->dsb_expected = ex;
+    // advance the caller's LCG by the draws consumed
+    uint32_t s = *lcg_state;
+    uint64_t m = h_cnt[1];
+    uint32_t a = 1664525u, cc = 1013904223u;
+    for (int k = 0; k < 64 && m; k++) {
+        if (m & 1ull) s = a * s + cc;
+        cc = a * cc + cc; a = a * a; m >>= 1;
+    }
+    *lcg_state = s;
+    float ms = 0; hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+    out->t_ms = ms;
+    return DNA_OK;
+}
+
+extern "C" int dna_hash_cells(dna_ctx *c, const float *pos4, uint64_t n, float cell_nm, uint32_t bits,
+                              uint32_t *key_out, uint32_t *cell_start_out, uint32_t *order_out)
+{
+    if (!c || (!pos4 && n) || !key_out || !cell_start_out || !order_out) { set_error("dna_hash_cells: NULL argument"); return DNA_ERR_ARG; }
+    if (bits < 1 || bits > 26 || !(cell_nm > 0.0f)) { set_error("dna_hash_cells: bits must be 1..26 and cell_nm > 0"); return DNA_ERR_ARG; }
+    if (n >= 0xFFFFFFF0ull) { set_error("dna_hash_cells: too many points"); return DNA_ERR_CAPACITY; }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    uint32_t nb = 1u << bits;
+    size_t words = 4 * (size_t)n + 2 * ((size_t)nb + 1) + nb / 1024 + 64;   // pos(4n) key, start, cursor, order...
+    DNA_HIP(sgrow<uint32_t>(c->hs.buf, c->hs.cap, words + 2 * (size_t)n));
+    float4 *d_pos = (float4 *)c->hs.buf;
+    uint32_t *d_key = (uint32_t *)(d_pos + n), *d_start = d_key + n, *d_cur = d_start + nb + 1, *d_ord = d_cur + nb + 1;
+    uint32_t *d_tmp = d_ord + n;
+    DNA_HIP(hipMemcpyAsync(d_pos, pos4, n * 16, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemsetAsync(d_start, 0, 2 * ((size_t)nb + 1) * 4, st));
+    const uint32_t B = 256;
+    if (n) hipLaunchKernelGGL(k_hash_key, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, st, d_pos, n, cell_nm, nb - 1, d_key, d_start);
+    DNA_HIP(exclusive_scan_u32(d_start, nb, d_tmp, st));
+    if (n) hipLaunchKernelGGL(k_hash_scatter, dim3((uint32_t)((n + B - 1) / B)), dim3(B), 0, st, d_key, n, d_start, d_cur, d_ord);
+    hipLaunchKernelGGL(k_hash_sort_buckets, dim3((nb + B - 1) / B), dim3(B), 0, st, d_start, nb, d_ord);
+    DNA_HIP(hipGetLastError());
+    DNA_HIP(hipMemcpyAsync(key_out, d_key, n * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(cell_start_out, d_start, ((size_t)nb + 1) * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(order_out, d_ord, n * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    return DNA_OK;
+}
