@@ -117,7 +117,15 @@ typedef struct dna_chem_out {
     uint64_t chem_n;                   /* rad entries handed in              */
     uint64_t n_tracks;                 /* primaries with at least one radical */
     uint64_t n_events;                 /* candidate reaction events sampled  */
-    double   t_ms;                     /* device time                        */
+    double   t_ms;                     /* device time of the whole call      */
+    double   t_kernel_ms;              /* device time of the per-track kernel */
+    /* algorithmic work: pairs with a reaction channel (distance evaluated), pairs within
+     * the cut-off (reaction time sampled), product re-pairings with a Brownian step */
+    uint64_t n_pair_cand, n_pair_sampled, n_repair;
+    double   algo_flops;               /* FP64 operations by the static cost model */
+    uint64_t prof[8];                  /* per-phase 100 MHz clock sums over tracks: load,
+                                          initial pairs, event scan, reaction, re-pairing,
+                                          event-loop iterations */
 } dna_chem_out;
 
 /* DNA fiber target, DNATarget of src/physics/types.ts:23-37 / buildDNATarget

@@ -56,6 +56,9 @@ struct IrtArgs {
     double *tb;
     double *ev_t;
     uint2 *ev_ab;
+    uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
+    float4 *sorted;               // species-sorted copy of the initial particles
+    uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *overflow;
     double tc[DNA_MAX_CKPT];
@@ -63,29 +66,19 @@ struct IrtArgs {
     uint64_t seed;
     int sort_tracks;              // 1: bucket order is not input order -> sort per track
 };
-#define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 2)
+#define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 5 + 8)   // tallies, rxn, n_re, n_ev, 3 work counters, 8 phase timers
 
 // ------------------------------------------------------------ samplers (FP64)
-__device__ double sample_pdc(double a, double b, uint64_t ps)
-{
-    double p = 2.0 * sqrt(2.0 * b / a);
-    double q = 2.0 / sqrt(2.0 * b / a);
-    double M1 = 1.0 / (a * a), M2 = 3.0 * b / a;
-    double M = M1 > M2 ? M1 : M2;
-    for (int trial = 0; trial < 10000; trial++) {
-        double U = dna_u01(ps, 7u + 2u * (uint32_t)trial);
-        double X, v;
-        if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
-        else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
-        U = dna_u01(ps, 8u + 2u * (uint32_t)trial);
-        double sqX = sqrt(X);
-        double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
-        if ((X <= 2.0 * b / a && U <= lam) || (X >= 2.0 * b / a && U * M / X <= lam)) return X;
-    }
-    return -1.0;
-}
+// sampleIRT (irt-worker.js:116-184) is split in two.  Stage 1 is everything except the
+// SamplePDC rejection loop; it returns a reaction time t >= 0, -1 (no reaction) or -2
+// (the pair needs SamplePDC).  Stage-2 pairs are queued and the whole workgroup drains
+// the queue afterwards, each lane taking a new pair as soon as its trial loop ends, so
+// the long, uneven rejection loops (~100-200 trials at 100-500 nm) do not stall the
+// other lanes of a wavefront.  Draw indices are those of the one-pass sampler (U at
+// draw 6, trial t at draws 7+2t, 8+2t), so the result is unchanged.
+#define STAGE_PDC (-2.0)
 
-__device__ double sample_irt(double r0, int ri, double D, uint64_t ps)
+__device__ double irt_stage1(double r0, int ri, double D, uint64_t ps)
 {
     const IrtConst &C = c_irt;
     if (C.type[ri] == 0) {
@@ -103,25 +96,83 @@ __device__ double sample_irt(double r0, int ri, double D, uint64_t ps)
         double dr = r0e - sigma;
         return 0.25 * dr * dr / (D * ei * ei);
     }
-    double sc = C.sigma[ri], rc = C.rc[ri], kobs = C.kobs[ri], kdif = C.kdif[ri], kact = C.kact[ri];
-    double sig_chk = sc, r0_chk = r0;
+    double rc = C.rc[ri];
+    double sig_chk = C.sigma[ri], r0_chk = r0;
     if (rc != 0) { r0_chk = -rc / (1 - dna_exp(rc / r0)); sig_chk = C.sigma_eff[ri]; }
     if (sig_chk > r0_chk) return (C.prob[ri] > dna_u01(ps, 6u)) ? 0.0 : -1.0;
-    double Winf = sig_chk / r0_chk * kobs / kdif;
+    double Winf = sig_chk / r0_chk * C.kobs[ri] / C.kdif[ri];
     if (!(Winf > dna_u01(ps, 6u))) return -1;
-    double a, b;
-    if (rc == 0) { a = (1.0 / sc) * kact / kobs; b = (r0 - sc) / 2; }
-    else {
-        double v = kact / (4 * DNA_PI * sc * sc * dna_exp(-rc / sc));
-        double alpha = v + rc * D / (sc * sc * (1 - dna_exp(-rc / sc)));
-        double sh, ch; dna_sinhcosh(rc / (2 * sc), &sh, &ch);
-        a = 4 * sc * sc * alpha / (D * rc * rc) * sh * sh;
-        double sh0, ch0; dna_sinhcosh(rc / (2 * r0), &sh0, &ch0);
-        b = rc / 4 * (ch0 / sh0 - ch / sh);
+    return STAGE_PDC;
+}
+
+// Stage 0: true when sampleIRT can return a time for this pair (contact, or the first
+// uniform below the survival probability).  Same expressions as irt_stage1, so the
+// decision is identical; most pairs stop here with one draw and no transcendental.
+__device__ __forceinline__ bool irt_may_react(double r0, int ri, double D, uint64_t ps)
+{
+    const IrtConst &C = c_irt;
+    if (C.type[ri] == 0) {
+        double sigma = C.sigma[ri], rc = C.rc[ri];
+        if (sigma <= 0 || D <= 0) return false;
+        if (r0 <= sigma) return true;
+        double r0e = r0;
+        if (rc != 0) r0e = -rc / (1 - dna_exp(rc / r0));
+        if (r0e <= sigma) return true;
+        double Winf = sigma / r0e;
+        double U = dna_u01(ps, 6u);
+        return !(U <= 0 || U >= Winf);
     }
-    double X = sample_pdc(a, b, ps);
-    if (X > 0) return X / D;
-    return -1;
+    double rc = C.rc[ri];
+    double sig_chk = C.sigma[ri], r0_chk = r0;
+    if (rc != 0) { r0_chk = -rc / (1 - dna_exp(rc / r0)); sig_chk = C.sigma_eff[ri]; }
+    if (sig_chk > r0_chk) return true;
+    double Winf = sig_chk / r0_chk * C.kobs[ri] / C.kdif[ri];
+    return Winf > dna_u01(ps, 6u);
+}
+
+// Constants of one reaction in registers, and irt_may_react on them.
+struct RxK { int type; double sigma, rc, sigma_eff, prob, kobs, kdif, D; };
+
+__device__ __forceinline__ RxK rx_constants(int ri, double D)
+{
+    const IrtConst &C = c_irt;
+    RxK k;
+    k.type = C.type[ri]; k.sigma = C.sigma[ri]; k.rc = C.rc[ri]; k.sigma_eff = C.sigma_eff[ri];
+    k.prob = C.prob[ri]; k.kobs = C.kobs[ri]; k.kdif = C.kdif[ri]; k.D = D;
+    return k;
+}
+
+__device__ __forceinline__ bool may_react_k(const RxK &k, double r0, uint64_t ps)
+{
+    if (k.type == 0) {
+        if (k.sigma <= 0 || k.D <= 0) return false;
+        if (r0 <= k.sigma) return true;
+        double r0e = r0;
+        if (k.rc != 0) r0e = -k.rc / (1 - dna_exp(k.rc / r0));
+        if (r0e <= k.sigma) return true;
+        double Winf = k.sigma / r0e;
+        double U = dna_u01(ps, 6u);
+        return !(U <= 0 || U >= Winf);
+    }
+    double sig_chk = k.sigma, r0_chk = r0;
+    if (k.rc != 0) { r0_chk = -k.rc / (1 - dna_exp(k.rc / r0)); sig_chk = k.sigma_eff; }
+    if (sig_chk > r0_chk) return true;
+    double Winf = sig_chk / r0_chk * k.kobs / k.kdif;
+    return Winf > dna_u01(ps, 6u);
+}
+
+// SamplePDC parameters a, b (irt-worker.js:145-160)
+__device__ void pdc_ab(double r0, int ri, double D, double &a, double &b)
+{
+    const IrtConst &C = c_irt;
+    double sc = C.sigma[ri], rc = C.rc[ri], kobs = C.kobs[ri], kact = C.kact[ri];
+    if (rc == 0) { a = (1.0 / sc) * kact / kobs; b = (r0 - sc) / 2; return; }
+    double v = kact / (4 * DNA_PI * sc * sc * dna_exp(-rc / sc));
+    double alpha = v + rc * D / (sc * sc * (1 - dna_exp(-rc / sc)));
+    double sh, ch; dna_sinhcosh(rc / (2 * sc), &sh, &ch);
+    a = 4 * sc * sc * alpha / (D * rc * rc) * sh * sh;
+    double sh0, ch0; dna_sinhcosh(rc / (2 * r0), &sh0, &ch0);
+    b = rc / 4 * (ch0 / sh0 - ch / sh);
 }
 
 // ------------------------------------------------------------ grouping kernels
@@ -223,11 +274,179 @@ struct TrackShared {
     unsigned long long cnt[5];   // running OH, e-aq, H, H2O2, H2
     unsigned long long rxc[9];   // reaction counts
     unsigned long long n_re;
+    uint32_t npend;              // queued stage-2 (SamplePDC) pairs
+    uint32_t pend_next;          // drain cursor
+    uint32_t long_n, long_next;  // pairs handed from the per-lane to the wavefront schedule
+    uint32_t long_item[512];
+    uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
+    int      bri[9], nblk;                // channel blocks: reaction, units per row
+    uint32_t bcpr[9], bu0[9], units;
 };
+
+__device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
+                                           double t, uint32_t a, uint32_t b)
+{
+    uint32_t e = atomicAdd(&sh.n_ev, 1u);
+    if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
+    else atomicOr(A.overflow, 2u);
+}
+
+__device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
+                                             uint32_t a, uint32_t b, double r0)
+{
+    uint32_t q = atomicAdd(&sh.npend, 1u);
+    if (q < T.ev_cap) A.pend[T.ev_off + q] = make_uint4(a, b, (uint32_t)__double2loint(r0), (uint32_t)__double2hiint(r0));
+    else atomicOr(A.overflow, 2u);
+}
+
+// Stage 2 for queued pairs: the complete sampleIRT (irt-worker.js:116-184) — stage 1
+// again (erfcinv for a reacting TDC pair, the contact test), then, for PDC pairs that
+// survive the first draw, the SamplePDC rejection loop (irt-worker.js:91-113).  An
+// accepted X gives t = X / D; the event is kept if t_base + t < 1 us.  Two schedules:
+//  * per-lane: each lane owns one queued pair and takes the next as soon as its own
+//    trial loop ends (large queues: the initial pairs of a track);
+//  * per-wavefront: the 64 lanes run 64 consecutive trials of one pair at once and the
+//    first accepted trial wins, the same X as the sequential loop (small queues: the
+//    re-pairing of one reaction's products).
+// Called by all threads after a barrier; the caller resets the queue.
+struct PendPair { uint32_t a, b; double r0; int ri; double D; uint64_t ps; };
+
+__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t item)
+{
+    const IrtConst &C = c_irt;
+    const float4 *P = A.pos + T.part_off;
+    uint4 e = A.pend[T.ev_off + item];
+    PendPair q;
+    q.a = e.x; q.b = e.y;
+    q.r0 = __hiloint2double((int)e.w, (int)e.z);
+    int si = (int)P[q.a].w, sj = (int)P[q.b].w;
+    q.ri = C.rxnMap[si * 6 + sj];
+    q.D = C.D[si] + C.D[sj];
+    q.ps = dna_pair_state(tk, q.a, q.b);
+    return q;
+}
+
+__device__ __forceinline__ void pdc_setup(const PendPair &q, double &a, double &b, double &p, double &pq, double &M)
+{
+    pdc_ab(q.r0, q.ri, q.D, a, b);
+    p = 2.0 * sqrt(2.0 * b / a);
+    double qq = 2.0 / sqrt(2.0 * b / a);
+    double M1 = 1.0 / (a * a), M2 = 3.0 * b / a;
+    M = M1 > M2 ? M1 : M2;
+    pq = qq;
+}
+
+// one SamplePDC trial (irt-worker.js:95-111); returns accept
+__device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q, double M, uint64_t ps,
+                                          uint32_t trial, double &X)
+{
+    double U = dna_u01(ps, 7u + 2u * trial);
+    double v;
+    if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
+    else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
+    U = dna_u01(ps, 8u + 2u * trial);
+    double sqX = sqrt(X);
+    double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
+    return (X <= 2.0 * b / a && U <= lam) || (X >= 2.0 * b / a && U * M / X <= lam);
+}
+
+#define LANE_TRIALS 64u
+#define LONG_MAX 512u
+
+// per-lane schedule; a pair still running after LANE_TRIALS trials is handed to the
+// wavefront schedule (sh.long_item / sh.long_n) and resumed there at that trial
+__device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+                                uint32_t npend, double t_base)
+{
+    uint32_t item = atomicAdd(&sh.pend_next, 1u);
+    bool act = item < npend;
+    double a = 1, b = 1, p = 0, q = 0, M = 0;
+    PendPair pp; pp.a = 0; pp.b = 0; pp.D = 1; pp.ps = 0; pp.r0 = 0; pp.ri = 0;
+    uint32_t trial = 0;
+    bool need_setup = true;
+    while (act) {
+        bool done = false;
+        double X = -1;
+        if (need_setup) {
+            pp = load_pending(A, T, tk, item);
+            double t1 = irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+            if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; need_setup = false; }
+            else {
+                if (t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
+                done = true;
+            }
+        }
+        if (!done) {
+            bool acc = pdc_trial(a, b, p, q, M, pp.ps, trial, X);
+            trial++;
+            if (acc) {
+                if (X > 0) { double ta = t_base + X / pp.D; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
+                done = true;
+            } else if (trial == LANE_TRIALS) {
+                uint32_t l = atomicAdd(&sh.long_n, 1u);
+                if (l < LONG_MAX) { sh.long_item[l] = item; done = true; }
+            } else if (trial == 10000u) done = true;
+        }
+        if (done) {
+            item = atomicAdd(&sh.pend_next, 1u);
+            act = item < npend;
+            need_setup = true;
+        }
+    }
+}
+
+// wavefront schedule over items [from the queue, or the long list when use_long]
+__device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+                                uint32_t npend, double t_base, bool use_long)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nlong = sh.long_n < LONG_MAX ? sh.long_n : LONG_MAX;
+    for (;;) {
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(use_long ? &sh.long_next : &sh.pend_next, 1u);
+        slot = __shfl(slot, 0, 64);
+        if (slot >= (use_long ? nlong : npend)) break;
+        uint32_t item = use_long ? sh.long_item[slot] : slot;
+        PendPair pp = load_pending(A, T, tk, item);
+        uint32_t start = 0;
+        if (!use_long) {
+            double t1 = irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+            if (t1 != STAGE_PDC) {
+                if (lane == 0 && t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
+                continue;
+            }
+        } else start = LANE_TRIALS;
+        double a, b, p, q, M;
+        pdc_setup(pp, a, b, p, q, M);
+        double X = -1;
+        for (uint32_t base = start; base < 10000u; base += 64u) {
+            uint32_t trial = base + lane;
+            double Xl = 0;
+            bool acc = false;
+            if (trial < 10000u) acc = pdc_trial(a, b, p, q, M, pp.ps, trial, Xl);
+            unsigned long long m = __ballot(acc);
+            if (m) { int first = __ffsll((long long)m) - 1; X = __shfl(Xl, first, 64); break; }
+        }
+        if (lane == 0 && X > 0) { double ta = t_base + X / pp.D; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
+    }
+}
+
+// drain the whole queue; all threads call it after a barrier
+__device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
+{
+    const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
+    if (npend >= 2u * IRT_BLOCK) {
+        pdc_drain_lanes(A, T, tk, sh, npend, t_base);
+        __syncthreads();
+        pdc_drain_waves(A, T, tk, sh, npend, t_base, true);
+    } else {
+        pdc_drain_waves(A, T, tk, sh, npend, t_base, false);
+    }
+}
 
 // pair particle `idx` (a new product born at t_new) with all alive particles < n_total
 __device__ void pair_with_alive(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t idx,
-                                uint32_t n_total, double t_new, TrackShared &sh)
+                                uint32_t n_total, double t_new, TrackShared &sh, unsigned long long &wrep)
 {
     const IrtConst &C = c_irt;
     const float4 *P = A.pos + T.part_off;
@@ -242,6 +461,7 @@ __device__ void pair_with_alive(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         int sj = (int)pj.w;
         int ri = C.rxnMap[si * 6 + sj];
         if (ri < 0) continue;
+        wrep++;
         double Dj = C.D[sj];
         uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
         uint64_t ps = dna_pair_state(tk, a, b);
@@ -257,15 +477,8 @@ __device__ void pair_with_alive(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
         double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
         if (r2 > C.R_CUT2) continue;
-        double t = sample_irt(sqrt(r2), ri, Di + Dj, ps);
-        if (t >= 0) {
-            double ta = t_new + t;
-            if (ta < DNA_T_END_NS) {
-                uint32_t e = atomicAdd(&sh.n_ev, 1u);
-                if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = ta; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
-                else atomicOr(A.overflow, 2u);
-            }
-        }
+        double r0 = sqrt(r2);
+        if (irt_may_react(r0, ri, Di + Dj, ps)) push_pending(A, T, sh, a, b, r0);
     }
 }
 
@@ -312,10 +525,12 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             }
         ord = s_sort;
     }
-    if (tid == 0) { sh.n_ev = 0; sh.n_total = n; sh.n_re = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
+    if (tid == 0) { sh.n_ev = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
     for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_BLOCK) s_loc[i] = 0;
     __syncthreads();
 
+    // phase timers (100 MHz wall clock, lane 0): load, pairs, scan, react, repair, iterations
+    unsigned long long tp0 = wall_clock64(), tp1 = 0, tp2 = 0, tsc = 0, trc = 0, trp = 0, nit = 0;
     // 1. load + displacement (irt-worker.js:475-537)
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
@@ -351,45 +566,116 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     if (tid == 0) sh.cnt[4] = T.h2;
     __syncthreads();
 
-    // 2. initial pairs (irt-worker.js:541-558), circular enumeration
-    if (n > 1) {
-        uint32_t h = (n - 1) / 2;
-        bool even = (n & 1u) == 0;
-        for (uint32_t i = tid; i < n; i += IRT_BLOCK) {
-            if (!al[i]) continue;
-            float4 pi = P[i];
-            int si = (int)pi.w;
-            double Di = C.D[si];
-            uint32_t dmax = h + ((even && i < n / 2) ? 1u : 0u);
-            uint32_t j = i;
-            for (uint32_t d = 1; d <= dmax; d++) {
-                j = j + 1 == n ? 0 : j + 1;
-                if (!al[j]) continue;
-                float4 pj = P[j];
-                int sj = (int)pj.w;
-                int ri = C.rxnMap[si * 6 + sj];
-                if (ri < 0) continue;
+    tp1 = wall_clock64();
+    // 2. initial pairs (irt-worker.js:541-558).  The particles are copied in species
+    // order (Sg, original index in .w) and only the species blocks that have a reaction
+    // channel are enumerated, in units of one row particle x 64 partners, so every lane
+    // of a wavefront evaluates the same reaction with the same constants.  Within a
+    // species the pairs are enumerated circularly, (i, i+d mod m) for d = 1..(m-1)/2 and
+    // d = m/2 for i < m/2 when m is even: every unordered pair exactly once, equal rows.
+    // Pairs are identified by their original indices, so the enumeration order does not
+    // change any draw or the event order.
+    unsigned long long wcand = 0, wsamp = 0, wrep = 0;
+    extern __shared__ float4 dyn_lds[];
+    float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
+    if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
+    __syncthreads();
+    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+        int spc = (int)P[k].w;
+        if (al[k] && spc >= 0 && spc < 6) atomicAdd(&sh.gcnt[spc], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t o = 0;
+        for (int g = 0; g < 6; g++) { sh.goff[g] = o; o += sh.gcnt[g]; }
+        // channel blocks (irt-worker.js:216-226): (a, b, reaction)
+        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+        uint32_t u = 0;
+        int nb = 0;
+        for (int r = 0; r < 9; r++) {
+            uint32_t nA = sh.gcnt[BA[r]], nB = sh.gcnt[BB[r]];
+            uint32_t units = 0, cpr = 0;
+            if (BA[r] == BB[r]) { if (nA >= 2) { cpr = ((nA - 1) / 2 + 1 + 63) / 64; units = nA * cpr; } }
+            else if (nA && nB) { cpr = (nB + 63) / 64; units = nA * cpr; }
+            if (units) { sh.bri[nb] = r; sh.bcpr[nb] = cpr; sh.bu0[nb] = u; u += units; nb++; }
+        }
+        sh.nblk = nb; sh.units = u;
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+        float4 q = P[k];
+        int spc = (int)q.w;
+        if (al[k] && spc >= 0 && spc < 6) {
+            uint32_t slot = sh.goff[spc] + atomicAdd(&sh.gcur[spc], 1u);
+            Sg[slot] = make_float4(q.x, q.y, q.z, __int_as_float((int)k));
+        }
+    }
+    __syncthreads();
+    {
+        // block loop outermost: the reaction and its constants are wavefront-uniform
+        // (readfirstlane -> scalar registers), so the per-pair path has no table loads
+        const uint32_t lane = tid & 63u, wave = tid >> 6;
+        const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
+        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+        const double RC2 = c_irt.R_CUT2;
+        for (int q = 0; q < nb; q++) {
+            const int ri = __builtin_amdgcn_readfirstlane(sh.bri[q]);
+            const int sa = BA[ri], sb = BB[ri];
+            const uint32_t cpr = __builtin_amdgcn_readfirstlane(sh.bcpr[q]);
+            const uint32_t nA = __builtin_amdgcn_readfirstlane(sh.gcnt[sa]);
+            const uint32_t nB = __builtin_amdgcn_readfirstlane(sh.gcnt[sb]);
+            const uint32_t oA = __builtin_amdgcn_readfirstlane(sh.goff[sa]);
+            const uint32_t oB = __builtin_amdgcn_readfirstlane(sh.goff[sb]);
+            const uint32_t units = nA * cpr;
+            RxK k = rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]);
+            const uint32_t h = (nA - 1) / 2;
+            for (uint32_t ul = wave; ul < units; ul += IRT_BLOCK / 64) {
+                const uint32_t i = ul / cpr, c = ul - i * cpr;
+                const float4 pi = Sg[oA + i];
+                uint32_t j;
+                bool valid;
+                if (sa == sb) {
+                    uint32_t dmax = h + (((nA & 1u) == 0 && i < nA / 2) ? 1u : 0u);
+                    uint32_t d = c * 64u + lane + 1u;
+                    valid = d <= dmax;
+                    j = i + d; if (j >= nA) j -= nA;
+                    j += oA;
+                } else {
+                    uint32_t jj = c * 64u + lane;
+                    valid = jj < nB;
+                    j = oB + jj;
+                }
+                if (!valid) continue;
+                wcand++;
+                const float4 pj = Sg[j];
                 double ddx = (double)pi.x - (double)pj.x;
                 double ddy = (double)pi.y - (double)pj.y;
                 double ddz = (double)pi.z - (double)pj.z;
                 double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (r2 > C.R_CUT2) continue;
-                uint32_t a = i < j ? i : j, b = i < j ? j : i;
-                double t = sample_irt(sqrt(r2), ri, Di + C.D[sj], dna_pair_state(tk, a, b));
-                if (t >= 0 && t < DNA_T_END_NS) {
-                    uint32_t e = atomicAdd(&sh.n_ev, 1u);
-                    if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
-                    else atomicOr(A.overflow, 2u);
-                }
+                if (r2 > RC2) continue;
+                wsamp++;
+                uint32_t oi = (uint32_t)__float_as_int(pi.w), oj = (uint32_t)__float_as_int(pj.w);
+                uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
+                double r0 = sqrt(r2);
+                if (may_react_k(k, r0, dna_pair_state(tk, a, b)))
+                    push_pending(A, T, sh, a, b, r0);
             }
         }
     }
     __syncthreads();
+    unsigned long long tpl = wall_clock64();
+    pdc_drain(A, T, tk, sh, 0.0);
+    __syncthreads();
+    if (tid == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
+    __syncthreads();
     unsigned long long n_init_ev = sh.n_ev;
     int cp = 0;
+    tp2 = wall_clock64();
 
     // 3. event loop
     for (;;) {
+        unsigned long long q0 = wall_clock64();
+        nit++;
         uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
         double bt = 1e300; uint64_t bab = ~0ull;
         for (uint32_t e = tid; e < ne; e += IRT_BLOCK) {
@@ -406,6 +692,8 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         }
         if ((tid & 63) == 0) { sh.wt[tid >> 6] = bt; sh.wab[tid >> 6] = bab; }
         __syncthreads();
+        unsigned long long q1 = wall_clock64();
+        tsc += q1 - q0;
         if (tid == 0) {
             double t0 = sh.wt[0]; uint64_t ab0 = sh.wab[0];
             for (int w = 1; w < IRT_BLOCK / 64; w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
@@ -450,14 +738,37 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             }
         }
         __syncthreads();
+        unsigned long long q2 = wall_clock64();
+        trc += q2 - q1;
         if (!(sh.win_t < 1e300)) break;
         int np = sh.np;
         uint32_t base_total = sh.n_total - (uint32_t)np;
         for (int pp = 0; pp < np; pp++) {
             // product pp is paired with everything alive up to and including itself
-            pair_with_alive(A, T, tk, sh.prod_idx[pp], base_total + (uint32_t)pp + 1u, sh.win_t, sh);
+            pair_with_alive(A, T, tk, sh.prod_idx[pp], base_total + (uint32_t)pp + 1u, sh.win_t, sh, wrep);
         }
         __syncthreads();
+        if (sh.npend) {
+            pdc_drain(A, T, tk, sh, sh.win_t);
+            __syncthreads();
+            if (tid == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
+            __syncthreads();
+        }
+        trp += wall_clock64() - q2;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); wrep += __shfl_xor(wrep, o, 64);
+    }
+    if ((tid & 63) == 0) {
+        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 11;
+        if (wcand) atomicAdd(&g[0], wcand);
+        if (wsamp) atomicAdd(&g[1], wsamp);
+        if (wrep) atomicAdd(&g[2], wrep);
+    }
+    if (tid == 0) {
+        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
+        atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tpl - tp1); atomicAdd(&g[6], tp2 - tpl); atomicAdd(&g[2], tsc);
+        atomicAdd(&g[3], trc); atomicAdd(&g[4], trp); atomicAdd(&g[5], nit);
     }
     if (tid == 0) {
         while (cp < A.nc) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
@@ -651,10 +962,10 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
             t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
             t.ev_off = evs; evs += t.ev_cap;
         }
-        size_t bytes_part = part * (16 + 1 + 8);
+        size_t bytes_part = part * (16 + 16 + 1 + 8);
         DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch,
                             std::max<size_t>(bytes_part + 256, rad_n / 1024 * 4 + 64)));
-        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 16 + 256));
+        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + 256));
         // layout of the track/bucket block inside irt.tally
         size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
         DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, tally_bytes + ntr * sizeof(IrtTrack) + (K + 1) * 4 + (size_t)run * 4 + 1024));
@@ -672,13 +983,20 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         IrtArgs A;
         char *sp = (char *)c->irt.scratch;
         A.rad = d_rad; A.idx = d_idx; A.tracks = d_tr;
-        A.pos = (float4 *)sp; A.tb = (double *)(sp + part * 16); A.alive = (uint8_t *)(sp + part * 24);
+        A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32); A.alive = (uint8_t *)(sp + part * 40);
         A.ev_t = (double *)c->irt.events; A.ev_ab = (uint2 *)((char *)c->irt.events + evs * 8);
+        A.pend = (uint4 *)((char *)c->irt.events + evs * 16);
         A.tally = d_tally; A.overflow = g.misc + 2;
         for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
         A.nc = nc; A.seed = seed; A.sort_tracks = contiguous ? 0 : 1;
-        if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(ntr), dim3(IRT_BLOCK), 0, st, A);
-        else hipLaunchKernelGGL(k_irt<true>, dim3(ntr), dim3(IRT_BLOCK), 0, st, A);
+        DNA_HIP(hipEventRecord(c->ev[7], st));
+        // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB, three
+        // workgroups per CU), in global memory above
+        A.lds_slots = std::min<uint32_t>(maxn_all, 3072u);
+        size_t shmem = (size_t)A.lds_slots * 16;
+        if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(ntr), dim3(IRT_BLOCK), shmem, st, A);
+        else hipLaunchKernelGGL(k_irt<true>, dim3(ntr), dim3(IRT_BLOCK), shmem, st, A);
+        DNA_HIP(hipEventRecord(c->ev[4], st));
         DNA_HIP(hipGetLastError());
         uint32_t ovf = 0;
         DNA_HIP(hipMemcpyAsync(&ovf, g.misc + 2, 4, hipMemcpyDeviceToHost, st));
@@ -688,7 +1006,9 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         DNA_HIP(hipStreamSynchronize(st));
         if (ovf & 2u) { ev_mult *= 4; continue; }
         if (ovf & 1u) { continue; }
-        float ms = 0; hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+        float ms = 0, kms = 0;
+        hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+        hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
         for (uint32_t i = 0; i < n_inst; i++) {
             const unsigned long long *g2 = &ht[(size_t)i * TALLY_STRIDE];
             dna_chem_out &o = outs[i];
@@ -700,7 +1020,16 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
             for (int r = 0; r < 9; r++) o.rxn_counts[r] = g2[DNA_MAX_CKPT * 5 + r];
             o.n_reacted = g2[DNA_MAX_CKPT * 5 + 9];
             o.n_events = g2[DNA_MAX_CKPT * 5 + 10];
+            o.n_pair_cand = g2[DNA_MAX_CKPT * 5 + 11];
+            o.n_pair_sampled = g2[DNA_MAX_CKPT * 5 + 12];
+            o.n_repair = g2[DNA_MAX_CKPT * 5 + 13];
+            for (int q = 0; q < 8; q++) o.prof[q] = g2[DNA_MAX_CKPT * 5 + 14 + q];
+            // static FP64 cost model (counted on the restated worker source): distance and
+            // cut-off test 9, sqrt + first sampling stage 6, Brownian re-pair step
+            // (3 x [log 20 + cos 25 + sqrt + 4] + sqrt(2 D dt) 3 + distance 9 + sampling 6) 167
+            o.algo_flops = 9.0 * (double)o.n_pair_cand + 6.0 * (double)o.n_pair_sampled + 167.0 * (double)o.n_repair;
             o.t_ms = ms;
+            o.t_kernel_ms = kms;
         }
         return DNA_OK;
     }

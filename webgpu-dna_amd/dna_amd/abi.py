@@ -42,7 +42,9 @@ class DnaChemOut(C.Structure):
         ("tally", (C.c_uint64 * 5) * DNA_MAX_CKPT), ("G", (C.c_double * 5) * DNA_MAX_CKPT),
         ("rxn_counts", C.c_uint64 * DNA_N_RXN), ("n_reacted", C.c_uint64),
         ("chem_n", C.c_uint64), ("n_tracks", C.c_uint64), ("n_events", C.c_uint64),
-        ("t_ms", C.c_double),
+        ("t_ms", C.c_double), ("t_kernel_ms", C.c_double),
+        ("n_pair_cand", C.c_uint64), ("n_pair_sampled", C.c_uint64), ("n_repair", C.c_uint64),
+        ("algo_flops", C.c_double), ("prof", C.c_uint64 * 8),
     ]
 
 
