@@ -58,6 +58,8 @@ struct IrtArgs {
     uint2 *ev_ab;
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
+    uint32_t *lst;                // 6 species lists of capacity cap per track (alive particles)
+    uint32_t *slot;               // position of each particle in its species list
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *overflow;
@@ -207,31 +209,53 @@ __global__ void k_pidmax(const float4 *rad, uint64_t n, uint32_t *pidmax)
 }
 
 // kept[i] = 1 for a particle entry; key[i] = inst*(pidmax+1) + pid; counts and markers
+// Runs of equal tags in a wavefront: grouped inputs put one primary's entries on
+// consecutive lanes, so counting with one atomic per run (not per entry) removes the
+// contention of thousands of same-address atomics.
+__device__ __forceinline__ uint32_t run_length_if_head(uint64_t tag)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    uint64_t prev = __shfl_up(tag, 1, 64);
+    bool head = lane == 0 || prev != tag;
+    unsigned long long H = __ballot(head);
+    if (!head) return 0;
+    unsigned long long above = lane == 63 ? 0ull : (H & ~((2ull << lane) - 1ull));
+    uint32_t next = above ? (uint32_t)(__ffsll((long long)above) - 1) : 64u;
+    return next - lane;
+}
+
+// kept[i] = 1 for a particle entry; key[i] = inst*(pidmax+1) + pid; counts and markers
 __global__ void k_count(const float4 *rad, uint64_t n, const uint64_t *off, uint32_t n_inst, uint32_t K1,
                         uint32_t *kept, uint32_t *key, uint32_t *cnt, uint32_t *h2)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint32_t pid; int sp;
-    uint32_t kk = 0, kf = 0;
-    if (decode_w(rad[i].w, pid, sp) && sp != 4) {
-        uint32_t inst = find_inst(off, n_inst, i);
-        kk = inst * K1 + pid;
-        if (sp == 7) atomicAdd(&h2[kk], 1u);
-        else { kf = 1; atomicAdd(&cnt[kk], 1u); }
+    uint32_t kk = 0, cat = 0;        // cat: 0 skipped, 1 particle, 2 H2 marker
+    if (i < n) {
+        uint32_t pid; int sp;
+        if (decode_w(rad[i].w, pid, sp) && sp != 4) {
+            uint32_t inst = find_inst(off, n_inst, i);
+            kk = inst * K1 + pid;
+            cat = sp == 7 ? 2u : 1u;
+        }
+        kept[i] = cat == 1u;
+        key[i] = kk;
     }
-    kept[i] = kf;
-    key[i] = kk;
+    uint64_t tag = i < n ? (((uint64_t)kk << 2) | cat) : ~0ull;
+    uint32_t len = run_length_if_head(tag);
+    if (len && i < n && cat) atomicAdd(cat == 2u ? &h2[kk] : &cnt[kk], len);
 }
 
 __global__ void k_first(const uint32_t *kept, const uint32_t *S, const uint32_t *key, uint64_t n,
                         uint32_t *ck, uint32_t *firstj)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n || !kept[i]) return;
-    uint32_t j = S[i];
-    ck[j] = key[i];
-    atomicMin(&firstj[key[i]], j);
+    bool on = i < n && kept[i];
+    uint32_t k = on ? key[i] : 0;
+    uint32_t j = on ? S[i] : 0;
+    if (on) ck[j] = k;
+    uint64_t tag = on ? (uint64_t)k : (~0ull - (i & 1ull));
+    uint32_t len = run_length_if_head(tag);
+    if (len && on) atomicMin(&firstj[k], j);
 }
 
 __global__ void k_changes(const uint32_t *ck, uint32_t m, uint32_t *changes)
@@ -277,6 +301,7 @@ struct TrackShared {
     uint32_t npend;              // queued stage-2 (SamplePDC) pairs
     uint32_t pend_next;          // drain cursor
     uint32_t long_n, long_next;  // pairs handed from the per-lane to the wavefront schedule
+    uint32_t llen[6];            // alive particles per species list
     uint32_t long_item[512];
     uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
     int      bri[9], nblk;                // channel blocks: reaction, units per row
@@ -444,42 +469,66 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, Trac
     }
 }
 
-// pair particle `idx` (a new product born at t_new) with all alive particles < n_total
-__device__ void pair_with_alive(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t idx,
-                                uint32_t n_total, double t_new, TrackShared &sh, unsigned long long &wrep)
+// Re-pair a new product `idx` (born at t_new) with every alive particle that has a
+// reaction channel with it (irt-worker.js:430-465).  Alive particles are kept in
+// per-species lists, so only partner species with a channel are visited and every
+// lane of a pass evaluates the same reaction.  `skip` excludes the second product of
+// a two-product reaction while the first is paired (it does not exist yet at that point
+// of the worker's loop, irt-worker.js:618-644).
+__device__ void pair_with_species(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t idx,
+                                  uint32_t skip, double t_new, TrackShared &sh, unsigned long long &wrep)
 {
-    const IrtConst &C = c_irt;
     const float4 *P = A.pos + T.part_off;
-    const uint8_t *al = A.alive + T.part_off;
     const double *tb = A.tb + T.part_off;
-    float4 pi = P[idx];
-    int si = (int)pi.w;
-    double Di = C.D[si];
-    for (uint32_t jj = threadIdx.x; jj < n_total; jj += IRT_BLOCK) {
-        if (jj == idx || !al[jj]) continue;
-        float4 pj = P[jj];
-        int sj = (int)pj.w;
-        int ri = C.rxnMap[si * 6 + sj];
+    const uint32_t *lst = A.lst + 6 * T.part_off;
+    const float4 pi = P[idx];
+    const int si = __builtin_amdgcn_readfirstlane((int)pi.w);
+    const double RC2 = c_irt.R_CUT2;
+    for (int sj = 0; sj < 6; sj++) {
+        const int ri = c_irt.rxnMap[si * 6 + sj];
         if (ri < 0) continue;
-        wrep++;
-        double Dj = C.D[sj];
-        uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
-        uint64_t ps = dna_pair_state(tk, a, b);
-        double dt = t_new - tb[jj];
-        double xj = pj.x, yj = pj.y, zj = pj.z;
-        if (dt > 0) {
-            double sig = sqrt(2.0 * Dj * dt);
-            double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
-            double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
-            double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
-            xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
+        const uint32_t len = __builtin_amdgcn_readfirstlane(sh.llen[sj]);
+        const RxK k = rx_constants(ri, c_irt.D[si] + c_irt.D[sj]);
+        const double Dj = c_irt.D[sj];
+        const uint32_t *L = lst + (size_t)sj * T.cap;
+        for (uint32_t pos = threadIdx.x; pos < len; pos += IRT_BLOCK) {
+            uint32_t jj = L[pos];
+            if (jj == idx || jj == skip) continue;
+            wrep++;
+            float4 pj = P[jj];
+            uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
+            uint64_t ps = dna_pair_state(tk, a, b);
+            double dt = t_new - tb[jj];
+            double xj = pj.x, yj = pj.y, zj = pj.z;
+            if (dt > 0) {
+                double sig = sqrt(2.0 * Dj * dt);
+                double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
+                double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
+                double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
+                xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
+            }
+            double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
+            double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (r2 > RC2) continue;
+            double r0 = sqrt(r2);
+            if (may_react_k(k, r0, ps)) push_pending(A, T, sh, a, b, r0);
         }
-        double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
-        double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        if (r2 > C.R_CUT2) continue;
-        double r0 = sqrt(r2);
-        if (irt_may_react(r0, ri, Di + Dj, ps)) push_pending(A, T, sh, a, b, r0);
     }
+}
+
+// species-list maintenance (one lane): swap-remove and append
+__device__ __forceinline__ void list_remove(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, int s, uint32_t k)
+{
+    uint32_t *L = A.lst + 6 * T.part_off + (size_t)s * T.cap;
+    uint32_t *slot = A.slot + T.part_off;
+    uint32_t pos = slot[k], last = L[--sh.llen[s]];
+    L[pos] = last; slot[last] = pos;
+}
+__device__ __forceinline__ void list_add(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, int s, uint32_t k)
+{
+    uint32_t *L = A.lst + 6 * T.part_off + (size_t)s * T.cap;
+    uint32_t pos = sh.llen[s]++;
+    L[pos] = k; A.slot[T.part_off + k] = pos;
 }
 
 __device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh, int c,
@@ -662,6 +711,18 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             }
         }
     }
+    // species lists of the alive initial particles = the species groups of Sg
+    for (uint32_t g = tid; g < n; g += IRT_BLOCK) {
+        int spc = 0;
+        while (spc < 5 && g >= sh.goff[spc + 1]) spc++;
+        if (g < sh.goff[spc] + sh.gcnt[spc]) {
+            uint32_t pos = g - sh.goff[spc];
+            uint32_t k = (uint32_t)__float_as_int(Sg[g].w);
+            A.lst[6 * T.part_off + (size_t)spc * T.cap + pos] = k;
+            A.slot[T.part_off + k] = pos;
+        }
+    }
+    if (tid < 6) sh.llen[tid] = sh.gcnt[tid];
     __syncthreads();
     unsigned long long tpl = wall_clock64();
     pdc_drain(A, T, tk, sh, 0.0);
@@ -670,6 +731,14 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     __syncthreads();
     unsigned long long n_init_ev = sh.n_ev;
     int cp = 0;
+    // the event loop reads alive flags at random: keep them in LDS, where the sorted
+    // copy of the pair phase lived
+    if ((size_t)T.cap <= (size_t)A.lds_slots * 16u) {
+        uint8_t *la = (uint8_t *)dyn_lds;
+        for (uint32_t k = tid; k < T.cap; k += IRT_BLOCK) la[k] = k < n ? al[k] : 0;
+        __syncthreads();
+        al = la;
+    }
     tp2 = wall_clock64();
 
     // 3. event loop
@@ -680,8 +749,8 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         double bt = 1e300; uint64_t bab = ~0ull;
         for (uint32_t e = tid; e < ne; e += IRT_BLOCK) {
             uint2 ab = A.ev_ab[T.ev_off + e];
-            if (!al[ab.x] || !al[ab.y]) continue;
             double t = A.ev_t[T.ev_off + e];
+            if (!al[ab.x] || !al[ab.y]) continue;
             uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
             if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
         }
@@ -704,6 +773,8 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
                 int si = (int)P[a].w, sj = (int)P[b].w;
                 int ri = C.rxnMap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
                 al[a] = 0; al[b] = 0;
+                list_remove(A, T, sh, si, a);
+                list_remove(A, T, sh, sj, b);
                 sh.n_re++; sh.rxc[ri]++;
                 if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
                 if (sj == 0) sh.cnt[0]--; else if (sj == 1) sh.cnt[1]--; else if (sj == 2) sh.cnt[2]--; else if (sj == 4) sh.cnt[3]--;
@@ -728,6 +799,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
                         else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
                         else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
                         P[id] = q; al[id] = 1; tb[id] = t0;
+                        list_add(A, T, sh, ps, id);
                         if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
                         sh.prod_idx[made++] = id;
                         nt++;
@@ -745,7 +817,8 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         uint32_t base_total = sh.n_total - (uint32_t)np;
         for (int pp = 0; pp < np; pp++) {
             // product pp is paired with everything alive up to and including itself
-            pair_with_alive(A, T, tk, sh.prod_idx[pp], base_total + (uint32_t)pp + 1u, sh.win_t, sh, wrep);
+            uint32_t skip = (pp == 0 && np == 2) ? sh.prod_idx[1] : 0xFFFFFFFFu;
+            pair_with_species(A, T, tk, sh.prod_idx[pp], skip, sh.win_t, sh, wrep);
         }
         __syncthreads();
         if (sh.npend) {
@@ -962,7 +1035,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
             t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
             t.ev_off = evs; evs += t.ev_cap;
         }
-        size_t bytes_part = part * (16 + 16 + 1 + 8);
+        size_t bytes_part = part * (16 + 16 + 8 + 4 * 7 + 1);
         DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch,
                             std::max<size_t>(bytes_part + 256, rad_n / 1024 * 4 + 64)));
         DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + 256));
@@ -983,7 +1056,8 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         IrtArgs A;
         char *sp = (char *)c->irt.scratch;
         A.rad = d_rad; A.idx = d_idx; A.tracks = d_tr;
-        A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32); A.alive = (uint8_t *)(sp + part * 40);
+        A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
+        A.lst = (uint32_t *)(sp + part * 40); A.slot = (uint32_t *)(sp + part * 64); A.alive = (uint8_t *)(sp + part * 68);
         A.ev_t = (double *)c->irt.events; A.ev_ab = (uint2 *)((char *)c->irt.events + evs * 8);
         A.pend = (uint4 *)((char *)c->irt.events + evs * 16);
         A.tally = d_tally; A.overflow = g.misc + 2;
