@@ -34,6 +34,99 @@ def chain_step(ctx, p, dna, seed):
     return t, c, s, n_therm
 
 
+def chain_step_surrogate(ctx, p, dna, surr, sigma_max, seed):
+    """The same chain with the surrogate's chemistry (IRT for the tracks it flags)."""
+    import numpy as np
+    from dna_amd import surrogate as S
+    from dna_amd.abi import DNA_COPY_RESULTS
+    t = ctx.transport(p, copy=DNA_COPY_RESULTS)
+    E = np.full(p.n, p.be, np.float32)
+    r = S.surrogate_chemistry(ctx, surr, t["rad_n"], E, sigma_max, seed=seed)
+    s = ctx.score(dna, None, t["rad_n"], lcg_state=0x12345678)
+    return t, r, s
+
+
+def surrogate_leg(ctx, dna, p10, G_irt_1us, steps, log):
+    """Learned chemistry: 200k one-primary tracks (E log-uniform 1-20 keV) from this GPU
+    code, 95% for training on the GPU and 5% held out; then a 65,536-track inference
+    batch timed against plain IRT, and the 10 keV chain with surrogate chemistry."""
+    import numpy as np
+    from dna_amd import Surrogate
+    from dna_amd import surrogate as S
+    tc = S.checkpoints_16()
+    out = {}
+    s = Surrogate(ctx, 70, 256, 3, 80, seed=1)
+    t0 = time.perf_counter()
+    d = S.generate(ctx, s, 190_000, seed=S.E10_IRT_SEED, chunk=16384)
+    t1 = time.perf_counter()
+    log(f"surrogate: generated 190000 training tracks in {t1 - t0:.1f} s")
+    losses = S.fit(s, d["X"], d["G"])
+    t2 = time.perf_counter()
+    log(f"surrogate: trained in {t2 - t1:.1f} s, losses {losses}")
+    frac = 0.1
+    sigma_max = S.calibrate_sigma(s, d["X"], frac)
+    out.update({"train_tracks": int(d["X"].shape[0]), "gen_s": round(t1 - t0, 2),
+                "train_s": round(t2 - t1, 2), "nll": [round(x, 4) for x in losses],
+                "sigma_max": round(sigma_max, 4), "fallback_target": frac})
+
+    def batch(n, seed, E_lo=1000.0, E_hi=20000.0):
+        E = S.log_uniform_energies(n, seed, E_lo, E_hi)
+        t = ctx.transport(S.transport_params(n, seed=seed & 0xFFFFFFFF), energies=E, copy=0)
+        return E, t
+
+    # held-out 5%: 10,000 tracks from another seed stream
+    E, t = batch(10_000, 0x5EED0005)
+    o, tt = ctx.irt_tracks(None, t["rad_n"], E.size, E.size, 1.0, tc, seed=7)
+    G_ref = S.aggregate_G(S.per_track_G(tt, E), E)
+    r = S.surrogate_chemistry(ctx, s, t["rad_n"], E, sigma_max, seed=7)
+    err = S.g_error(r["G"], G_ref)
+    out["held_out"] = {"tracks": int(E.size), "g_err_max": round(err["max_rel"], 4),
+                       "g_err_mean": round(err["mean_rel"], 4), "outputs_compared": err["n_outputs"],
+                       "fallback": r["n_fallback"]}
+    log(f"surrogate: held-out error {err}, fallback {r['n_fallback']}")
+    # inference batch of 65,536 tracks: device time of plain IRT vs surrogate + fallback IRT
+    E, t = batch(65_536, 0x5EED0065)
+    irt_ms, sur_ms, pred_ms, fb_ms, nfb = [], [], [], [], 0
+    for k in range(max(2, min(steps, 5))):
+        o, tt = ctx.irt_tracks(None, t["rad_n"], E.size, E.size, 1.0, tc, seed=11)
+        r = S.surrogate_chemistry(ctx, s, t["rad_n"], E, sigma_max, seed=11)
+        irt_ms.append(o.t_ms)
+        sur_ms.append(r["t_predict_ms"] + r["t_irt_ms"])
+        pred_ms.append(r["t_predict_ms"])
+        fb_ms.append(r["t_irt_ms"])
+        nfb = r["n_fallback"]
+    G_ref = S.aggregate_G(S.per_track_G(tt, E), E)
+    err = S.g_error(r["G"], G_ref)
+    irt_m, sur_m = float(np.median(irt_ms)), float(np.median(sur_ms))
+    no_pad = (70 + 31) // 32 * 32
+    flops = 2.0 * E.size * (no_pad * 256 + 2 * 256 * 256 + 256 * 160)
+    out["inference"] = {"tracks": int(E.size), "irt_ms": round(irt_m, 3),
+                        "surrogate_ms": round(sur_m, 3),
+                        "predict_ms": round(float(np.median(pred_ms)), 3),
+                        "fallback_irt_ms": round(float(np.median(fb_ms)), 3), "fallback": int(nfb),
+                        "speedup": round(irt_m / sur_m, 2), "g_err_max": round(err["max_rel"], 4),
+                        "g_err_mean": round(err["mean_rel"], 4),
+                        "network_bf16_gflop": round(flops / 1e9, 2)}
+    log(f"surrogate: inference {out['inference']}")
+    # the 10 keV chain with surrogate chemistry
+    times = []
+    for k in range(steps + 1):
+        a = time.perf_counter()
+        tr, rr, sc = chain_step_surrogate(ctx, p10, dna, s, sigma_max, seed=1)
+        if k:
+            times.append(time.perf_counter() - a)
+    G1 = rr["G"].reshape(16, 5)[15]
+    rel = [abs(G1[i] - G_irt_1us[i]) / G_irt_1us[i] for i in range(5) if G_irt_1us[i] >= 0.05]
+    out["chain_10keV"] = {"primaries_per_s": round(N_PRIM * len(times) / sum(times), 2),
+                          "ms_per_step": round(1e3 * sum(times) / len(times), 3),
+                          "fallback": rr["n_fallback"],
+                          "G_1us": [round(float(x), 4) for x in G1],
+                          "g_err_1us_vs_irt_chain": round(max(rel), 4)}
+    log(f"surrogate: chain {out['chain_10keV']}")
+    s.close()
+    return out
+
+
 def cpu_baseline(sample_prims, threads):
     """The CPU restatement (oracle/) timed on this host: the same chain on a bounded
     sample of the same workload (primaries 0..sample-1 of the 10 keV run)."""
@@ -62,10 +155,11 @@ def cpu_baseline(sample_prims, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU scaling is not part of it "
@@ -94,6 +188,8 @@ def main():
         print(f"step {s + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms", file=sys.stderr, flush=True)
     total = time.perf_counter() - t_all0
     ms_per_step = 1e3 * total / args.steps
+    times.sort()
+    median_ms = 1e3 * times[len(times) // 2]
     value = N_PRIM * args.steps / total
     t, c, sc, n_therm = last
     # roofline of the dominant kernel (IRT, one workgroup per track): algorithmic FP64
@@ -110,6 +206,7 @@ def main():
         "metric": "10 keV primaries/s, full chain; surrogate vs IRT speedup at <=3% G error",
         "value": round(value, 2), "unit": "10 keV primaries/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "median_ms_per_step": round(median_ms, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (reference geometry and seeding, no external data)",
         "config": {"workload": "full reference chain: 4096 primaries at 10 keV, transport + prechemistry, "
@@ -124,6 +221,12 @@ def main():
                         "G_1us": [round(c.G[6][s], 4) for s in range(5)],
                         "ssb_dir": sc["ssb_count"], "dsb": sc["dsb"]},
     }
+    if not args.no_surrogate:
+        def log(m):
+            print(m, file=sys.stderr, flush=True)
+        G1 = [c.G[6][s] for s in range(5)]
+        out["surrogate"] = surrogate_leg(ctx, dna, p, G1, args.steps, log)
+        out["surrogate_speedup"] = out["surrogate"]["inference"]["speedup"]
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)

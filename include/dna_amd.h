@@ -191,6 +191,56 @@ int dna_hash_cells(struct dna_ctx *ctx, const float *pos4, uint64_t n, float cel
                    uint32_t hash_bits, uint32_t *key_out, uint32_t *cell_start_out,
                    uint32_t *order_out);
 
+/* Phase C on a subset of tracks with per-track output (the reference runs every track;
+ * this is the split the surrogate fallback needs).  Same semantics as dna_run_irt, plus:
+ * select[pid] != 0 (pid < n_prim) picks the primaries whose tracks run (NULL = all); a
+ * selected track's history is the one it has in the full run.  track_tally (host,
+ * [n_prim][n_ckpt][5], may be NULL) receives per-primary species counts; with it, only
+ * primaries pid < n_prim run.  out aggregates the tracks that ran. */
+int dna_run_irt_tracks(struct dna_ctx *ctx, const float *rad, uint64_t rad_n, uint32_t n_therm,
+                       float E_eV, const uint8_t *select, uint32_t n_prim, const double *t_ckpt,
+                       int n_ckpt, uint64_t seed, uint32_t *track_tally, dna_chem_out *out);
+
+/* ------------------------------------------------- learned chemistry surrogate
+ * No reference counterpart.  An MLP n_in -> n_hidden (x n_layers, ReLU) -> 2*n_out in
+ * BF16 on the matrix cores with FP32 accumulation; outputs are a mean and a log-variance
+ * per target.  Parameters are one flat FP32 vector: for each layer l the weights
+ * [din_l][dout_l] (row-major; din_0 = n_in rounded up to a multiple of 32, padded rows
+ * zero), then the biases of every layer.  Inputs and targets are normalised on the
+ * device with dna_surrogate_set_norm (identity by default). */
+typedef struct dna_surrogate dna_surrogate;
+
+int  dna_surrogate_create(struct dna_ctx *ctx, int n_in, int n_hidden, int n_layers, int n_out,
+                          uint64_t seed, dna_surrogate **out);
+void dna_surrogate_destroy(dna_surrogate *s);
+int  dna_surrogate_shape(const dna_surrogate *s, int *n_in, int *n_hidden, int *n_layers,
+                         int *n_out, uint64_t *n_params);
+int  dna_surrogate_get_params(dna_surrogate *s, float *params);
+int  dna_surrogate_set_params(dna_surrogate *s, const float *params);
+int  dna_surrogate_set_norm(dna_surrogate *s, const float *x_mean, const float *x_std,
+                            const float *y_mean, const float *y_std);
+/* raw outputs Y [B][2*n_out] = (normalised mean, log-variance) for X [B][n_in] */
+int  dna_surrogate_forward(dna_surrogate *s, const float *X, int B, float *Y);
+/* mean Gaussian NLL and its gradient w.r.t. every parameter (flat layout) on one batch */
+int  dna_surrogate_grad(dna_surrogate *s, const float *X, const float *T, int B,
+                        double *loss, float *grad);
+/* Adam, shuffled mini-batches; *mean_loss = mean NLL over the last epoch */
+int  dna_surrogate_train(dna_surrogate *s, const float *X, const float *T, int N, int batch,
+                         int epochs, float lr, float weight_decay, uint64_t seed,
+                         double *mean_loss);
+/* Per-track features [n][70]: 64-bin histogram of distance from the track centroid (log
+ * bins 0.5 nm .. 20 um, fractions), initial OH, e-aq, H, H3O+, H2 per 100 eV, log10(E/keV).
+ * rad == NULL: track p = primary p of the last dna_run_transport (n must equal its n). */
+int  dna_track_features(dna_surrogate *s, const float *rad, const uint64_t *inst_off,
+                        uint32_t n, const float *E, float *F);
+/* Surrogate chemistry for the n primaries of the last dna_run_transport: mean and sigma
+ * per target ([n][n_out], may be NULL) and fallback[p] = 1 where a sigma exceeds
+ * sigma_max; those tracks go to dna_run_irt_tracks.  t_ms (may be NULL): device ms of
+ * features, network, whole call. */
+int  dna_surrogate_predict(dna_surrogate *s, const float *E, uint32_t n, float sigma_max,
+                           float *mu, float *sigma, uint8_t *fallback, uint32_t *n_fallback,
+                           float *t_ms);
+
 #ifdef __cplusplus
 }
 #endif

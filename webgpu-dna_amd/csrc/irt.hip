@@ -62,6 +62,7 @@ struct IrtArgs {
     uint32_t *slot;               // position of each particle in its species list
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
+    uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
     uint32_t *overflow;
     double tc[DNA_MAX_CKPT];
     int nc;
@@ -847,6 +848,8 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         while (cp < A.nc) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE;
         for (int i = 0; i < A.nc * 5; i++) if (s_loc[i]) atomicAdd(&g[i], s_loc[i]);
+        if (A.track_tally)
+            for (int i = 0; i < A.nc * 5; i++) A.track_tally[(size_t)blockIdx.x * A.nc * 5 + i] = (uint32_t)s_loc[i];
         for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
@@ -928,9 +931,14 @@ struct GroupScratch {
     uint32_t *kept, *key, *S, *ck, *cnt, *h2, *firstj, *bstart, *cursor, *idx, *misc;
 };
 
+// sel (optional, instance 0 only): run only the tracks of primaries pid < n_sel with
+// sel[pid] != 0; every other track is grouped as usual (capacities and RNG keys are those
+// of the full run, so a selected track's history is the one the full run gives it).
+// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
 static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
                           const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
-                          dna_chem_out *outs)
+                          dna_chem_out *outs, const uint8_t *sel = nullptr, uint32_t n_sel = 0,
+                          uint32_t *track_tally = nullptr)
 {
     hipStream_t st = c->stream;
     uint32_t n_inst = (uint32_t)off.size() - 1;
@@ -1006,9 +1014,10 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         if (cnt[k]) {
             IrtTrack t; memset(&t, 0, sizeof(t));
             t.inst = (uint32_t)(k / K1); t.pid = (uint32_t)(k % K1); t.start = run; t.n = cnt[k]; t.h2 = h2[k];
-            tr.push_back(t);
             if (cnt[k] > maxN[t.inst]) maxN[t.inst] = cnt[k];
             run += cnt[k];
+            if (sel && !(t.inst == 0 && t.pid < n_sel && sel[t.pid])) continue;
+            tr.push_back(t);
         }
     }
     bstart[K] = run;
@@ -1041,12 +1050,15 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + 256));
         // layout of the track/bucket block inside irt.tally
         size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
-        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, tally_bytes + ntr * sizeof(IrtTrack) + (K + 1) * 4 + (size_t)run * 4 + 1024));
+        size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
+        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally,
+                            tally_bytes + ntr * sizeof(IrtTrack) + (K + 1) * 4 + (size_t)run * 4 + tt_bytes + 1280));
         char *tbase = (char *)c->irt.tally;
         unsigned long long *d_tally = (unsigned long long *)tbase;
         IrtTrack *d_tr = (IrtTrack *)(tbase + ((tally_bytes + 255) & ~(size_t)255));
         uint32_t *d_bst = (uint32_t *)((char *)d_tr + ((ntr * sizeof(IrtTrack) + 255) & ~(size_t)255));
         uint32_t *d_idx = d_bst + ((K + 1 + 63) & ~(uint64_t)63);
+        uint32_t *d_tt = track_tally ? d_idx + (((uint64_t)run + 63) & ~(uint64_t)63) : nullptr;
         DNA_HIP(hipMemsetAsync(d_tally, 0, tally_bytes, st));
         DNA_HIP(hipMemcpyAsync(d_tr, tr.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
         DNA_HIP(hipMemcpyAsync(d_bst, bstart.data(), (K + 1) * 4, hipMemcpyHostToDevice, st));
@@ -1060,7 +1072,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         A.lst = (uint32_t *)(sp + part * 40); A.slot = (uint32_t *)(sp + part * 64); A.alive = (uint8_t *)(sp + part * 68);
         A.ev_t = (double *)c->irt.events; A.ev_ab = (uint2 *)((char *)c->irt.events + evs * 8);
         A.pend = (uint4 *)((char *)c->irt.events + evs * 16);
-        A.tally = d_tally; A.overflow = g.misc + 2;
+        A.tally = d_tally; A.track_tally = d_tt; A.overflow = g.misc + 2;
         for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
         A.nc = nc; A.seed = seed; A.sort_tracks = contiguous ? 0 : 1;
         DNA_HIP(hipEventRecord(c->ev[7], st));
@@ -1076,10 +1088,18 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         DNA_HIP(hipMemcpyAsync(&ovf, g.misc + 2, 4, hipMemcpyDeviceToHost, st));
         std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE);
         DNA_HIP(hipMemcpyAsync(ht.data(), d_tally, tally_bytes, hipMemcpyDeviceToHost, st));
+        std::vector<uint32_t> htt(track_tally ? (size_t)ntr * nc * 5 : 0);
+        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), d_tt, tt_bytes, hipMemcpyDeviceToHost, st));
         DNA_HIP(hipEventRecord(c->ev[6], st));
         DNA_HIP(hipStreamSynchronize(st));
         if (ovf & 2u) { ev_mult *= 4; continue; }
         if (ovf & 1u) { continue; }
+        if (track_tally) {
+            memset(track_tally, 0, (size_t)n_sel * nc * 5 * 4);
+            for (uint32_t j = 0; j < ntr; j++)
+                if (tr[j].inst == 0 && tr[j].pid < n_sel)
+                    memcpy(track_tally + (size_t)tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
+        }
         float ms = 0, kms = 0;
         hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
         hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
@@ -1153,4 +1173,24 @@ extern "C" int dna_run_irt_batch(dna_ctx *c, const float *rad, const uint64_t *i
     int rc = stage_rad(c, rad + 4 * inst_off[0], rad_n, &d_rad);
     if (rc) return rc;
     return run_irt_device(c, d_rad, rad_n, off, n_therm, E_eV, t_ckpt, n_ckpt, seed, out);
+}
+
+extern "C" int dna_run_irt_tracks(dna_ctx *c, const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                                  const uint8_t *select, uint32_t n_prim, const double *t_ckpt, int n_ckpt,
+                                  uint64_t seed, uint32_t *track_tally, dna_chem_out *out)
+{
+    if (!c || !out || !t_ckpt || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT || ((select || track_tally) && n_prim == 0)) {
+        set_error("dna_run_irt_tracks: bad arguments"); return DNA_ERR_ARG;
+    }
+    DNA_HIP(hipSetDevice(c->device));
+    const float4 *d_rad = nullptr;
+    int rc = stage_rad(c, rad, rad_n, &d_rad);
+    if (rc) return rc;
+    if (track_tally && rad_n == 0) memset(track_tally, 0, (size_t)n_prim * n_ckpt * 5 * 4);
+    std::vector<uint64_t> off = {0, rad_n};
+    std::vector<uint8_t> all;
+    const uint8_t *sel = select;
+    if (!sel && track_tally) { all.assign(n_prim, 1); sel = all.data(); }
+    return run_irt_device(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, seed, out, sel,
+                          sel ? n_prim : 0, track_tally);
 }

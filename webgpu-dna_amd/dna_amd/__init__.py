@@ -4,4 +4,4 @@ runValidation).  The compute runs in libdna_amd.so (HIP, gfx950); this package i
 host-side mirror of the reference's TypeScript host."""
 from .abi import DnaParams  # noqa: F401
 from .geometry import DNATarget, default_params  # noqa: F401
-from .lib import Context, DnaError, lib  # noqa: F401
+from .lib import Context, DnaError, Surrogate, lib  # noqa: F401

@@ -42,13 +42,33 @@ def lib():
     L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
                             C.POINTER(DnaScoreOut)]
     L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
+    L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
+                                     C.POINTER(DnaChemOut)]
+    ip = C.POINTER(C.c_int)
+    L.dna_surrogate_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, u64, C.POINTER(vp)]
+    L.dna_surrogate_destroy.argtypes = [vp]
+    L.dna_surrogate_destroy.restype = None
+    L.dna_surrogate_shape.argtypes = [vp, ip, ip, ip, ip, C.POINTER(u64)]
+    L.dna_surrogate_get_params.argtypes = [vp, vp]
+    L.dna_surrogate_set_params.argtypes = [vp, vp]
+    L.dna_surrogate_set_norm.argtypes = [vp, vp, vp, vp, vp]
+    L.dna_surrogate_forward.argtypes = [vp, vp, C.c_int, vp]
+    L.dna_surrogate_grad.argtypes = [vp, vp, vp, C.c_int, C.POINTER(C.c_double), vp]
+    L.dna_surrogate_train.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                      u64, C.POINTER(C.c_double)]
+    L.dna_track_features.argtypes = [vp, vp, vp, u32, vp, vp]
+    L.dna_surrogate_predict.argtypes = [vp, vp, u32, C.c_float, vp, vp, vp, C.POINTER(u32), vp]
     _lib = L
     return L
 
 
 EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_version",
                     "dna_run_transport", "dna_run_irt", "dna_run_irt_batch", "dna_score",
-                    "dna_hash_cells"]
+                    "dna_hash_cells", "dna_run_irt_tracks", "dna_surrogate_create",
+                    "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
+                    "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
+                    "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
+                    "dna_surrogate_predict"]
 
 
 def _check(fn, rc):
@@ -129,6 +149,24 @@ class Context:
                                                             seed, outs))
         return list(outs)
 
+    def irt_tracks(self, rad, rad_n, n_prim, n_therm, E_eV, t_ckpt, seed=1, select=None,
+                   per_track=True):
+        """IRT on the tracks of primaries pid < n_prim (those with select[pid] != 0 when given).
+        rad: (n,4) float32 array, or None for the rad_n device-resident radicals.
+        Returns (aggregate DnaChemOut, per-primary species counts [n_prim, n_ckpt, 5] or None)."""
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        sel = None if select is None else np.ascontiguousarray(select, np.uint8)
+        if sel is not None and sel.size != n_prim:
+            raise ValueError("select must have n_prim entries")
+        tc = np.ascontiguousarray(t_ckpt, np.float64)
+        tt = np.zeros((n_prim, tc.size, 5), np.uint32) if per_track else None
+        o = DnaChemOut()
+        n = rad_n if r is None else r.shape[0]
+        _check("dna_run_irt_tracks", lib().dna_run_irt_tracks(
+            self.h, _ptr(r), n, n_therm, E_eV, _ptr(sel), n_prim, _ptr(tc), tc.size, seed,
+            _ptr(tt), C.byref(o)))
+        return o, tt
+
     # ---- scoring
     def score(self, target, rad=None, rad_n=0, lcg_state=0x12345678, want_hits=False):
         r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
@@ -156,3 +194,116 @@ class Context:
         _check("dna_hash_cells", lib().dna_hash_cells(self.h, _ptr(p), n, cell_nm, bits, _ptr(key),
                                                       _ptr(start), _ptr(order)))
         return key, start, order
+
+
+class Surrogate:
+    """dna_surrogate: the BF16 MLP chemistry surrogate on the context's GPU."""
+
+    def __init__(self, ctx, n_in=70, n_hidden=256, n_layers=3, n_out=80, seed=1):
+        h = C.c_void_p()
+        _check("dna_surrogate_create", lib().dna_surrogate_create(ctx.h, n_in, n_hidden, n_layers,
+                                                                  n_out, seed, C.byref(h)))
+        self.h, self.ctx = h, ctx
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        npar = C.c_uint64()
+        _check("dna_surrogate_shape", lib().dna_surrogate_shape(h, C.byref(a), C.byref(b), C.byref(c),
+                                                                C.byref(d), C.byref(npar)))
+        self.n_in, self.n_hidden, self.n_layers, self.n_out = a.value, b.value, c.value, d.value
+        self.n_params = npar.value
+        self.n_in_pad = (self.n_in + 31) // 32 * 32
+
+    def close(self):
+        if self.h:
+            lib().dna_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def layer_dims(self):
+        """[(din, dout)] per layer, as laid out in the flat parameter vector."""
+        dims = []
+        for l in range(self.n_layers + 1):
+            di = self.n_in_pad if l == 0 else self.n_hidden
+            do = 2 * self.n_out if l == self.n_layers else self.n_hidden
+            dims.append((di, do))
+        return dims
+
+    def unflatten(self, flat):
+        """Split a flat parameter vector into ([W_l (din, dout)], [b_l (dout,)])."""
+        Ws, bs, off = [], [], 0
+        for di, do in self.layer_dims():
+            Ws.append(flat[off:off + di * do].reshape(di, do))
+            off += di * do
+        for di, do in self.layer_dims():
+            bs.append(flat[off:off + do])
+            off += do
+        return Ws, bs
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        _check("dna_surrogate_get_params", lib().dna_surrogate_get_params(self.h, _ptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError("parameter vector has the wrong size")
+        _check("dna_surrogate_set_params", lib().dna_surrogate_set_params(self.h, _ptr(p)))
+
+    def set_norm(self, x_mean, x_std, y_mean, y_std):
+        a = [np.ascontiguousarray(v, np.float32) for v in (x_mean, x_std, y_mean, y_std)]
+        _check("dna_surrogate_set_norm", lib().dna_surrogate_set_norm(self.h, *[_ptr(v) for v in a]))
+        self.norm = a
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.empty((X.shape[0], 2 * self.n_out), np.float32)
+        _check("dna_surrogate_forward", lib().dna_surrogate_forward(self.h, _ptr(X), X.shape[0], _ptr(Y)))
+        return Y
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_double()
+        _check("dna_surrogate_grad", lib().dna_surrogate_grad(self.h, _ptr(X), _ptr(T), X.shape[0],
+                                                              C.byref(loss), _ptr(g)))
+        return loss.value, g
+
+    def train(self, X, T, batch=512, epochs=10, lr=1e-3, weight_decay=0.0, seed=1):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = C.c_double()
+        _check("dna_surrogate_train", lib().dna_surrogate_train(self.h, _ptr(X), _ptr(T), X.shape[0],
+                                                                batch, epochs, lr, weight_decay, seed,
+                                                                C.byref(loss)))
+        return loss.value
+
+    def features(self, E, rad=None, inst_off=None):
+        """Per-track features [n, 70]; rad None = the primaries resident from the last transport."""
+        Ee = np.ascontiguousarray(E, np.float32)
+        n = Ee.size
+        F = np.empty((n, 70), np.float32)
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        off = None if inst_off is None else np.ascontiguousarray(inst_off, np.uint64)
+        _check("dna_track_features", lib().dna_track_features(self.h, _ptr(r), _ptr(off), n, _ptr(Ee),
+                                                              _ptr(F)))
+        return F
+
+    def predict(self, E, sigma_max, want_values=True):
+        """Surrogate chemistry for the resident primaries: (mu, sigma, fallback, n_fallback, t_ms)."""
+        Ee = np.ascontiguousarray(E, np.float32)
+        n = Ee.size
+        mu = np.empty((n, self.n_out), np.float32) if want_values else None
+        sg = np.empty((n, self.n_out), np.float32) if want_values else None
+        fb = np.empty(n, np.uint8)
+        nf = C.c_uint32()
+        t = np.zeros(3, np.float32)
+        _check("dna_surrogate_predict", lib().dna_surrogate_predict(self.h, _ptr(Ee), n, sigma_max,
+                                                                    _ptr(mu), _ptr(sg), _ptr(fb),
+                                                                    C.byref(nf), _ptr(t)))
+        return mu, sg, fb, nf.value, t
