@@ -25,6 +25,21 @@ PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE summary
+    (profiles/chain_10keV_4096_hbm_traffic.txt, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "chain_10keV_4096_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 3 and w[0] == "traffic_bytes_per_launch" and w[1] == kernel:
+                    return int(w[2])
+    except OSError:
+        pass
+    return None
+
+
 def chain_step(ctx, p, dna, seed):
     from dna_amd.abi import DNA_COPY_RESULTS
     t = ctx.transport(p, copy=DNA_COPY_RESULTS)
@@ -160,7 +175,20 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (a plain copy kernel of known "
+                         "size that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_calib:
+        import torch
+        n = args.copy_calib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes": n * 4, "copies": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; multi-GPU scaling is not part of it "
               "(use --gpus 1)", file=sys.stderr)
@@ -199,7 +227,9 @@ def main():
     achieved = flops / (irt_ms * 1e-3) / 1e12
     roof = {"bound": "mfma", "kernel": "k_irt", "achieved": round(achieved, 3),
             "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic("k_irt"),
+            "traffic_source": "profiles/chain_10keV_4096_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated)",
             "note": "FP64 VALU kernel; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s",
             "avg_launch_ms": round(irt_ms, 4)}
     out = {
