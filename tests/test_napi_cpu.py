@@ -1,0 +1,74 @@
+"""The Node-API addon and the JavaScript host mirror, without a GPU: the addon loads and
+exports its functions, a missing device gives null (src/gpu/device.ts:12-21), argument
+errors throw, and buildDNATarget in V8 (the reference's JS engine) gives the same f32
+fiber arrays as the Python host bit for bit."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ADDON = os.path.join(ROOT, "webgpu-dna_amd", "lib", "dna_amd.node")
+HOST = os.path.join(ROOT, "webgpu-dna_amd", "js", "dna_host.js")
+NODE = shutil.which("node")
+
+pytestmark = pytest.mark.skipif(NODE is None or not os.path.exists(ADDON),
+                                reason="node or the built addon is missing")
+
+
+def _node(script, timeout=60):
+    r = subprocess.run([NODE, "-e", script], capture_output=True, text=True, timeout=timeout)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def _no_gpu():
+    try:
+        import torch
+        return torch.cuda.device_count() == 0
+    except Exception:
+        return True
+
+
+def test_addon_exports():
+    out = json.loads(_node(f"const a=require({json.dumps(ADDON)}); "
+                           "console.log(JSON.stringify({k:Object.keys(a), v:a.version()}))"))
+    assert set(out["k"]) == {"version", "lastError", "ctxCreate", "ctxDestroy", "runTransport", "runIrt",
+                             "score", "hashCells"}
+    assert "gfx950" in out["v"]
+
+
+@pytest.mark.skipif(not _no_gpu(), reason="a GPU is present")
+def test_no_device_is_null_not_faked():
+    out = json.loads(_node(f"const a=require({json.dumps(ADDON)}); const c=a.ctxCreate(0); "
+                           "console.log(JSON.stringify({c:c, e:a.lastError()}))"))
+    assert out["c"] is None and "device" in out["e"]
+    r = subprocess.run([NODE, os.path.join(ROOT, "webgpu-dna_amd", "js", "run_validation.js"), "--np", "4",
+                        "--energies", "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "no GPU device" in r.stderr
+
+
+def test_bad_context_throws():
+    out = _node(f"const a=require({json.dumps(ADDON)}); "
+                "try { a.runTransport({}, {n:1}); console.log('no throw'); } "
+                "catch (e) { console.log(e.constructor.name + ':' + e.message); }")
+    assert out.startswith("TypeError:") and "context" in out
+
+
+def test_dna_target_v8_matches_python_bits(tmp_path):
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "webgpu-dna_amd"))
+    from dna_amd import DNATarget
+    base = str(tmp_path)
+    _node(f"const h=require({json.dumps(HOST)}); const fs=require('fs'); const t=h.buildDNATarget(); "
+          f"for (const k of ['fy','fz','rbb0','rbb1']) fs.writeFileSync({json.dumps(base)}+'/'+k, Buffer.from(t[k].buffer)); "
+          f"fs.writeFileSync({json.dumps(base)}+'/meta.json', JSON.stringify({{n_bp:t.n_bp, x0:t.x0, n_bp_per:t.n_bp_per}}));")
+    t = DNATarget()
+    meta = json.load(open(os.path.join(base, "meta.json")))
+    assert meta == {"n_bp": t.n_bp, "x0": t.x0, "n_bp_per": t.n_bp_per}
+    for k in ("fy", "fz", "rbb0", "rbb1"):
+        js = np.fromfile(os.path.join(base, k), np.float32)
+        assert np.array_equal(js.view(np.uint32), getattr(t, k).view(np.uint32)), k

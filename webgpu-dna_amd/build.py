@@ -47,7 +47,32 @@ def build(verbose=True):
     os.replace(tmp, so)
     if verbose:
         print(f"built {so}")
+    addon = build_napi(verbose)
     return so
+
+
+NODE_INCLUDE = os.environ.get("DNA_NODE_INCLUDE", "/usr/include/node")
+
+
+def build_napi(verbose=True):
+    """Node-API addon lib/dna_amd.node over libdna_amd.so (skipped, with a message, when
+    the Node headers are absent)."""
+    if not os.path.exists(os.path.join(NODE_INCLUDE, "node_api.h")):
+        if verbose:
+            print(f"node_api.h not found under {NODE_INCLUDE}: Node addon not built")
+        return None
+    out = os.path.join(OUT, "dna_amd.node")
+    tmp = out + ".tmp"
+    cmd = ["gcc", "-O2", "-Wall", "-shared", "-fPIC", f"-I{NODE_INCLUDE}", "-DNODE_GYP_MODULE_NAME=dna_amd",
+           os.path.join(HERE, "napi", "dna_napi.c"), "-o", tmp, f"-L{OUT}", "-ldna_amd",
+           "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Node addon build failed:\n{r.stderr[-4000:]}")
+    os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":
