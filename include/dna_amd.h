@@ -123,9 +123,10 @@ typedef struct dna_chem_out {
      * the cut-off (reaction time sampled), product re-pairings with a Brownian step */
     uint64_t n_pair_cand, n_pair_sampled, n_repair;
     double   algo_flops;               /* FP64 operations by the static cost model */
-    uint64_t prof[8];                  /* per-phase 100 MHz clock sums over tracks: load,
-                                          initial pairs, event scan, reaction, re-pairing,
-                                          event-loop iterations */
+    uint64_t prof[8];                  /* per-phase 100 MHz clock sums over tracks: [0] load,
+                                          [1] initial pairs, [2] event scan, [3] reaction,
+                                          [4] re-pairing scan, [5] event-loop iterations,
+                                          [6] initial sampling drain, [7] re-pairing drain */
 } dna_chem_out;
 
 /* DNA fiber target, DNATarget of src/physics/types.ts:23-37 / buildDNATarget

@@ -33,6 +33,7 @@ struct IrtConst {
     double sigma[9], sigma_eff[9], rc[9], kdif[9], kact[9], prob[9], kobs[9];
     double D[6], sqrtD[6];
     double R_CUT2, sig_oh, sig_h, sig_eaq;
+    float R_CUT_F;                    // R_CUT rounded up to FP32 (a bound, not the test)
     int rxnMap[36];
     int type[9], nprod[9], prods[9][2];
 };
@@ -164,6 +165,35 @@ __device__ __forceinline__ bool may_react_k(const RxK &k, double r0, uint64_t ps
     return Winf > dna_u01(ps, 6u);
 }
 
+// Necessary condition for may_react_k, cheap enough to run on every candidate first.
+// With R = max(rc, 0), the Onsager distance satisfies r0e >= r0 - R (rc < 0: 1 - e^-x <= x
+// gives r0e >= r0; rc > 0: e^x - 1 <= x e^x gives r0e >= r0 - rc).  So a pair at
+// r0 - R > sc (sc = the contact radius sigma or sigma_eff) is not in contact, and it can
+// pass U < Winf only if (r0 - R) U < S, where S = sigma (TDC) or sigma_chk kobs/kdif (PDC).
+// The test runs in FP32 with relative margins of 1e-5, far above its rounding (< 1e-6),
+// on a lower bound of r0; a pair it rejects is one the exact FP64 test rejects too.
+struct RxB { float S, sc, R; };
+
+__device__ __forceinline__ RxB rx_bound(const RxK &k)
+{
+    RxB b;
+    double sg = (k.type == 0 || k.rc == 0) ? k.sigma : k.sigma_eff;
+    b.S = k.type == 0 ? (float)k.sigma : (float)(sg * k.kobs / k.kdif);
+    b.sc = (float)sg;
+    b.R = k.rc > 0 ? (float)k.rc : 0.f;
+    b.S *= 1.00001f; b.sc *= 1.00001f; b.R *= 1.00001f;
+    return b;
+}
+
+// rmin: a lower bound of the pair distance; U: the pair's draw 6 (0 = keep: a PDC pair with
+// U = 0 passes any Winf > 0)
+__device__ __forceinline__ bool cannot_react(const RxB &b, float rmin, double U)
+{
+    float a = rmin - b.R;
+    if (!(a > b.sc)) return false;
+    return a * (float)U * 0.99999f > b.S;
+}
+
 // SamplePDC parameters a, b (irt-worker.js:145-160)
 __device__ void pdc_ab(double r0, int ri, double D, double &a, double &b)
 {
@@ -289,6 +319,7 @@ __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint
 
 struct TrackShared {
     uint32_t n_ev;
+    uint32_t ev_lds;             // events [0, ev_lds) live in LDS (event loop), the rest in HBM
     uint32_t n_total;
     double   win_t;
     uint64_t win_ab;
@@ -300,6 +331,7 @@ struct TrackShared {
     unsigned long long rxc[9];   // reaction counts
     unsigned long long n_re;
     uint32_t npend;              // queued stage-2 (SamplePDC) pairs
+    uint32_t nkeep;              // re-pairings that pass the first sampling stage
     uint32_t pend_next;          // drain cursor
     uint32_t long_n, long_next;  // pairs handed from the per-lane to the wavefront schedule
     uint32_t llen[6];            // alive particles per species list
@@ -307,13 +339,30 @@ struct TrackShared {
     uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
     int      bri[9], nblk;                // channel blocks: reaction, units per row
     uint32_t bcpr[9], bu0[9], units;
+    // product re-pairing: segments (product p, partner species sj) over the species lists
+    int      nseg;
+    uint32_t seg_u0[13], seg_ps[12];      // first flattened index; p << 4 | sj
+    float    seg_S[12], seg_sc[12], seg_R[12], seg_Dj[12];
+    float4   prod_pos[2];
 };
+
+extern __shared__ float4 dyn_lds[];
+
+// Event storage in LDS during the event loop, behind the alive flags (cap bytes): times
+// [L] then pairs packed a << 16 | b [L] (cap <= 65535).
+__device__ __forceinline__ double *ev_lds_t(const IrtTrack &T)
+{
+    return (double *)((char *)dyn_lds + (((size_t)T.cap + 15u) & ~(size_t)15u));
+}
+__device__ __forceinline__ uint32_t *ev_lds_ab(const IrtTrack &T, uint32_t L) { return (uint32_t *)(ev_lds_t(T) + L); }
 
 __device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
                                            double t, uint32_t a, uint32_t b)
 {
     uint32_t e = atomicAdd(&sh.n_ev, 1u);
-    if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
+    const uint32_t L = sh.ev_lds;
+    if (e < L) { ev_lds_t(T)[e] = t; ev_lds_ab(T, L)[e] = (a << 16) | b; }
+    else if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
     else atomicOr(A.overflow, 2u);
 }
 
@@ -337,7 +386,32 @@ __device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T
 // Called by all threads after a barrier; the caller resets the queue.
 struct PendPair { uint32_t a, b; double r0; int ri; double D; uint64_t ps; };
 
-__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t item)
+// r0 of a product re-pairing (irt-worker.js:443-456): the partner, the one born before
+// t_base (the product itself was born at t_base), takes a Brownian step over
+// t_base - tbirth with the pair's draws 0-5, then r0 = |product - partner|; -1 beyond R_CUT.
+__device__ __forceinline__ double repair_r0(const double *tb, float4 pa, float4 pb, uint32_t a, uint32_t b,
+                                            uint64_t ps, double t_base)
+{
+    const IrtConst &C = c_irt;
+    const bool a_old = tb[a] < t_base;
+    const float4 pi = a_old ? pb : pa, pj = a_old ? pa : pb;
+    const double dt = t_base - tb[a_old ? a : b];
+    double xj = pj.x, yj = pj.y, zj = pj.z;
+    if (dt > 0) {
+        double sig = sqrt(2.0 * C.D[(int)pj.w] * dt);
+        double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
+        double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
+        double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
+        xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
+    }
+    double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
+    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    return r2 > C.R_CUT2 ? -1.0 : sqrt(r2);
+}
+
+// A queued pair with its r0 (r0 < 0: a re-pairing beyond R_CUT, no event).
+__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t item,
+                                                 double t_base)
 {
     const IrtConst &C = c_irt;
     const float4 *P = A.pos + T.part_off;
@@ -345,7 +419,8 @@ __device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrac
     PendPair q;
     q.a = e.x; q.b = e.y;
     q.r0 = __hiloint2double((int)e.w, (int)e.z);
-    int si = (int)P[q.a].w, sj = (int)P[q.b].w;
+    const float4 pa = P[q.a], pb = P[q.b];
+    int si = (int)pa.w, sj = (int)pb.w;
     q.ri = C.rxnMap[si * 6 + sj];
     q.D = C.D[si] + C.D[sj];
     q.ps = dna_pair_state(tk, q.a, q.b);
@@ -362,7 +437,11 @@ __device__ __forceinline__ void pdc_setup(const PendPair &q, double &a, double &
     pq = qq;
 }
 
-// one SamplePDC trial (irt-worker.js:95-111); returns accept
+// one SamplePDC trial (irt-worker.js:95-111); returns accept.  The acceptance value
+// lam = exp(-b^2/X) (1 - a sqrt(pi X) erfcx(...)) is at most exp(-b^2/X) (a > 0 and
+// erfcx > 0), so a trial whose uniform exceeds that bound is rejected before the FP64
+// exp and erfcx: the bound is taken in FP32 with a 1e-4 relative and 1e-37 absolute
+// margin, far above its rounding, so only trials the exact test rejects are skipped.
 __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q, double M, uint64_t ps,
                                           uint32_t trial, double &X)
 {
@@ -371,6 +450,14 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
     if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
     else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
     U = dna_u01(ps, 8u + 2u * trial);
+    {
+        const bool c1 = X <= 2.0 * b / a, c2 = X >= 2.0 * b / a;
+        const float ub = expf(-(float)(b * b / X)) * 1.0001f + 1e-37f;
+        const float fU = (float)U;
+        const bool may1 = c1 && !(fU > ub);
+        const bool may2 = c2 && !(fU * (float)M * 0.9999f > ub * (float)X * 1.0001f);
+        if (!may1 && !may2) return false;
+    }
     double sqX = sqrt(X);
     double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
     return (X <= 2.0 * b / a && U <= lam) || (X >= 2.0 * b / a && U * M / X <= lam);
@@ -394,8 +481,8 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         bool done = false;
         double X = -1;
         if (need_setup) {
-            pp = load_pending(A, T, tk, item);
-            double t1 = irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+            pp = load_pending(A, T, tk, item, t_base);
+            double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
             if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; need_setup = false; }
             else {
                 if (t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
@@ -433,10 +520,10 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         slot = __shfl(slot, 0, 64);
         if (slot >= (use_long ? nlong : npend)) break;
         uint32_t item = use_long ? sh.long_item[slot] : slot;
-        PendPair pp = load_pending(A, T, tk, item);
+        PendPair pp = load_pending(A, T, tk, item, t_base);
         uint32_t start = 0;
         if (!use_long) {
-            double t1 = irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+            double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
             if (t1 != STAGE_PDC) {
                 if (lane == 0 && t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
                 continue;
@@ -457,6 +544,36 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
+// Queued re-pairings (r0 = -1) get their r0 and the exact first sampling stage in one
+// dense pass; the pairs that can react are compacted to the front of the queue (through
+// the space behind it), so the drain sees only those.  All threads, after a barrier.
+__device__ void repair_prepare(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
+{
+    const IrtConst &C = c_irt;
+    const float4 *P = A.pos + T.part_off;
+    const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
+    if (threadIdx.x == 0) sh.nkeep = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < npend; i += IRT_BLOCK) {
+        uint4 e = A.pend[T.ev_off + i];
+        const float4 pa = P[e.x], pb = P[e.y];
+        const uint64_t ps = dna_pair_state(tk, e.x, e.y);
+        double r0 = repair_r0(A.tb + T.part_off, pa, pb, e.x, e.y, ps, t_base);
+        if (r0 < 0) continue;
+        const int si = (int)pa.w, sj = (int)pb.w;
+        if (!may_react_k(rx_constants(C.rxnMap[si * 6 + sj], C.D[si] + C.D[sj]), r0, ps)) continue;
+        uint32_t k = atomicAdd(&sh.nkeep, 1u);
+        if (npend + k < T.ev_cap)
+            A.pend[T.ev_off + npend + k] = make_uint4(e.x, e.y, (uint32_t)__double2loint(r0), (uint32_t)__double2hiint(r0));
+        else atomicOr(A.overflow, 2u);
+    }
+    __syncthreads();
+    const uint32_t nk = (npend + sh.nkeep <= T.ev_cap) ? sh.nkeep : 0u;
+    for (uint32_t k = threadIdx.x; k < nk; k += IRT_BLOCK) A.pend[T.ev_off + k] = A.pend[T.ev_off + npend + k];
+    __syncthreads();
+    if (threadIdx.x == 0) sh.npend = nk;
+}
+
 // drain the whole queue; all threads call it after a barrier
 __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
 {
@@ -470,50 +587,48 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, Trac
     }
 }
 
-// Re-pair a new product `idx` (born at t_new) with every alive particle that has a
-// reaction channel with it (irt-worker.js:430-465).  Alive particles are kept in
-// per-species lists, so only partner species with a channel are visited and every
-// lane of a pass evaluates the same reaction.  `skip` excludes the second product of
-// a two-product reaction while the first is paired (it does not exist yet at that point
-// of the worker's loop, irt-worker.js:618-644).
-__device__ void pair_with_species(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t idx,
-                                  uint32_t skip, double t_new, TrackShared &sh, unsigned long long &wrep)
+// Re-pair the products of the reaction at t_new with every alive particle that has a
+// reaction channel with them (irt-worker.js:430-465).  Alive particles are kept in
+// per-species lists; the (product, partner species) segments of one reaction are
+// flattened into one index range (table built by the reacting thread).  Product 0 is not
+// paired with product 1: it does not exist yet at that point of the worker's loop
+// (irt-worker.js:618-644).  Each pair first meets the necessary condition of
+// cannot_react on a lower bound of its distance after the partner's Brownian step; the
+// survivors are queued with r0 = -1 and get the step, r0 and the sampling in the drain.
+__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, uint64_t tk, double t_new,
+                              unsigned long long &wrep)
 {
     const float4 *P = A.pos + T.part_off;
     const double *tb = A.tb + T.part_off;
     const uint32_t *lst = A.lst + 6 * T.part_off;
-    const float4 pi = P[idx];
-    const int si = __builtin_amdgcn_readfirstlane((int)pi.w);
-    const double RC2 = c_irt.R_CUT2;
-    for (int sj = 0; sj < 6; sj++) {
-        const int ri = c_irt.rxnMap[si * 6 + sj];
-        if (ri < 0) continue;
-        const uint32_t len = __builtin_amdgcn_readfirstlane(sh.llen[sj]);
-        const RxK k = rx_constants(ri, c_irt.D[si] + c_irt.D[sj]);
-        const double Dj = c_irt.D[sj];
-        const uint32_t *L = lst + (size_t)sj * T.cap;
-        for (uint32_t pos = threadIdx.x; pos < len; pos += IRT_BLOCK) {
-            uint32_t jj = L[pos];
-            if (jj == idx || jj == skip) continue;
-            wrep++;
-            float4 pj = P[jj];
-            uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
-            uint64_t ps = dna_pair_state(tk, a, b);
-            double dt = t_new - tb[jj];
-            double xj = pj.x, yj = pj.y, zj = pj.z;
-            if (dt > 0) {
-                double sig = sqrt(2.0 * Dj * dt);
-                double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
-                double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
-                double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
-                xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
-            }
-            double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
-            double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (r2 > RC2) continue;
-            double r0 = sqrt(r2);
-            if (may_react_k(k, r0, ps)) push_pending(A, T, sh, a, b, r0);
+    const int ns = sh.nseg, np = sh.np;
+    const uint32_t tot = sh.seg_u0[ns];
+    for (uint32_t u = threadIdx.x; u < tot; u += IRT_BLOCK) {
+        int sg = 0;
+        while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
+        const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
+        const uint32_t idx = sh.prod_idx[p];
+        const uint32_t jj = lst[(size_t)sj * T.cap + (u - sh.seg_u0[sg])];
+        if (jj == idx || (p == 0 && np == 2 && jj == sh.prod_idx[1])) continue;
+        wrep++;
+        const float4 pi = sh.prod_pos[p], pj = P[jj];
+        const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
+        const uint64_t ps = dna_pair_state(tk, a, b);
+        const double dt = t_new - tb[jj];
+        // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so the
+        // step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
+        float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
+        float rmin = sqrtf(fx * fx + fy * fy + fz * fz) * 0.99999f;
+        if (dt > 0) {
+            float m2 = -2.f * (logf((float)(dna_u01(ps, 0) + 1e-30)) + logf((float)(dna_u01(ps, 2) + 1e-30)) +
+                               logf((float)(dna_u01(ps, 4) + 1e-30)));
+            float sgm = sqrtf((float)(2.0 * (double)sh.seg_Dj[sg] * dt));
+            rmin -= sgm * sqrtf(m2 * 1.0001f + 1e-4f) * 1.0001f;
         }
+        if (rmin > c_irt.R_CUT_F) continue;
+        RxB kb; kb.S = sh.seg_S[sg]; kb.sc = sh.seg_sc[sg]; kb.R = sh.seg_R[sg];
+        if (cannot_react(kb, rmin, dna_u01(ps, 6u))) continue;
+        push_pending(A, T, sh, a, b, -1.0);
     }
 }
 
@@ -575,12 +690,12 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             }
         ord = s_sort;
     }
-    if (tid == 0) { sh.n_ev = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
+    if (tid == 0) { sh.n_ev = 0; sh.ev_lds = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
     for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_BLOCK) s_loc[i] = 0;
     __syncthreads();
 
     // phase timers (100 MHz wall clock, lane 0): load, pairs, scan, react, repair, iterations
-    unsigned long long tp0 = wall_clock64(), tp1 = 0, tp2 = 0, tsc = 0, trc = 0, trp = 0, nit = 0;
+    unsigned long long tp0 = wall_clock64(), tp1 = 0, tp2 = 0, tsc = 0, trc = 0, trp = 0, trd = 0, nit = 0;
     // 1. load + displacement (irt-worker.js:475-537)
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
@@ -626,7 +741,6 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     // Pairs are identified by their original indices, so the enumeration order does not
     // change any draw or the event order.
     unsigned long long wcand = 0, wsamp = 0, wrep = 0;
-    extern __shared__ float4 dyn_lds[];
     float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
     if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
     __syncthreads();
@@ -678,6 +792,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             const uint32_t oB = __builtin_amdgcn_readfirstlane(sh.goff[sb]);
             const uint32_t units = nA * cpr;
             RxK k = rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]);
+            const RxB kb = rx_bound(k);
             const uint32_t h = (nA - 1) / 2;
             for (uint32_t ul = wave; ul < units; ul += IRT_BLOCK / 64) {
                 const uint32_t i = ul / cpr, c = ul - i * cpr;
@@ -706,9 +821,9 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
                 wsamp++;
                 uint32_t oi = (uint32_t)__float_as_int(pi.w), oj = (uint32_t)__float_as_int(pj.w);
                 uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
-                double r0 = sqrt(r2);
-                if (may_react_k(k, r0, dna_pair_state(tk, a, b)))
-                    push_pending(A, T, sh, a, b, r0);
+                const uint64_t ps = dna_pair_state(tk, a, b);
+                if (cannot_react(kb, sqrtf((float)r2) * 0.99999f, dna_u01(ps, 6u))) continue;
+                push_pending(A, T, sh, a, b, sqrt(r2));
             }
         }
     }
@@ -737,6 +852,21 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     if ((size_t)T.cap <= (size_t)A.lds_slots * 16u) {
         uint8_t *la = (uint8_t *)dyn_lds;
         for (uint32_t k = tid; k < T.cap; k += IRT_BLOCK) la[k] = k < n ? al[k] : 0;
+        // and the events behind them, when all current ones fit: the scan below reads every
+        // event each iteration; later events beyond the LDS capacity go to HBM
+        const size_t lds_bytes = (size_t)A.lds_slots * 16u, abytes = ((size_t)T.cap + 15u) & ~(size_t)15u;
+        const uint32_t L = (T.cap <= 65535u && lds_bytes > abytes) ? (uint32_t)((lds_bytes - abytes) / 12u) : 0u;
+        const uint32_t ne0 = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
+        if (L >= 64u && ne0 <= L) {
+            double *lt = ev_lds_t(T);
+            uint32_t *lab = ev_lds_ab(T, L);
+            for (uint32_t e = tid; e < ne0; e += IRT_BLOCK) {
+                uint2 ab = A.ev_ab[T.ev_off + e];
+                lt[e] = A.ev_t[T.ev_off + e];
+                lab[e] = (ab.x << 16) | ab.y;
+            }
+            if (tid == 0) sh.ev_lds = L;
+        }
         __syncthreads();
         al = la;
     }
@@ -748,7 +878,19 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         nit++;
         uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
         double bt = 1e300; uint64_t bab = ~0ull;
-        for (uint32_t e = tid; e < ne; e += IRT_BLOCK) {
+        const uint32_t L = sh.ev_lds, nl = ne < L ? ne : L;
+        if (nl) {
+            const double *lt = ev_lds_t(T);
+            const uint32_t *lab = ev_lds_ab(T, L);
+            for (uint32_t e = tid; e < nl; e += IRT_BLOCK) {
+                const uint32_t ab = lab[e], a = ab >> 16, b = ab & 0xFFFFu;
+                if (!al[a] || !al[b]) continue;
+                const double t = lt[e];
+                uint64_t k = ((uint64_t)a << 32) | b;
+                if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
+            }
+        }
+        for (uint32_t e = nl + tid; e < ne; e += IRT_BLOCK) {
             uint2 ab = A.ev_ab[T.ev_off + e];
             double t = A.ev_t[T.ev_off + e];
             if (!al[ab.x] || !al[ab.y]) continue;
@@ -808,27 +950,45 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
                     sh.np = made;
                     sh.n_total = nt;
                 }
+                // segments of the re-pairing: (product, partner species with a channel)
+                uint32_t u = 0;
+                int ns = 0;
+                for (int pp = 0; pp < sh.np; pp++) {
+                    const uint32_t id = sh.prod_idx[pp];
+                    const int sp = (int)P[id].w;
+                    sh.prod_pos[pp] = P[id];
+                    for (int sj2 = 0; sj2 < 6; sj2++) {
+                        const int rj = C.rxnMap[sp * 6 + sj2];
+                        if (rj < 0 || sh.llen[sj2] == 0) continue;
+                        const RxB bnd = rx_bound(rx_constants(rj, C.D[sp] + C.D[sj2]));
+                        sh.seg_u0[ns] = u; sh.seg_ps[ns] = ((uint32_t)pp << 4) | (uint32_t)sj2;
+                        sh.seg_S[ns] = bnd.S; sh.seg_sc[ns] = bnd.sc; sh.seg_R[ns] = bnd.R;
+                        sh.seg_Dj[ns] = (float)C.D[sj2];
+                        u += sh.llen[sj2];
+                        ns++;
+                    }
+                }
+                sh.seg_u0[ns] = u; sh.nseg = ns;
             }
         }
         __syncthreads();
         unsigned long long q2 = wall_clock64();
         trc += q2 - q1;
         if (!(sh.win_t < 1e300)) break;
-        int np = sh.np;
-        uint32_t base_total = sh.n_total - (uint32_t)np;
-        for (int pp = 0; pp < np; pp++) {
-            // product pp is paired with everything alive up to and including itself
-            uint32_t skip = (pp == 0 && np == 2) ? sh.prod_idx[1] : 0xFFFFFFFFu;
-            pair_with_species(A, T, tk, sh.prod_idx[pp], skip, sh.win_t, sh, wrep);
-        }
+        if (sh.np) pair_products(A, T, sh, tk, sh.win_t, wrep);
         __syncthreads();
+        unsigned long long q3 = wall_clock64();
         if (sh.npend) {
-            pdc_drain(A, T, tk, sh, sh.win_t);
+            repair_prepare(A, T, tk, sh, sh.win_t);
+            __syncthreads();
+            if (sh.npend) pdc_drain(A, T, tk, sh, sh.win_t);
             __syncthreads();
             if (tid == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
             __syncthreads();
         }
-        trp += wall_clock64() - q2;
+        unsigned long long q4 = wall_clock64();
+        trp += q3 - q2;
+        trd += q4 - q3;
     }
     for (int o = 32; o > 0; o >>= 1) {
         wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); wrep += __shfl_xor(wrep, o, 64);
@@ -842,7 +1002,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     if (tid == 0) {
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
         atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tpl - tp1); atomicAdd(&g[6], tp2 - tpl); atomicAdd(&g[2], tsc);
-        atomicAdd(&g[3], trc); atomicAdd(&g[4], trp); atomicAdd(&g[5], nit);
+        atomicAdd(&g[3], trc); atomicAdd(&g[4], trp); atomicAdd(&g[5], nit); atomicAdd(&g[7], trd);
     }
     if (tid == 0) {
         while (cp < A.nc) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
@@ -900,6 +1060,7 @@ static void build_const(IrtConst &C)
     for (int r = 0; r < 9; r++) { C.rxnMap[RA[r] * 6 + RB[r]] = r; C.rxnMap[RB[r] * 6 + RA[r]] = r; }
     double R_CUT = 1.45 + 2 * sqrt(8 * 9.46 * 1000);
     C.R_CUT2 = R_CUT * R_CUT;
+    C.R_CUT_F = (float)R_CUT * 1.00001f;
     static const double g[13] = {-4.06217193e-08, 3.06848412e-06, -9.93217814e-05, 1.80172797e-03,
         -2.01135480e-02, 1.42939448e-01, -6.48348714e-01, 1.85227848e+00, -3.36450378e+00,
         4.37785068e+00, -4.20557339e+00, 3.81679083e+00, -2.34069784e-01};
