@@ -192,6 +192,16 @@ int dna_hash_cells(struct dna_ctx *ctx, const float *pos4, uint64_t n, float cel
                    uint32_t hash_bits, uint32_t *key_out, uint32_t *cell_start_out,
                    uint32_t *order_out);
 
+/* Launch options (no reference counterpart: WebGPU fixes its workgroup sizes in the
+ * shaders).  DNA_OPT_IRT_BLOCK: IRT workgroup size, 0 = autotune (default: the first
+ * launch of each track-size class times 256/128/64-thread workgroups on up to 3072 of
+ * its tracks and keeps the fastest), or 64 / 128 / 256.  DNA_OPT_IRT_BLOCK_LAST (get
+ * only): the size the last IRT launch used.  Results do not depend on these options. */
+#define DNA_OPT_IRT_BLOCK       1
+#define DNA_OPT_IRT_BLOCK_LAST  2
+int dna_set_option(struct dna_ctx *ctx, int option, int value);
+int dna_get_option(struct dna_ctx *ctx, int option, int *value);
+
 /* Phase C on a subset of tracks with per-track output (the reference runs every track;
  * this is the split the surrogate fallback needs).  Same semantics as dna_run_irt, plus:
  * select[pid] != 0 (pid < n_prim) picks the primaries whose tracks run (NULL = all); a

@@ -89,3 +89,33 @@ def test_irt_worker_codes_case(ctx):
     _same(g, o)
     # two initial H2 markers counted from 1 ps on (test_worker_codes.cjs:75-85)
     assert g.tally[0][4] >= 2
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+def test_irt_workgroup_size_does_not_change_results(ctx, rad10k, block):
+    from dna_amd.abi import DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST
+    print(f"irt 10 keV with {block}-thread workgroups", flush=True)
+    ctx.set_option(DNA_OPT_IRT_BLOCK, block)
+    try:
+        g = ctx.irt(rad10k, None, 48, 10000.0, CK, seed=77)
+        assert ctx.get_option(DNA_OPT_IRT_BLOCK_LAST) == block
+    finally:
+        ctx.set_option(DNA_OPT_IRT_BLOCK, 0)
+    o = O.irt(rad10k, 48, 10000.0, CK, seed=77)
+    _same(g, o)
+
+
+def test_irt_autotune_picks_a_size_and_keeps_results(ctx):
+    from dna_amd.abi import DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST
+    p = default_params(1000.0, 1536)
+    t = ctx.transport(p)
+    print("irt autotune on 1536 x 1 keV", flush=True)
+    ctx.set_option(DNA_OPT_IRT_BLOCK, 0)
+    g = ctx.irt(None, t["rad_n"], 1536, 1000.0, CK, seed=5)
+    assert ctx.get_option(DNA_OPT_IRT_BLOCK_LAST) in (64, 128, 256)
+    ctx.set_option(DNA_OPT_IRT_BLOCK, 256)
+    try:
+        r = ctx.irt(None, t["rad_n"], 1536, 1000.0, CK, seed=5)
+    finally:
+        ctx.set_option(DNA_OPT_IRT_BLOCK, 0)
+    _same(g, r)

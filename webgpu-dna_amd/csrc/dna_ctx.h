@@ -1,5 +1,6 @@
 // dna_ctx.h — the opaque context behind the C ABI.
 #pragma once
+#include <map>
 #include "dna_internal.h"
 
 struct IrtBuffers {
@@ -57,4 +58,8 @@ struct dna_ctx {
     IrtBuffers irt;
     ScoreBuffers sc;
     HashBuffers hs;
+    // launch tuning: IRT workgroup size (0 = autotune per track-size class)
+    int irt_block = 0;
+    int irt_last_block = 0;            // workgroup size of the last IRT launch
+    std::map<int, int> irt_tuned;      // log2(mean particles per track) -> workgroup size
 };

@@ -309,7 +309,8 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 }
 
 // ------------------------------------------------------------ per-track kernel
-#define IRT_BLOCK 256
+#define IRT_BLOCK 256                // largest workgroup; the launch picks 64, 128 or 256
+#define IRT_NT (blockDim.x)
 #define SORT_MAX 8192
 
 __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint64_t ab2)
@@ -554,7 +555,7 @@ __device__ void repair_prepare(const IrtArgs &A, const IrtTrack &T, uint64_t tk,
     const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
     if (threadIdx.x == 0) sh.nkeep = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < npend; i += IRT_BLOCK) {
+    for (uint32_t i = threadIdx.x; i < npend; i += IRT_NT) {
         uint4 e = A.pend[T.ev_off + i];
         const float4 pa = P[e.x], pb = P[e.y];
         const uint64_t ps = dna_pair_state(tk, e.x, e.y);
@@ -569,7 +570,7 @@ __device__ void repair_prepare(const IrtArgs &A, const IrtTrack &T, uint64_t tk,
     }
     __syncthreads();
     const uint32_t nk = (npend + sh.nkeep <= T.ev_cap) ? sh.nkeep : 0u;
-    for (uint32_t k = threadIdx.x; k < nk; k += IRT_BLOCK) A.pend[T.ev_off + k] = A.pend[T.ev_off + npend + k];
+    for (uint32_t k = threadIdx.x; k < nk; k += IRT_NT) A.pend[T.ev_off + k] = A.pend[T.ev_off + npend + k];
     __syncthreads();
     if (threadIdx.x == 0) sh.npend = nk;
 }
@@ -578,7 +579,7 @@ __device__ void repair_prepare(const IrtArgs &A, const IrtTrack &T, uint64_t tk,
 __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
 {
     const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
-    if (npend >= 2u * IRT_BLOCK) {
+    if (npend >= 2u * IRT_NT) {
         pdc_drain_lanes(A, T, tk, sh, npend, t_base);
         __syncthreads();
         pdc_drain_waves(A, T, tk, sh, npend, t_base, true);
@@ -603,7 +604,7 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
     const uint32_t *lst = A.lst + 6 * T.part_off;
     const int ns = sh.nseg, np = sh.np;
     const uint32_t tot = sh.seg_u0[ns];
-    for (uint32_t u = threadIdx.x; u < tot; u += IRT_BLOCK) {
+    for (uint32_t u = threadIdx.x; u < tot; u += IRT_NT) {
         int sg = 0;
         while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
         const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
@@ -654,7 +655,7 @@ __device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared 
 }
 
 template <bool SORT>
-__global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
+__global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
 {
     const IrtConst &C = c_irt;
     __shared__ TrackShared sh;
@@ -674,11 +675,11 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     const uint32_t *ord = idx;
     if (SORT) {
         uint32_t np2 = 1; while (np2 < n) np2 <<= 1;
-        for (uint32_t i = tid; i < np2; i += IRT_BLOCK) s_sort[i] = i < n ? idx[i] : 0xFFFFFFFFu;
+        for (uint32_t i = tid; i < np2; i += IRT_NT) s_sort[i] = i < n ? idx[i] : 0xFFFFFFFFu;
         __syncthreads();
         for (uint32_t k = 2; k <= np2; k <<= 1)
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < np2; i += IRT_BLOCK) {
+                for (uint32_t i = tid; i < np2; i += IRT_NT) {
                     uint32_t l = i ^ j;
                     if (l > i) {
                         uint32_t x = s_sort[i], y = s_sort[l];
@@ -691,14 +692,14 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         ord = s_sort;
     }
     if (tid == 0) { sh.n_ev = 0; sh.ev_lds = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
-    for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_BLOCK) s_loc[i] = 0;
+    for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_NT) s_loc[i] = 0;
     __syncthreads();
 
     // phase timers (100 MHz wall clock, lane 0): load, pairs, scan, react, repair, iterations
     unsigned long long tp0 = wall_clock64(), tp1 = 0, tp2 = 0, tsc = 0, trc = 0, trp = 0, trd = 0, nit = 0;
     // 1. load + displacement (irt-worker.js:475-537)
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
-    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+    for (uint32_t k = tid; k < n; k += IRT_NT) {
         float4 e = A.rad[ord[k]];
         double x = e.x, y = e.y, z = e.z;
         int s = (int)fmod(floor((double)e.w + 0.5), 8.0);
@@ -744,7 +745,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
     if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
     __syncthreads();
-    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+    for (uint32_t k = tid; k < n; k += IRT_NT) {
         int spc = (int)P[k].w;
         if (al[k] && spc >= 0 && spc < 6) atomicAdd(&sh.gcnt[spc], 1u);
     }
@@ -766,7 +767,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         sh.nblk = nb; sh.units = u;
     }
     __syncthreads();
-    for (uint32_t k = tid; k < n; k += IRT_BLOCK) {
+    for (uint32_t k = tid; k < n; k += IRT_NT) {
         float4 q = P[k];
         int spc = (int)q.w;
         if (al[k] && spc >= 0 && spc < 6) {
@@ -794,7 +795,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
             RxK k = rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]);
             const RxB kb = rx_bound(k);
             const uint32_t h = (nA - 1) / 2;
-            for (uint32_t ul = wave; ul < units; ul += IRT_BLOCK / 64) {
+            for (uint32_t ul = wave; ul < units; ul += IRT_NT / 64u) {
                 const uint32_t i = ul / cpr, c = ul - i * cpr;
                 const float4 pi = Sg[oA + i];
                 uint32_t j;
@@ -828,7 +829,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         }
     }
     // species lists of the alive initial particles = the species groups of Sg
-    for (uint32_t g = tid; g < n; g += IRT_BLOCK) {
+    for (uint32_t g = tid; g < n; g += IRT_NT) {
         int spc = 0;
         while (spc < 5 && g >= sh.goff[spc + 1]) spc++;
         if (g < sh.goff[spc] + sh.gcnt[spc]) {
@@ -851,7 +852,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
     // copy of the pair phase lived
     if ((size_t)T.cap <= (size_t)A.lds_slots * 16u) {
         uint8_t *la = (uint8_t *)dyn_lds;
-        for (uint32_t k = tid; k < T.cap; k += IRT_BLOCK) la[k] = k < n ? al[k] : 0;
+        for (uint32_t k = tid; k < T.cap; k += IRT_NT) la[k] = k < n ? al[k] : 0;
         // and the events behind them, when all current ones fit: the scan below reads every
         // event each iteration; later events beyond the LDS capacity go to HBM
         const size_t lds_bytes = (size_t)A.lds_slots * 16u, abytes = ((size_t)T.cap + 15u) & ~(size_t)15u;
@@ -860,7 +861,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         if (L >= 64u && ne0 <= L) {
             double *lt = ev_lds_t(T);
             uint32_t *lab = ev_lds_ab(T, L);
-            for (uint32_t e = tid; e < ne0; e += IRT_BLOCK) {
+            for (uint32_t e = tid; e < ne0; e += IRT_NT) {
                 uint2 ab = A.ev_ab[T.ev_off + e];
                 lt[e] = A.ev_t[T.ev_off + e];
                 lab[e] = (ab.x << 16) | ab.y;
@@ -882,7 +883,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         if (nl) {
             const double *lt = ev_lds_t(T);
             const uint32_t *lab = ev_lds_ab(T, L);
-            for (uint32_t e = tid; e < nl; e += IRT_BLOCK) {
+            for (uint32_t e = tid; e < nl; e += IRT_NT) {
                 const uint32_t ab = lab[e], a = ab >> 16, b = ab & 0xFFFFu;
                 if (!al[a] || !al[b]) continue;
                 const double t = lt[e];
@@ -890,7 +891,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
                 if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
             }
         }
-        for (uint32_t e = nl + tid; e < ne; e += IRT_BLOCK) {
+        for (uint32_t e = nl + tid; e < ne; e += IRT_NT) {
             uint2 ab = A.ev_ab[T.ev_off + e];
             double t = A.ev_t[T.ev_off + e];
             if (!al[ab.x] || !al[ab.y]) continue;
@@ -908,7 +909,7 @@ __global__ void __launch_bounds__(IRT_BLOCK) k_irt(IrtArgs A)
         tsc += q1 - q0;
         if (tid == 0) {
             double t0 = sh.wt[0]; uint64_t ab0 = sh.wab[0];
-            for (int w = 1; w < IRT_BLOCK / 64; w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
+            for (int w = 1; w < (int)(IRT_NT / 64u); w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
             sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
             if (t0 < 1e300) {
                 while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
@@ -1236,13 +1237,44 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         A.tally = d_tally; A.track_tally = d_tt; A.overflow = g.misc + 2;
         for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
         A.nc = nc; A.seed = seed; A.sort_tracks = contiguous ? 0 : 1;
-        DNA_HIP(hipEventRecord(c->ev[7], st));
-        // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB, three
-        // workgroups per CU), in global memory above
+        // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in
+        // global memory above; the event loop reuses that LDS for alive flags and events
         A.lds_slots = std::min<uint32_t>(maxn_all, 3072u);
         size_t shmem = (size_t)A.lds_slots * 16;
-        if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(ntr), dim3(IRT_BLOCK), shmem, st, A);
-        else hipLaunchKernelGGL(k_irt<true>, dim3(ntr), dim3(IRT_BLOCK), shmem, st, A);
+        auto launch = [&](uint32_t grid, int block) {
+            if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(grid), dim3(block), shmem, st, A);
+            else hipLaunchKernelGGL(k_irt<true>, dim3(grid), dim3(block), shmem, st, A);
+        };
+        int block = c->irt_block;
+        if (block == 0) {
+            // autotune per track-size class (log2 of the mean particles per track): the
+            // first launch of a class times each workgroup size on its first tracks (the
+            // results do not depend on the size; the trial tallies are cleared)
+            int cls = 0;
+            for (uint64_t m = run / ntr; m > 1; m >>= 1) cls++;
+            auto it = c->irt_tuned.find(cls);
+            if (it != c->irt_tuned.end()) block = it->second;
+            else if (ntr >= 1024) {
+                const uint32_t K = std::min<uint32_t>(ntr, 3072u);
+                float best = 1e30f;
+                for (int cand : {256, 128, 64}) {
+                    DNA_HIP(hipEventRecord(c->ev[0], st));
+                    launch(K, cand);
+                    DNA_HIP(hipGetLastError());
+                    DNA_HIP(hipEventRecord(c->ev[1], st));
+                    DNA_HIP(hipEventSynchronize(c->ev[1]));
+                    float ms = 0;
+                    DNA_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                    if (ms < best) { best = ms; block = cand; }
+                }
+                c->irt_tuned[cls] = block;
+                DNA_HIP(hipMemsetAsync(d_tally, 0, tally_bytes, st));
+                DNA_HIP(hipMemsetAsync(g.misc + 2, 0, 4, st));
+            } else block = (run / ntr < 512) ? 128 : 256;
+        }
+        c->irt_last_block = block;
+        DNA_HIP(hipEventRecord(c->ev[7], st));
+        launch(ntr, block);
         DNA_HIP(hipEventRecord(c->ev[4], st));
         DNA_HIP(hipGetLastError());
         uint32_t ovf = 0;

@@ -83,6 +83,36 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     return DNA_OK;
 }
 
+extern "C" int dna_set_option(dna_ctx *c, int option, int value)
+{
+    if (!c) { set_error("dna_set_option: NULL context"); return DNA_ERR_ARG; }
+    switch (option) {
+    case DNA_OPT_IRT_BLOCK:
+        if (value != 0 && value != 64 && value != 128 && value != 256) {
+            set_error("dna_set_option: DNA_OPT_IRT_BLOCK must be 0 (autotune), 64, 128 or 256");
+            return DNA_ERR_ARG;
+        }
+        c->irt_block = value;
+        if (value == 0) c->irt_tuned.clear();
+        return DNA_OK;
+    default:
+        set_error("dna_set_option: unknown option");
+        return DNA_ERR_ARG;
+    }
+}
+
+extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
+{
+    if (!c || !value) { set_error("dna_get_option: NULL argument"); return DNA_ERR_ARG; }
+    switch (option) {
+    case DNA_OPT_IRT_BLOCK: *value = c->irt_block; return DNA_OK;
+    case DNA_OPT_IRT_BLOCK_LAST: *value = c->irt_last_block; return DNA_OK;
+    default:
+        set_error("dna_get_option: unknown option");
+        return DNA_ERR_ARG;
+    }
+}
+
 extern "C" void dna_ctx_destroy(dna_ctx *c)
 {
     if (!c) return;

@@ -42,6 +42,8 @@ def lib():
     L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
                             C.POINTER(DnaScoreOut)]
     L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
+    L.dna_set_option.argtypes = [vp, C.c_int, C.c_int]
+    L.dna_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
                                      C.POINTER(DnaChemOut)]
     ip = C.POINTER(C.c_int)
@@ -64,7 +66,7 @@ def lib():
 
 EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_version",
                     "dna_run_transport", "dna_run_irt", "dna_run_irt_batch", "dna_score",
-                    "dna_hash_cells", "dna_run_irt_tracks", "dna_surrogate_create",
+                    "dna_hash_cells", "dna_set_option", "dna_get_option", "dna_run_irt_tracks", "dna_surrogate_create",
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
@@ -99,6 +101,15 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    # ---- launch options (DNA_OPT_*)
+    def set_option(self, option, value):
+        _check("dna_set_option", lib().dna_set_option(self.h, option, value))
+
+    def get_option(self, option):
+        v = C.c_int()
+        _check("dna_get_option", lib().dna_get_option(self.h, option, C.byref(v)))
+        return v.value
 
     # ---- Phase A + B
     def transport(self, params, energies=None, copy=DNA_COPY_ALL):
