@@ -202,6 +202,20 @@ int dna_hash_cells(struct dna_ctx *ctx, const float *pos4, uint64_t n, float cel
 int dna_set_option(struct dna_ctx *ctx, int option, int value);
 int dna_get_option(struct dna_ctx *ctx, int option, int *value);
 
+/* Captured batches: the instances of dna_run_irt_batch (rad == NULL: entries
+ * [inst_off[0], inst_off[n_inst]) of the radicals resident from the last transport),
+ * planned once (grouping, buffers, workgroup size) and captured as one hipGraph;
+ * dna_irt_plan_run replays it for a seed and fills out[n_inst], identical to
+ * dna_run_irt_batch with the same seed.  A plan owns its buffers and a copy of the
+ * radicals; a replay that exceeds the planned capacities is re-planned transparently.
+ * (The reference runs one Worker per call, src/chemistry/worker.ts:32.) */
+typedef struct dna_irt_plan dna_irt_plan;
+int  dna_irt_plan_create(struct dna_ctx *ctx, const float *rad, const uint64_t *inst_off,
+                         uint32_t n_inst, const uint32_t *n_therm, const float *E_eV,
+                         const double *t_ckpt, int n_ckpt, dna_irt_plan **out);
+int  dna_irt_plan_run(dna_irt_plan *plan, uint64_t seed, dna_chem_out *out /* [n_inst] */);
+void dna_irt_plan_destroy(dna_irt_plan *plan);
+
 /* Phase C on a subset of tracks with per-track output (the reference runs every track;
  * this is the split the surrogate fallback needs).  Same semantics as dna_run_irt, plus:
  * select[pid] != 0 (pid < n_prim) picks the primaries whose tracks run (NULL = all); a

@@ -115,3 +115,34 @@ def test_g_1us_against_e10(ctx, E):
     for s, key in enumerate(("OH", "eaq", "H", "H2O2", "H2")):
         want = row[key]
         assert abs(G[s] - want) < max(0.02 * want, 2.5 * sem[s]), (key, G[s], want, sem[s])
+
+
+def test_captured_irt_batch_replays_match_eager(ctx):
+    """The 512-instance batch captured as one hipGraph (dna_irt_plan): every replay, with
+    the program's own eager work and synchronisations in between, equals the eager
+    dna_run_irt_batch result for the same seed."""
+    n = 512
+    E = S.log_uniform_energies(n, E10_IRT_SEED, 1000.0, 20000.0)
+    p = S.transport_params(n, seed=E10_IRT_SEED)
+    t = ctx.transport(p, energies=E)
+    rad = t["rad"]
+    pid = (np.floor(rad[:, 3].astype(np.float64) + 0.5) // 8).astype(np.int64)
+    off = np.searchsorted(pid, np.arange(n + 1)).astype(np.uint64)
+    tc = S.checkpoints_16()
+    nth = np.ones(n, np.uint32)
+    print("irt plan from the resident radicals", flush=True)
+    plan = ctx.irt_plan(None, off, nth, E, tc)
+    try:
+        for k, seed in enumerate((21, 22, 21, 23)):
+            print(f"replay {k} seed {seed}", flush=True)
+            gp = plan.run(seed)
+            ge = ctx.irt_batch(rad, off, nth, E, tc, seed=seed)
+            ctx.transport(default_params(1000.0, 64), copy=DNA_COPY_RESULTS)     # eager work in between
+            for i in range(n):
+                a = [[gp[i].tally[c][s] for s in range(5)] for c in range(16)]
+                b = [[ge[i].tally[c][s] for s in range(5)] for c in range(16)]
+                assert a == b, (seed, i)
+                assert list(gp[i].rxn_counts) == list(ge[i].rxn_counts), (seed, i)
+                assert gp[i].n_tracks == ge[i].n_tracks
+    finally:
+        plan.close()

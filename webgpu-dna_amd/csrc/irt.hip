@@ -68,6 +68,7 @@ struct IrtArgs {
     double tc[DNA_MAX_CKPT];
     int nc;
     uint64_t seed;
+    const uint64_t *seed_ptr;     // when set, the seed is read here (graph replays)
     int sort_tracks;              // 1: bucket order is not input order -> sort per track
 };
 #define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 5 + 8)   // tallies, rxn, n_re, n_ev, 3 work counters, 8 phase timers
@@ -219,6 +220,12 @@ __device__ __forceinline__ bool decode_w(float wf, uint32_t &pid, int &sp)
     if (pd >= 4294967295.0) return false;
     pid = (uint32_t)pd;
     return true;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, size_t n)
+{
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
 }
 
 __device__ __forceinline__ uint32_t find_inst(const uint64_t *off, uint32_t n_inst, uint64_t i)
@@ -668,7 +675,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     float4 *P = A.pos + T.part_off;
     uint8_t *al = A.alive + T.part_off;
     double *tb = A.tb + T.part_off;
-    const uint64_t tk = dna_track_key(A.seed, T.inst, T.pid);
+    const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
 
     // input order of this track's entries (bucket order is already input order unless
     // the input interleaved primaries; then sort the <= SORT_MAX indices)
@@ -1093,52 +1100,59 @@ struct GroupScratch {
     uint32_t *kept, *key, *S, *ck, *cnt, *h2, *firstj, *bstart, *cursor, *idx, *misc;
 };
 
-// sel (optional, instance 0 only): run only the tracks of primaries pid < n_sel with
-// sel[pid] != 0; every other track is grouped as usual (capacities and RNG keys are those
-// of the full run, so a selected track's history is the one the full run gives it).
-// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
-static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
-                          const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
-                          dna_chem_out *outs, const uint8_t *sel = nullptr, uint32_t n_sel = 0,
-                          uint32_t *track_tally = nullptr)
+
+static int ensure_const(dna_ctx *c)
 {
-    hipStream_t st = c->stream;
-    uint32_t n_inst = (uint32_t)off.size() - 1;
     if (!g_const_ready[c->device]) {
         IrtConst C; build_const(C);
         DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_irt), &C, sizeof(C)));
         g_const_ready[c->device] = true;
     }
-    for (uint32_t i = 0; i < n_inst; i++) {
-        memset(&outs[i], 0, sizeof(dna_chem_out));
-        outs[i].n_ckpt = nc;
-        for (int k = 0; k < nc; k++) outs[i].t_ckpt[k] = t_ckpt[k];
-        outs[i].chem_n = off[i + 1] - off[i];
-    }
-    if (rad_n == 0) return DNA_OK;
+    return DNA_OK;
+}
+
+// The track list of one set of inputs.  Entries of one (instance, primary) form a track;
+// the grouping scratch lives in the context until the next IRT call.
+struct IrtGroup {
+    uint32_t n_inst = 0;
+    uint64_t rad_n = 0, K = 0, K1 = 0;
+    uint32_t run = 0, maxn_all = 0, gb = 0;
+    bool contiguous = true;
+    std::vector<IrtTrack> tr;
+    std::vector<uint32_t> bstart, maxN;
+    GroupScratch g;
+};
+
+// Group on the device and build the track list on the host.  sel (optional, instance 0
+// only): keep only the tracks of primaries pid < n_sel with sel[pid] != 0; every other
+// track is grouped as usual (capacities and RNG keys are those of the full run, so a
+// selected track's history is the one the full run gives it).
+static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
+                     const uint8_t *sel, uint32_t n_sel, IrtGroup &G)
+{
+    hipStream_t st = c->stream;
+    const uint32_t n_inst = (uint32_t)off.size() - 1;
+    G.n_inst = n_inst; G.rad_n = rad_n;
     if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_run_irt: too many radical entries"); return DNA_ERR_CAPACITY; }
-    DNA_HIP(hipEventRecord(c->ev[5], st));
-    // instance offsets + pid range
-    uint64_t *d_off = nullptr;
-    std::vector<char> dummy;
     size_t misc_words = 64 + 2 * (n_inst + 1);
     // stage 1 scratch: kept, key, S (rad_n + 1 each), ck (rad_n), misc
     size_t need1 = 4 * ((size_t)rad_n + 1) + misc_words;
     DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, need1));
     uint32_t *base = (uint32_t *)c->irt.part;
-    GroupScratch g;
+    GroupScratch &g = G.g;
     g.kept = base; g.key = g.kept + rad_n + 1; g.S = g.key + rad_n + 1; g.ck = g.S + rad_n + 1;
     g.misc = g.ck + rad_n + 1;
-    d_off = (uint64_t *)(((uintptr_t)(g.misc + 16) + 7) & ~(uintptr_t)7);
+    uint64_t *d_off = (uint64_t *)(((uintptr_t)(g.misc + 16) + 7) & ~(uintptr_t)7);
     DNA_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemsetAsync(g.misc, 0, 16 * 4, st));
     const uint32_t B = 256;
-    uint32_t gb = (uint32_t)((rad_n + B - 1) / B);
-    hipLaunchKernelGGL(k_pidmax, dim3(gb < 4096 ? gb : 4096), dim3(B), 0, st, d_rad, rad_n, g.misc);
+    G.gb = (uint32_t)((rad_n + B - 1) / B);
+    hipLaunchKernelGGL(k_pidmax, dim3(G.gb < 4096 ? G.gb : 4096), dim3(B), 0, st, d_rad, rad_n, g.misc);
     uint32_t h_misc[16];
     DNA_HIP(hipMemcpyAsync(h_misc, g.misc, 16 * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipStreamSynchronize(st));
-    uint64_t K1 = (uint64_t)h_misc[0] + 1, K = K1 * n_inst;
+    const uint64_t K1 = (uint64_t)h_misc[0] + 1, K = K1 * n_inst;
+    G.K1 = K1; G.K = K;
     if (K > (1ull << 28)) { set_error("dna_run_irt: primary-id range too large"); return DNA_ERR_CAPACITY; }
     // stage 2 scratch: per-key arrays cnt, h2, firstj, bstart, cursor
     DNA_HIP(dgrow<uint32_t>(c->irt.tracks, c->irt.cap_tracks, 5 * (K + 1)));
@@ -1147,14 +1161,14 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
     DNA_HIP(hipMemsetAsync(g.cnt, 0, 2 * (K + 1) * 4, st));
     DNA_HIP(hipMemsetAsync(g.firstj, 0xFF, (K + 1) * 4, st));
     DNA_HIP(hipMemsetAsync(g.cursor, 0, (K + 1) * 4, st));
-    hipLaunchKernelGGL(k_count, dim3(gb), dim3(B), 0, st, d_rad, rad_n, d_off, n_inst, (uint32_t)K1,
+    hipLaunchKernelGGL(k_count, dim3(G.gb), dim3(B), 0, st, d_rad, rad_n, d_off, n_inst, (uint32_t)K1,
                        g.kept, g.key, g.cnt, g.h2);
     DNA_HIP(hipGetLastError());
     // exclusive scan of kept flags -> S (the scan helper scans in place)
     DNA_HIP(hipMemcpyAsync(g.S, g.kept, rad_n * 4, hipMemcpyDeviceToDevice, st));
     DNA_HIP(dgrow<uint32_t>(c->irt.scratch, c->irt.cap_scratch, rad_n / 1024 + 16));
     DNA_HIP(exclusive_scan_u32(g.S, (uint32_t)rad_n, (uint32_t *)c->irt.scratch, st));
-    hipLaunchKernelGGL(k_first, dim3(gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, g.ck, g.firstj);
+    hipLaunchKernelGGL(k_first, dim3(G.gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, g.ck, g.firstj);
     uint32_t m_kept = 0;
     DNA_HIP(hipMemcpyAsync(&m_kept, g.S + rad_n, 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipStreamSynchronize(st));
@@ -1166,123 +1180,234 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
     DNA_HIP(hipMemcpyAsync(h2.data(), g.h2, K * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipMemcpyAsync(h_misc, g.misc, 16 * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipStreamSynchronize(st));
-    // tracks
-    std::vector<uint32_t> maxN(n_inst, 0);
-    std::vector<IrtTrack> tr;
-    std::vector<uint32_t> bstart(K + 1, 0);
+    G.maxN.assign(n_inst, 0);
+    G.tr.clear();
+    G.bstart.assign(K + 1, 0);
     uint32_t run = 0;
     for (uint64_t k = 0; k < K; k++) {
-        bstart[k] = run;
+        G.bstart[k] = run;
         if (cnt[k]) {
             IrtTrack t; memset(&t, 0, sizeof(t));
             t.inst = (uint32_t)(k / K1); t.pid = (uint32_t)(k % K1); t.start = run; t.n = cnt[k]; t.h2 = h2[k];
-            if (cnt[k] > maxN[t.inst]) maxN[t.inst] = cnt[k];
+            if (cnt[k] > G.maxN[t.inst]) G.maxN[t.inst] = cnt[k];
             run += cnt[k];
             if (sel && !(t.inst == 0 && t.pid < n_sel && sel[t.pid])) continue;
-            tr.push_back(t);
+            G.tr.push_back(t);
         }
     }
-    bstart[K] = run;
-    uint32_t ntr = (uint32_t)tr.size();
-    bool contiguous = ntr == 0 || h_misc[1] == ntr - 1;
-    uint32_t maxn_all = 0;
-    for (auto &t : tr) maxn_all = t.n > maxn_all ? t.n : maxn_all;
-    if (!contiguous && maxn_all > SORT_MAX) {
+    G.bstart[K] = run;
+    G.run = run;
+    const uint32_t ntr = (uint32_t)G.tr.size();
+    G.contiguous = ntr == 0 || h_misc[1] == ntr - 1;
+    G.maxn_all = 0;
+    for (auto &t : G.tr) G.maxn_all = t.n > G.maxn_all ? t.n : G.maxn_all;
+    if (!G.contiguous && G.maxn_all > SORT_MAX) {
         set_error("dna_run_irt: radicals of a primary are interleaved with others and the primary has more than 8192 entries; group the input by primary");
         return DNA_ERR_CAPACITY;
     }
+    return DNA_OK;
+}
+
+// Per-track capacities of one attempt: particle slots (after an overflow the worker's
+// CAP = 2 maxN + 512 of the instance) and event slots ev_mult n + 1024.
+static void irt_size(IrtGroup &G, int attempt, int ev_mult, uint64_t &part, uint64_t &evs)
+{
+    part = 0; evs = 0;
+    for (auto &t : G.tr) {
+        uint32_t capg = 2 * G.maxN[t.inst] + 512;
+        uint32_t capt = 2 * t.n + 512;
+        if (attempt > 0) capt = capg;
+        t.cap_global = capg; t.cap = capt < capg ? capt : capg;
+        t.part_off = part; part += t.cap;
+        t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
+        t.ev_off = evs; evs += t.ev_cap;
+    }
+}
+
+// Device buffers of one launch and their byte sizes.
+struct IrtBufs {
+    char *scratch = nullptr, *events = nullptr;
+    unsigned long long *tally = nullptr;   // n_inst x TALLY_STRIDE, then the overflow word
+    IrtTrack *tr = nullptr;
+    uint32_t *bst = nullptr, *idx = nullptr, *tt = nullptr;
+    uint32_t *ovf = nullptr;
+    size_t tally_bytes = 0;
+};
+
+static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 7 + 1) + 256, rad_n / 1024 * 4 + 64); }
+static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
+{
+    return tally_bytes + 256 + G.tr.size() * sizeof(IrtTrack) + (G.K + 1) * 4 + (size_t)G.run * 4 + tt_bytes + 1536;
+}
+
+// carve tally | tracks | bucket starts | bucket indices | per-track tallies out of one block
+static void carve_block(const IrtGroup &G, char *base, size_t tally_bytes, bool want_tt, IrtBufs &b)
+{
+    b.tally_bytes = tally_bytes;
+    b.tally = (unsigned long long *)base;
+    b.ovf = (uint32_t *)(base + tally_bytes);
+    b.tr = (IrtTrack *)(base + ((tally_bytes + 8 + 255) & ~(size_t)255));
+    b.bst = (uint32_t *)((char *)b.tr + ((G.tr.size() * sizeof(IrtTrack) + 255) & ~(size_t)255));
+    b.idx = b.bst + ((G.K + 1 + 63) & ~(uint64_t)63);
+    b.tt = want_tt ? b.idx + (((uint64_t)G.run + 63) & ~(uint64_t)63) : nullptr;
+}
+
+// upload the track list and bucket the entries (k_bucket) into b.idx
+static int irt_prepare(dna_ctx *c, const IrtGroup &G, const IrtBufs &b)
+{
+    hipStream_t st = c->stream;
+    const uint32_t ntr = (uint32_t)G.tr.size();
+    DNA_HIP(hipMemsetAsync(b.tally, 0, b.tally_bytes + 8, st));
+    DNA_HIP(hipMemcpyAsync(b.tr, G.tr.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(b.bst, G.bstart.data(), (G.K + 1) * 4, hipMemcpyHostToDevice, st));
+    const GroupScratch &g = G.g;
+    hipLaunchKernelGGL(k_bucket, dim3(G.gb), dim3(256), 0, st, g.kept, g.S, g.key, G.rad_n, b.bst, g.firstj, g.cursor,
+                       G.contiguous ? 1 : 0, b.idx);
+    DNA_HIP(hipGetLastError());
+    return DNA_OK;
+}
+
+static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b, uint64_t part, uint64_t evs,
+                        const double *t_ckpt, int nc, uint64_t seed)
+{
+    IrtArgs A;
+    memset(&A, 0, sizeof(A));
+    char *sp = b.scratch;
+    A.rad = d_rad; A.idx = b.idx; A.tracks = b.tr;
+    A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
+    A.lst = (uint32_t *)(sp + part * 40); A.slot = (uint32_t *)(sp + part * 64); A.alive = (uint8_t *)(sp + part * 68);
+    A.ev_t = (double *)b.events; A.ev_ab = (uint2 *)(b.events + evs * 8);
+    A.pend = (uint4 *)(b.events + evs * 16);
+    A.tally = b.tally; A.track_tally = b.tt; A.overflow = b.ovf;
+    for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
+    A.nc = nc; A.seed = seed; A.seed_ptr = nullptr; A.sort_tracks = G.contiguous ? 0 : 1;
+    // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in global
+    // memory above; the event loop reuses that LDS for alive flags and events
+    A.lds_slots = std::min<uint32_t>(G.maxn_all, 3072u);
+    return A;
+}
+
+static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
+{
+    const size_t shmem = (size_t)A.lds_slots * 16;
+    if (G.contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(grid), dim3(block), shmem, st, A);
+    else hipLaunchKernelGGL(k_irt<true>, dim3(grid), dim3(block), shmem, st, A);
+}
+
+// Workgroup size: the option, or autotuned per track-size class (log2 of the mean
+// particles per track): the first launch of a class times each size on its first tracks
+// (results do not depend on the size; the trial tallies are cleared afterwards).
+static int irt_pick_block(dna_ctx *c, const IrtGroup &G, const IrtArgs &A, const IrtBufs &b, int &block)
+{
+    hipStream_t st = c->stream;
+    block = c->irt_block;
+    if (block) return DNA_OK;
+    const uint32_t ntr = (uint32_t)G.tr.size();
+    int cls = 0;
+    for (uint64_t m = G.run / ntr; m > 1; m >>= 1) cls++;
+    auto it = c->irt_tuned.find(cls);
+    if (it != c->irt_tuned.end()) { block = it->second; return DNA_OK; }
+    if (ntr < 1024) { block = (G.run / ntr < 512) ? 128 : 256; return DNA_OK; }
+    const uint32_t Kt = std::min<uint32_t>(ntr, 3072u);
+    float best = 1e30f;
+    for (int cand : {256, 128, 64}) {
+        DNA_HIP(hipEventRecord(c->ev[0], st));
+        irt_launch(G, A, Kt, cand, st);
+        DNA_HIP(hipGetLastError());
+        DNA_HIP(hipEventRecord(c->ev[1], st));
+        DNA_HIP(hipEventSynchronize(c->ev[1]));
+        float ms = 0;
+        DNA_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        if (ms < best) { best = ms; block = cand; }
+    }
+    c->irt_tuned[cls] = block;
+    DNA_HIP(hipMemsetAsync(b.tally, 0, b.tally_bytes + 8, st));
+    return DNA_OK;
+}
+
+// host tallies -> dna_chem_out of every instance
+static void irt_outputs(const IrtGroup &G, const unsigned long long *ht, const uint32_t *n_therm, const float *E,
+                        int nc, float ms, float kms, dna_chem_out *outs)
+{
+    for (uint32_t i = 0; i < G.n_inst; i++) {
+        const unsigned long long *g2 = &ht[(size_t)i * TALLY_STRIDE];
+        dna_chem_out &o = outs[i];
+        double per100 = (double)n_therm[i] * (double)E[i] / 100.0;
+        for (int k = 0; k < nc; k++) for (int s = 0; s < 5; s++) {
+            o.tally[k][s] = g2[k * 5 + s];
+            o.G[k][s] = per100 > 0 ? (double)o.tally[k][s] / per100 : 0.0;
+        }
+        for (int r = 0; r < 9; r++) o.rxn_counts[r] = g2[DNA_MAX_CKPT * 5 + r];
+        o.n_reacted = g2[DNA_MAX_CKPT * 5 + 9];
+        o.n_events = g2[DNA_MAX_CKPT * 5 + 10];
+        o.n_pair_cand = g2[DNA_MAX_CKPT * 5 + 11];
+        o.n_pair_sampled = g2[DNA_MAX_CKPT * 5 + 12];
+        o.n_repair = g2[DNA_MAX_CKPT * 5 + 13];
+        for (int q = 0; q < 8; q++) o.prof[q] = g2[DNA_MAX_CKPT * 5 + 14 + q];
+        // static FP64 cost model (counted on the restated worker source): distance and
+        // cut-off test 9, sqrt + first sampling stage 6, Brownian re-pair step
+        // (3 x [log 20 + cos 25 + sqrt + 4] + sqrt(2 D dt) 3 + distance 9 + sampling 6) 167
+        o.algo_flops = 9.0 * (double)o.n_pair_cand + 6.0 * (double)o.n_pair_sampled + 167.0 * (double)o.n_repair;
+        o.t_ms = ms;
+        o.t_kernel_ms = kms;
+    }
+}
+
+static void init_outs(uint32_t n_inst, const std::vector<uint64_t> &off, const double *t_ckpt, int nc, dna_chem_out *outs)
+{
+    for (uint32_t i = 0; i < n_inst; i++) {
+        memset(&outs[i], 0, sizeof(dna_chem_out));
+        outs[i].n_ckpt = nc;
+        for (int k = 0; k < nc; k++) outs[i].t_ckpt[k] = t_ckpt[k];
+        outs[i].chem_n = off[i + 1] - off[i];
+    }
+}
+
+// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
+static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
+                          const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
+                          dna_chem_out *outs, const uint8_t *sel = nullptr, uint32_t n_sel = 0,
+                          uint32_t *track_tally = nullptr)
+{
+    hipStream_t st = c->stream;
+    const uint32_t n_inst = (uint32_t)off.size() - 1;
+    int rc = ensure_const(c); if (rc) return rc;
+    init_outs(n_inst, off, t_ckpt, nc, outs);
+    if (rad_n == 0) return DNA_OK;
+    DNA_HIP(hipEventRecord(c->ev[5], st));
+    IrtGroup G;
+    rc = irt_group(c, d_rad, rad_n, off, sel, n_sel, G); if (rc) return rc;
+    const uint32_t ntr = (uint32_t)G.tr.size();
     for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
-    for (auto &t : tr) outs[t.inst].n_tracks++;
+    for (auto &t : G.tr) outs[t.inst].n_tracks++;
     if (ntr == 0) return DNA_OK;
     int ev_mult = 8;
     for (int attempt = 0; attempt < 4; attempt++) {
         uint64_t part = 0, evs = 0;
-        for (auto &t : tr) {
-            uint32_t capg = 2 * maxN[t.inst] + 512;
-            uint32_t capt = 2 * t.n + 512;
-            if (attempt > 0) capt = capg;           // after an overflow use the worker's CAP
-            t.cap_global = capg; t.cap = capt < capg ? capt : capg;
-            t.part_off = part; part += t.cap;
-            t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
-            t.ev_off = evs; evs += t.ev_cap;
-        }
-        size_t bytes_part = part * (16 + 16 + 8 + 4 * 7 + 1);
-        DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch,
-                            std::max<size_t>(bytes_part + 256, rad_n / 1024 * 4 + 64)));
+        irt_size(G, attempt, ev_mult, part, evs);
+        DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch, bytes_scratch(part, rad_n)));
         DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + 256));
-        // layout of the track/bucket block inside irt.tally
-        size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
-        size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
-        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally,
-                            tally_bytes + ntr * sizeof(IrtTrack) + (K + 1) * 4 + (size_t)run * 4 + tt_bytes + 1280));
-        char *tbase = (char *)c->irt.tally;
-        unsigned long long *d_tally = (unsigned long long *)tbase;
-        IrtTrack *d_tr = (IrtTrack *)(tbase + ((tally_bytes + 255) & ~(size_t)255));
-        uint32_t *d_bst = (uint32_t *)((char *)d_tr + ((ntr * sizeof(IrtTrack) + 255) & ~(size_t)255));
-        uint32_t *d_idx = d_bst + ((K + 1 + 63) & ~(uint64_t)63);
-        uint32_t *d_tt = track_tally ? d_idx + (((uint64_t)run + 63) & ~(uint64_t)63) : nullptr;
-        DNA_HIP(hipMemsetAsync(d_tally, 0, tally_bytes, st));
-        DNA_HIP(hipMemcpyAsync(d_tr, tr.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
-        DNA_HIP(hipMemcpyAsync(d_bst, bstart.data(), (K + 1) * 4, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_bucket, dim3(gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, d_bst, g.firstj, g.cursor,
-                           contiguous ? 1 : 0, d_idx);
-        DNA_HIP(hipMemsetAsync(g.misc + 2, 0, 4, st));
-        IrtArgs A;
-        char *sp = (char *)c->irt.scratch;
-        A.rad = d_rad; A.idx = d_idx; A.tracks = d_tr;
-        A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
-        A.lst = (uint32_t *)(sp + part * 40); A.slot = (uint32_t *)(sp + part * 64); A.alive = (uint8_t *)(sp + part * 68);
-        A.ev_t = (double *)c->irt.events; A.ev_ab = (uint2 *)((char *)c->irt.events + evs * 8);
-        A.pend = (uint4 *)((char *)c->irt.events + evs * 16);
-        A.tally = d_tally; A.track_tally = d_tt; A.overflow = g.misc + 2;
-        for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
-        A.nc = nc; A.seed = seed; A.sort_tracks = contiguous ? 0 : 1;
-        // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in
-        // global memory above; the event loop reuses that LDS for alive flags and events
-        A.lds_slots = std::min<uint32_t>(maxn_all, 3072u);
-        size_t shmem = (size_t)A.lds_slots * 16;
-        auto launch = [&](uint32_t grid, int block) {
-            if (contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(grid), dim3(block), shmem, st, A);
-            else hipLaunchKernelGGL(k_irt<true>, dim3(grid), dim3(block), shmem, st, A);
-        };
-        int block = c->irt_block;
-        if (block == 0) {
-            // autotune per track-size class (log2 of the mean particles per track): the
-            // first launch of a class times each workgroup size on its first tracks (the
-            // results do not depend on the size; the trial tallies are cleared)
-            int cls = 0;
-            for (uint64_t m = run / ntr; m > 1; m >>= 1) cls++;
-            auto it = c->irt_tuned.find(cls);
-            if (it != c->irt_tuned.end()) block = it->second;
-            else if (ntr >= 1024) {
-                const uint32_t K = std::min<uint32_t>(ntr, 3072u);
-                float best = 1e30f;
-                for (int cand : {256, 128, 64}) {
-                    DNA_HIP(hipEventRecord(c->ev[0], st));
-                    launch(K, cand);
-                    DNA_HIP(hipGetLastError());
-                    DNA_HIP(hipEventRecord(c->ev[1], st));
-                    DNA_HIP(hipEventSynchronize(c->ev[1]));
-                    float ms = 0;
-                    DNA_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-                    if (ms < best) { best = ms; block = cand; }
-                }
-                c->irt_tuned[cls] = block;
-                DNA_HIP(hipMemsetAsync(d_tally, 0, tally_bytes, st));
-                DNA_HIP(hipMemsetAsync(g.misc + 2, 0, 4, st));
-            } else block = (run / ntr < 512) ? 128 : 256;
-        }
+        const size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
+        const size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
+        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, bytes_block(G, tally_bytes, tt_bytes)));
+        IrtBufs b;
+        b.scratch = (char *)c->irt.scratch; b.events = (char *)c->irt.events;
+        carve_block(G, (char *)c->irt.tally, tally_bytes, track_tally != nullptr, b);
+        rc = irt_prepare(c, G, b); if (rc) return rc;
+        IrtArgs A = irt_args(G, d_rad, b, part, evs, t_ckpt, nc, seed);
+        int block = 256;
+        rc = irt_pick_block(c, G, A, b, block); if (rc) return rc;
         c->irt_last_block = block;
         DNA_HIP(hipEventRecord(c->ev[7], st));
-        launch(ntr, block);
+        irt_launch(G, A, ntr, block, st);
         DNA_HIP(hipEventRecord(c->ev[4], st));
         DNA_HIP(hipGetLastError());
         uint32_t ovf = 0;
-        DNA_HIP(hipMemcpyAsync(&ovf, g.misc + 2, 4, hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipMemcpyAsync(&ovf, b.ovf, 4, hipMemcpyDeviceToHost, st));
         std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE);
-        DNA_HIP(hipMemcpyAsync(ht.data(), d_tally, tally_bytes, hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipMemcpyAsync(ht.data(), b.tally, tally_bytes, hipMemcpyDeviceToHost, st));
         std::vector<uint32_t> htt(track_tally ? (size_t)ntr * nc * 5 : 0);
-        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), d_tt, tt_bytes, hipMemcpyDeviceToHost, st));
+        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), b.tt, tt_bytes, hipMemcpyDeviceToHost, st));
         DNA_HIP(hipEventRecord(c->ev[6], st));
         DNA_HIP(hipStreamSynchronize(st));
         if (ovf & 2u) { ev_mult *= 4; continue; }
@@ -1290,34 +1415,13 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         if (track_tally) {
             memset(track_tally, 0, (size_t)n_sel * nc * 5 * 4);
             for (uint32_t j = 0; j < ntr; j++)
-                if (tr[j].inst == 0 && tr[j].pid < n_sel)
-                    memcpy(track_tally + (size_t)tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
+                if (G.tr[j].inst == 0 && G.tr[j].pid < n_sel)
+                    memcpy(track_tally + (size_t)G.tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
         }
         float ms = 0, kms = 0;
         hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
         hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
-        for (uint32_t i = 0; i < n_inst; i++) {
-            const unsigned long long *g2 = &ht[(size_t)i * TALLY_STRIDE];
-            dna_chem_out &o = outs[i];
-            double per100 = (double)n_therm[i] * (double)E[i] / 100.0;
-            for (int k = 0; k < nc; k++) for (int s = 0; s < 5; s++) {
-                o.tally[k][s] = g2[k * 5 + s];
-                o.G[k][s] = per100 > 0 ? (double)o.tally[k][s] / per100 : 0.0;
-            }
-            for (int r = 0; r < 9; r++) o.rxn_counts[r] = g2[DNA_MAX_CKPT * 5 + r];
-            o.n_reacted = g2[DNA_MAX_CKPT * 5 + 9];
-            o.n_events = g2[DNA_MAX_CKPT * 5 + 10];
-            o.n_pair_cand = g2[DNA_MAX_CKPT * 5 + 11];
-            o.n_pair_sampled = g2[DNA_MAX_CKPT * 5 + 12];
-            o.n_repair = g2[DNA_MAX_CKPT * 5 + 13];
-            for (int q = 0; q < 8; q++) o.prof[q] = g2[DNA_MAX_CKPT * 5 + 14 + q];
-            // static FP64 cost model (counted on the restated worker source): distance and
-            // cut-off test 9, sqrt + first sampling stage 6, Brownian re-pair step
-            // (3 x [log 20 + cos 25 + sqrt + 4] + sqrt(2 D dt) 3 + distance 9 + sampling 6) 167
-            o.algo_flops = 9.0 * (double)o.n_pair_cand + 6.0 * (double)o.n_pair_sampled + 167.0 * (double)o.n_repair;
-            o.t_ms = ms;
-            o.t_kernel_ms = kms;
-        }
+        irt_outputs(G, ht.data(), n_therm, E, nc, ms, kms, outs);
         return DNA_OK;
     }
     set_error("dna_run_irt: event/particle capacity kept overflowing");
@@ -1350,6 +1454,15 @@ extern "C" int dna_run_irt(dna_ctx *c, const float *rad, uint64_t rad_n, uint32_
     return run_irt_device(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, seed, out);
 }
 
+static int check_batch_args(const char *fn, const uint64_t *inst_off, uint32_t n_inst, std::vector<uint64_t> &off)
+{
+    off.assign(inst_off, inst_off + n_inst + 1);
+    for (uint32_t i = 0; i < n_inst; i++)
+        if (off[i + 1] < off[i]) { set_error(std::string(fn) + ": offsets must be non-decreasing"); return DNA_ERR_ARG; }
+    for (auto &o : off) o -= inst_off[0];
+    return DNA_OK;
+}
+
 extern "C" int dna_run_irt_batch(dna_ctx *c, const float *rad, const uint64_t *inst_off, uint32_t n_inst,
                                  const uint32_t *n_therm, const float *E_eV, const double *t_ckpt, int n_ckpt,
                                  uint64_t seed, dna_chem_out *out)
@@ -1357,13 +1470,12 @@ extern "C" int dna_run_irt_batch(dna_ctx *c, const float *rad, const uint64_t *i
     if (!c || !out || !rad || !inst_off || !n_therm || !E_eV || !t_ckpt || n_inst == 0 || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) {
         set_error("dna_run_irt_batch: bad arguments"); return DNA_ERR_ARG;
     }
-    std::vector<uint64_t> off(inst_off, inst_off + n_inst + 1);
-    for (uint32_t i = 0; i < n_inst; i++) if (off[i + 1] < off[i]) { set_error("dna_run_irt_batch: offsets must be non-decreasing"); return DNA_ERR_ARG; }
+    std::vector<uint64_t> off;
+    int rc = check_batch_args("dna_run_irt_batch", inst_off, n_inst, off); if (rc) return rc;
     DNA_HIP(hipSetDevice(c->device));
-    uint64_t rad_n = off[n_inst] - off[0];
-    for (auto &o : off) o -= inst_off[0];
+    uint64_t rad_n = off[n_inst];
     const float4 *d_rad = nullptr;
-    int rc = stage_rad(c, rad + 4 * inst_off[0], rad_n, &d_rad);
+    rc = stage_rad(c, rad + 4 * inst_off[0], rad_n, &d_rad);
     if (rc) return rc;
     return run_irt_device(c, d_rad, rad_n, off, n_therm, E_eV, t_ckpt, n_ckpt, seed, out);
 }
@@ -1386,4 +1498,158 @@ extern "C" int dna_run_irt_tracks(dna_ctx *c, const float *rad, uint64_t rad_n, 
     if (!sel && track_tally) { all.assign(n_prim, 1); sel = all.data(); }
     return run_irt_device(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, seed, out, sel,
                           sel ? n_prim : 0, track_tally);
+}
+
+// ------------------------------------------------------------ captured batches
+// A plan holds its own copy of the radicals, the track list and every buffer of the launch,
+// and one instantiated hipGraph: clear the tallies (a fill kernel: no memset nodes), copy
+// the seed in, the IRT kernel, copy tallies and the overflow word out.  A replay with a new
+// seed runs the same instances with fresh random streams.
+struct dna_irt_plan {
+    dna_ctx *c = nullptr;
+    IrtGroup G;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> n_therm;
+    std::vector<float> E;
+    std::vector<double> tck;
+    int nc = 0;
+    float4 *rad = nullptr;
+    char *buf = nullptr;
+    unsigned long long *h_tally = nullptr;   // pinned: tallies then the overflow word
+    uint64_t *h_seed = nullptr;              // pinned
+    uint64_t *d_seed = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int block = 256, ev_mult = 8, attempt = 0;
+    size_t tally_bytes = 0;
+};
+
+static void plan_release_graph(dna_irt_plan *P)
+{
+    if (P->exec) { hipGraphExecDestroy(P->exec); P->exec = nullptr; }
+    if (P->graph) { hipGraphDestroy(P->graph); P->graph = nullptr; }
+    if (P->buf) { hipFree(P->buf); P->buf = nullptr; }
+}
+
+// (re)build buffers and graph for the current ev_mult / attempt
+static int plan_build(dna_irt_plan *P)
+{
+    dna_ctx *c = P->c;
+    hipStream_t st = c->stream;
+    plan_release_graph(P);
+    int rc = irt_group(c, P->rad, P->G.rad_n, P->off, nullptr, 0, P->G); if (rc) return rc;
+    IrtGroup &G = P->G;
+    const uint32_t ntr = (uint32_t)G.tr.size();
+    uint64_t part = 0, evs = 0;
+    irt_size(G, P->attempt, P->ev_mult, part, evs);
+    P->tally_bytes = (size_t)G.n_inst * TALLY_STRIDE * 8;
+    const size_t s_bytes = (bytes_scratch(part, G.rad_n) + 255) & ~(size_t)255;
+    const size_t e_bytes = (evs * 32 + 256 + 255) & ~(size_t)255;
+    const size_t b_bytes = bytes_block(G, P->tally_bytes, 0);
+    DNA_HIP(hipMalloc((void **)&P->buf, s_bytes + e_bytes + b_bytes + 256));
+    IrtBufs b;
+    b.scratch = P->buf; b.events = P->buf + s_bytes;
+    carve_block(G, P->buf + s_bytes + e_bytes, P->tally_bytes, false, b);
+    P->d_seed = (uint64_t *)(P->buf + s_bytes + e_bytes + b_bytes);
+    rc = irt_prepare(c, G, b); if (rc) return rc;
+    IrtArgs A = irt_args(G, P->rad, b, part, evs, P->tck.data(), P->nc, 0);
+    if (ntr) { rc = irt_pick_block(c, G, A, b, P->block); if (rc) return rc; }
+    A.seed_ptr = P->d_seed;
+    DNA_HIP(hipStreamSynchronize(st));
+    // capture
+    DNA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    const size_t words = P->tally_bytes / 8 + 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, b.tally, words);
+    hipMemcpyAsync(P->d_seed, P->h_seed, 8, hipMemcpyHostToDevice, st);
+    if (ntr) irt_launch(G, A, ntr, P->block, st);
+    hipMemcpyAsync(P->h_tally, b.tally, P->tally_bytes + 8, hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamEndCapture(st, &P->graph);
+    if (e != hipSuccess) { set_error(std::string("dna_irt_plan: capture failed: ") + hipGetErrorString(e)); return DNA_ERR_HIP; }
+    DNA_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return DNA_OK;
+}
+
+extern "C" void dna_irt_plan_destroy(dna_irt_plan *P)
+{
+    if (!P) return;
+    hipStreamSynchronize(P->c->stream);
+    plan_release_graph(P);
+    if (P->rad) hipFree(P->rad);
+    if (P->h_tally) hipHostFree(P->h_tally);
+    if (P->h_seed) hipHostFree(P->h_seed);
+    delete P;
+}
+
+extern "C" int dna_irt_plan_create(dna_ctx *c, const float *rad, const uint64_t *inst_off, uint32_t n_inst,
+                                   const uint32_t *n_therm, const float *E_eV, const double *t_ckpt, int n_ckpt,
+                                   dna_irt_plan **out)
+{
+    if (!c || !out || !inst_off || !n_therm || !E_eV || !t_ckpt || n_inst == 0 || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) {
+        set_error("dna_irt_plan_create: bad arguments"); return DNA_ERR_ARG;
+    }
+    std::vector<uint64_t> off;
+    int rc = check_batch_args("dna_irt_plan_create", inst_off, n_inst, off); if (rc) return rc;
+    const uint64_t rad_n = off[n_inst];
+    if (rad_n == 0) { set_error("dna_irt_plan_create: no radical entries"); return DNA_ERR_ARG; }
+    if (!rad && inst_off[0] + rad_n > c->rad_n) {
+        set_error("dna_irt_plan_create: rad == NULL but fewer radicals are resident"); return DNA_ERR_ARG;
+    }
+    DNA_HIP(hipSetDevice(c->device));
+    rc = ensure_const(c); if (rc) return rc;
+    dna_irt_plan *P = new dna_irt_plan();
+    P->c = c; P->off = off; P->nc = n_ckpt;
+    P->n_therm.assign(n_therm, n_therm + n_inst);
+    P->E.assign(E_eV, E_eV + n_inst);
+    P->tck.assign(t_ckpt, t_ckpt + n_ckpt);
+    P->G.rad_n = rad_n;
+    hipError_t e = hipMalloc((void **)&P->rad, rad_n * 16 + 16);
+    if (e == hipSuccess) {
+        if (rad) e = hipMemcpy(P->rad, rad + 4 * inst_off[0], rad_n * 16, hipMemcpyHostToDevice);
+        else e = hipMemcpy(P->rad, c->d_rad + inst_off[0], rad_n * 16, hipMemcpyDeviceToDevice);
+    }
+    if (e == hipSuccess) e = hipHostMalloc((void **)&P->h_tally, (size_t)n_inst * TALLY_STRIDE * 8 + 64, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&P->h_seed, 64, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        set_error(std::string("dna_irt_plan_create: ") + hipGetErrorString(e));
+        dna_irt_plan_destroy(P);
+        return DNA_ERR_HIP;
+    }
+    rc = plan_build(P);
+    if (rc) { dna_irt_plan_destroy(P); return rc; }
+    *out = P;
+    return DNA_OK;
+}
+
+extern "C" int dna_irt_plan_run(dna_irt_plan *P, uint64_t seed, dna_chem_out *out)
+{
+    if (!P || !out) { set_error("dna_irt_plan_run: bad arguments"); return DNA_ERR_ARG; }
+    dna_ctx *c = P->c;
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    for (int tries = 0; tries < 4; tries++) {
+        DNA_HIP(hipStreamSynchronize(st));
+        *P->h_seed = seed;
+        DNA_HIP(hipEventRecord(c->ev[5], st));
+        DNA_HIP(hipGraphLaunch(P->exec, st));
+        DNA_HIP(hipEventRecord(c->ev[6], st));
+        DNA_HIP(hipStreamSynchronize(st));
+        const uint32_t ovf = (uint32_t)P->h_tally[P->tally_bytes / 8];
+        if (ovf) {
+            // capacities exceeded: rebuild with the worker's CAP and more event slots
+            if (ovf & 2u) P->ev_mult *= 4;
+            P->attempt = 1;
+            int rc = plan_build(P); if (rc) return rc;
+            continue;
+        }
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+        init_outs(P->G.n_inst, P->off, P->tck.data(), P->nc, out);
+        for (uint32_t i = 0; i < P->G.n_inst; i++) out[i].n_tracks = 0;
+        for (auto &t : P->G.tr) out[t.inst].n_tracks++;
+        irt_outputs(P->G, P->h_tally, P->n_therm.data(), P->E.data(), P->nc, ms, ms, out);
+        c->irt_last_block = P->block;
+        return DNA_OK;
+    }
+    set_error("dna_irt_plan_run: event/particle capacity kept overflowing");
+    return DNA_ERR_CAPACITY;
 }

@@ -42,6 +42,10 @@ def lib():
     L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
                             C.POINTER(DnaScoreOut)]
     L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
+    L.dna_irt_plan_create.argtypes = [vp, vp, vp, u32, vp, vp, vp, C.c_int, C.POINTER(vp)]
+    L.dna_irt_plan_run.argtypes = [vp, u64, vp]
+    L.dna_irt_plan_destroy.argtypes = [vp]
+    L.dna_irt_plan_destroy.restype = None
     L.dna_set_option.argtypes = [vp, C.c_int, C.c_int]
     L.dna_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
@@ -66,7 +70,8 @@ def lib():
 
 EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_version",
                     "dna_run_transport", "dna_run_irt", "dna_run_irt_batch", "dna_score",
-                    "dna_hash_cells", "dna_set_option", "dna_get_option", "dna_run_irt_tracks", "dna_surrogate_create",
+                    "dna_hash_cells", "dna_irt_plan_create", "dna_irt_plan_run",
+                    "dna_irt_plan_destroy", "dna_set_option", "dna_get_option", "dna_run_irt_tracks", "dna_surrogate_create",
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
@@ -178,6 +183,10 @@ class Context:
             _ptr(tt), C.byref(o)))
         return o, tt
 
+    def irt_plan(self, rad, inst_off, n_therm, E, t_ckpt):
+        """A captured batch (IrtPlan); rad None = the resident radicals."""
+        return IrtPlan(self, rad, inst_off, n_therm, E, t_ckpt)
+
     # ---- scoring
     def score(self, target, rad=None, rad_n=0, lcg_state=0x12345678, want_hits=False):
         r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
@@ -205,6 +214,39 @@ class Context:
         _check("dna_hash_cells", lib().dna_hash_cells(self.h, _ptr(p), n, cell_nm, bits, _ptr(key),
                                                       _ptr(start), _ptr(order)))
         return key, start, order
+
+
+class IrtPlan:
+    """dna_irt_plan: IRT instances planned once and captured as a hipGraph; run(seed)
+    replays it."""
+
+    def __init__(self, ctx, rad, inst_off, n_therm, E, t_ckpt):
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        off = np.ascontiguousarray(inst_off, np.uint64)
+        self._keep = (np.ascontiguousarray(n_therm, np.uint32), np.ascontiguousarray(E, np.float32),
+                      np.ascontiguousarray(t_ckpt, np.float64))
+        nt, Ee, tc = self._keep
+        self.n_inst = off.size - 1
+        h = C.c_void_p()
+        _check("dna_irt_plan_create", lib().dna_irt_plan_create(ctx.h, _ptr(r), _ptr(off), self.n_inst, _ptr(nt),
+                                                                _ptr(Ee), _ptr(tc), tc.size, C.byref(h)))
+        self.h, self.ctx = h, ctx
+
+    def run(self, seed):
+        outs = (DnaChemOut * self.n_inst)()
+        _check("dna_irt_plan_run", lib().dna_irt_plan_run(self.h, seed, outs))
+        return list(outs)
+
+    def close(self):
+        if self.h:
+            lib().dna_irt_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Surrogate:
