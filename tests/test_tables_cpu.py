@@ -141,3 +141,25 @@ def test_vib_and_dea():
     for r in np.linspace(0, 0.999, 50):
         m = O.lib().oracle_sample_vib_mode(0.3, float(r))
         assert tab("VIB_LEV")[m] <= 0.3 or m == 0
+
+
+def test_born_guide_bracket_gives_the_full_bisection_index():
+    """The transport bisects a Born CDF row from a guide-table bracket: for r in
+    [k/64, (k+1)/64), start at [g[k], g[k+1]] with g[k] = #{j < 249 : C[j] < k/64} and
+    g[64] = 249.  On every row, for r at every bucket edge, every CDF value and random
+    points (r = 1.0 included), the bracket holds the index of the full-row bisection."""
+    rng = np.random.default_rng(3)
+    K = 64
+    for s in range(5):
+        rows = tab(f"XWC{s}").reshape(58, 250)
+        for C in rows:
+            C = C[:249]
+            g = [int(np.searchsorted(C, np.float32(k / K), side="left")) for k in range(K)] + [249]
+            rs = np.concatenate([np.arange(K + 1, dtype=np.float32) / K, C, np.nextafter(C, np.float32(2)),
+                                 rng.random(200).astype(np.float32), [np.float32(1.0), np.float32(0.0)]])
+            for r in rs.astype(np.float32):
+                if not (0.0 <= r <= 1.0):
+                    continue
+                full = int(np.searchsorted(C, r, side="right"))      # #{C[j] <= r}
+                k = min(int(np.float32(r) * np.float32(K)), K - 1)
+                assert g[k] <= full <= g[k + 1], (s, float(r), k, g[k], full, g[k + 1])

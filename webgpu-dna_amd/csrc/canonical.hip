@@ -113,6 +113,7 @@ __global__ void k_scatter(const float4 *raw, const uint64_t *key, uint32_t rad_n
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= rad_n) return;
     uint64_t kk = key[i];
+    if (kk == ~0ull) return;               // a slot reserved but not used
     uint32_t pid = (uint32_t)(kk >> 40), k = (uint32_t)(kk >> 20) & 0xFFFFFu, seq = (uint32_t)kk & 0xFFFFFu;
     out[off[base_p[pid] + k] + seq] = raw[i];
 }

@@ -31,6 +31,7 @@ struct dna_ctx {
     hipStream_t stream = nullptr;
     dna_tables h_tab;
     dna_tables *d_tab = nullptr;
+    uint8_t *d_guide = nullptr;          // Born CDF guide table
     uint32_t *d_ctr = nullptr, *h_ctr = nullptr;
     unsigned long long *d_u64 = nullptr;
     unsigned long long h_u64_tmp = 0;

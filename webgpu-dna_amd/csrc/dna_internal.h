@@ -24,8 +24,18 @@ struct SecRec {
 };
 
 // Device buffers and counters used by one transport run.
+// Guide table of the Born CDF rows: for shell s, energy row e and k = 0..64,
+// GUIDE[s][e][k] = the number of the first 249 entries of the row below k/64 (64 for k = 64
+// is replaced by 249): a bisection for r in [k/64, (k+1)/64) can start from
+// [GUIDE[k], GUIDE[k+1]] and ends on the same index as over the whole row.
+#define DNA_GUIDE_K 64
+#define DNA_GUIDE_BYTES (5 * DNA_N_WE * (DNA_GUIDE_K + 1))
+#define DNA_GUIDE_PAD ((DNA_GUIDE_BYTES + 3) & ~3)
+void build_born_guide(const dna_tables &t, uint8_t *guide /* DNA_GUIDE_PAD bytes */);
+
 struct TransportDev {
     const dna_tables *tab;      // device copy of the tables
+    const uint8_t *guide;       // Born CDF guide table (DNA_GUIDE_PAD bytes)
     dna_params p;
     const float *energies;      // [n] or null
     dna_result *results;        // [n]
@@ -45,6 +55,7 @@ enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, 
        C_SST = 8 /* 8..15 sec_stats */, C_DBG = 16 /* 16..23 dbg */, C_PNEXT = 24, C_SNEXT = 25,
        C_NCTR = 32 };
 
+size_t transport_slot_headroom(uint32_t n);
 hipError_t launch_phase_a(const TransportDev &d, hipStream_t st);
 hipError_t launch_phase_b(const TransportDev &d, uint32_t sec_n, hipStream_t st);
 

@@ -75,6 +75,12 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     DNA_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     DNA_HIP(hipMalloc((void **)&c->d_tab, sizeof(dna_tables)));
     DNA_HIP(hipMemcpy(c->d_tab, &c->h_tab, sizeof(dna_tables), hipMemcpyHostToDevice));
+    {
+        std::vector<uint8_t> guide(DNA_GUIDE_PAD, 0);
+        dna::build_born_guide(c->h_tab, guide.data());
+        DNA_HIP(hipMalloc((void **)&c->d_guide, DNA_GUIDE_PAD));
+        DNA_HIP(hipMemcpy(c->d_guide, guide.data(), DNA_GUIDE_PAD, hipMemcpyHostToDevice));
+    }
     DNA_HIP(hipMalloc((void **)&c->d_ctr, C_NCTR * sizeof(uint32_t)));
     DNA_HIP(hipHostMalloc((void **)&c->h_ctr, C_NCTR * sizeof(uint32_t), hipHostMallocDefault));
     DNA_HIP(hipMalloc((void **)&c->d_u64, 64 * sizeof(unsigned long long)));
@@ -82,6 +88,24 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     *out = c;
     return DNA_OK;
 }
+
+namespace dna {
+void build_born_guide(const dna_tables &t, uint8_t *guide)
+{
+    for (int s = 0; s < 5; s++)
+        for (int e = 0; e < (int)DNA_N_WE; e++) {
+            const float *C = t.XWC[s] + (size_t)e * DNA_N_WR;
+            uint8_t *g = guide + ((size_t)s * DNA_N_WE + e) * (DNA_GUIDE_K + 1);
+            for (int k = 0; k < DNA_GUIDE_K; k++) {
+                const float thr = (float)k / (float)DNA_GUIDE_K;
+                uint32_t n = 0;
+                while (n < DNA_N_WR - 1 && C[n] < thr) n++;
+                g[k] = (uint8_t)n;
+            }
+            g[DNA_GUIDE_K] = (uint8_t)(DNA_N_WR - 1);
+        }
+}
+}  // namespace dna
 
 extern "C" int dna_set_option(dna_ctx *c, int option, int value)
 {
@@ -118,7 +142,7 @@ extern "C" void dna_ctx_destroy(dna_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
-    void *dev[] = {c->d_tab, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
+    void *dev[] = {c->d_tab, c->d_guide, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
                    c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
                    c->irt.part, c->irt.rad_in, c->irt.tally, c->sc.hitbits, c->sc.flags, c->sc.rank,
@@ -176,6 +200,7 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
     // capacity: reference yields are ~0.13 radical entries and ~0.0145 secondaries per eV;
     // grow and repeat the (deterministic) run if a count overflows
     size_t rad_cap = p.max_rad ? p.max_rad : (size_t)(0.2 * sumE) + 65536;
+    rad_cap += transport_slot_headroom(p.n);       // chunk tails left unused by each wavefront
     size_t sec_cap = p.max_sec ? p.max_sec : (size_t)(0.025 * sumE) + 4096;
     const size_t HARD = (size_t)0xFFFFFFF0u;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -188,8 +213,9 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         DNA_HIP(hipMemsetAsync(c->d_ctr, 0, C_NCTR * sizeof(uint32_t), st));
         DNA_HIP(hipMemsetAsync(c->d_dose, 0, v3 * sizeof(uint32_t), st));
         DNA_HIP(hipMemsetAsync(c->d_u64, 0, 64 * sizeof(unsigned long long), st));
+        DNA_HIP(hipMemsetAsync(c->d_rad_key, 0xFF, rad_cap * sizeof(uint64_t), st));   // unused slots: invalid key
         TransportDev d;
-        d.tab = c->d_tab; d.p = p; d.energies = energies ? c->d_energies : nullptr;
+        d.tab = c->d_tab; d.guide = c->d_guide; d.p = p; d.energies = energies ? c->d_energies : nullptr;
         d.results = c->d_results; d.dose = c->d_dose; d.rad_raw = c->d_rad_raw; d.rad_key = c->d_rad_key;
         d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
         d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
@@ -213,6 +239,10 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         hipLaunchKernelGGL(k_dose_sum, dim3(1024), dim3(256), 0, st, c->d_dose, v3, c->d_u64);
         DNA_HIP(hipGetLastError());
         DNA_HIP(hipEventRecord(c->ev[4], st));
+        // radicals emitted = the total of the per-emitter counts (slots include unused ones)
+        uint32_t rad_n = 0;
+        DNA_HIP(hipMemcpyAsync(&rad_n, c->d_cnt + (size_t)p.n + sec_n, 4, hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipStreamSynchronize(st));
         // copies
         unsigned long long dsum = 0;
         DNA_HIP(hipMemcpyAsync(&c->h_u64_tmp, c->d_u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -227,23 +257,24 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
             DNA_HIP(hipMemcpyAsync(c->h_dose, c->d_dose, v3 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
             out->dose = c->h_dose;
         }
-        if ((copy_flags & DNA_COPY_RAD) && rad_raw) {
-            DNA_HIP(grow_host(c->h_rad, c->hcap_rad, (size_t)rad_raw * 4));
-            DNA_HIP(hipMemcpyAsync(c->h_rad, c->d_rad, (size_t)rad_raw * 16, hipMemcpyDeviceToHost, st));
+        if ((copy_flags & DNA_COPY_RAD) && rad_n) {
+            DNA_HIP(grow_host(c->h_rad, c->hcap_rad, (size_t)rad_n * 4));
+            DNA_HIP(hipMemcpyAsync(c->h_rad, c->d_rad, (size_t)rad_n * 16, hipMemcpyDeviceToHost, st));
             out->rad_buf = c->h_rad;
         }
         DNA_HIP(hipStreamSynchronize(st));
         dsum = c->h_u64_tmp;
         for (int i = 0; i < 8; i++) { out->counters[i] = c->h_ctr[i]; out->sec_stats[i] = c->h_ctr[C_SST + i]; out->dbg[i] = c->h_ctr[C_DBG + i]; }
+        out->counters[C_RAD] = rad_n;
         out->dose_sum = dsum;
-        out->rad_n_raw = rad_raw; out->rad_n_stored = rad_raw; out->rad_dropped = 0;
+        out->rad_n_raw = rad_n; out->rad_n_stored = rad_n; out->rad_dropped = 0;
         out->sec_n_raw = sec_raw; out->sec_n = sec_n; out->sec_dropped = 0;
         float ta = 0, tb = 0, tt = 0;
         hipEventElapsedTime(&ta, c->ev[0], c->ev[1]);
         hipEventElapsedTime(&tb, c->ev[2], c->ev[3]);
         hipEventElapsedTime(&tt, c->ev[0], c->ev[4]);
         out->t_prim_ms = ta; out->t_sec_ms = tb; out->t_total_ms = tt;
-        c->rad_n = rad_raw;
+        c->rad_n = rad_n;
         c->last_n = p.n;
         return DNA_OK;
     }

@@ -42,13 +42,15 @@ struct SmallTab {
     float XWE[DNA_N_WE];
     float XVE[DNA_N_VIB], XVS[DNA_N_VIB], XVMF[DNA_N_VIB * 9], VIB_LEV[9];
     float LOG_XE0, INV_LOG_XE_STEP, LOG_XAE0, INV_LOG_XAE_STEP;
+    float DEA_E[14], DEA_S[14];
+    uint8_t GUIDE[DNA_GUIDE_PAD];
 };
 
 __device__ __forceinline__ float fmx(float a, float b) { return a > b ? a : b; }
 __device__ __forceinline__ float fmn(float a, float b) { return a < b ? a : b; }
 __device__ __forceinline__ float clampf_(float x, float lo, float hi) { return fmn(fmx(x, lo), hi); }
 
-__device__ void stage_tables(SmallTab &S, const dna_tables *T)
+__device__ void stage_tables(SmallTab &S, const dna_tables *T, const uint8_t *guide)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     auto cp = [&](float *dst, const float *src, int n) { for (int i = tid; i < n; i += nt) dst[i] = src[i]; };
@@ -57,6 +59,8 @@ __device__ void stage_tables(SmallTab &S, const dna_tables *T)
     cp(S.XAE, T->XAE, DNA_N_AE); cp(S.XAC, T->XAC, DNA_N_AE * DNA_N_AC); cp(S.XWE, T->XWE, DNA_N_WE);
     cp(S.XVE, T->XVE, DNA_N_VIB); cp(S.XVS, T->XVS, DNA_N_VIB); cp(S.XVMF, T->XVMF, DNA_N_VIB * 9);
     cp(S.VIB_LEV, T->VIB_LEV, 9);
+    cp(S.DEA_E, c_DEA_E, 14); cp(S.DEA_S, c_DEA_S, 14);
+    for (int i = tid; i < DNA_GUIDE_PAD / 4; i += nt) ((uint32_t *)S.GUIDE)[i] = ((const uint32_t *)guide)[i];
     if (tid == 0) {
         S.LOG_XE0 = T->LOG_XE0; S.INV_LOG_XE_STEP = T->INV_LOG_XE_STEP;
         S.LOG_XAE0 = T->LOG_XAE0; S.INV_LOG_XAE_STEP = T->INV_LOG_XAE_STEP;
@@ -131,8 +135,12 @@ __device__ __forceinline__ float sample_W_sec(const SmallTab &S, const dna_table
     uint32_t iE = (fabsf(S.XWE[hi] - E) < fabsf(S.XWE[lo] - E)) ? hi : lo;
     const float *C = T->XWC[shell] + iE * DNA_N_WR;
     const float *W = T->XWT[shell] + iE * DNA_N_WR;
-    // count = #{ j < 249 : C[j] <= r }  (a prefix, the row being non-decreasing)
-    uint32_t a = 0, b = DNA_N_WR - 1;          // invariant: C[j] <= r for j < a; C[j] > r for j >= b
+    // count = #{ j < 249 : C[j] <= r }  (a prefix, the row being non-decreasing), bisected
+    // from the guide-table bracket of r (r * 64 is exact, so k = floor(64 r))
+    const uint8_t *g = S.GUIDE + (shell * DNA_N_WE + iE) * (DNA_GUIDE_K + 1);
+    uint32_t kq = (uint32_t)(r * (float)DNA_GUIDE_K);
+    kq = kq < DNA_GUIDE_K - 1u ? kq : DNA_GUIDE_K - 1u;
+    uint32_t a = g[kq], b = g[kq + 1];          // invariant: C[j] <= r for j < a; C[j] > r for j >= b
     while (a < b) { uint32_t m = (a + b) >> 1; if (C[m] <= r) a = m + 1; else b = m; }
     uint32_t jlo = a == 0 ? 0u : a - 1u;
     uint32_t jhi = (jlo + 1u < DNA_N_WR - 1u) ? jlo + 1u : DNA_N_WR - 1u;
@@ -142,23 +150,31 @@ __device__ __forceinline__ float sample_W_sec(const SmallTab &S, const dna_table
     return e0 + (e1 - e0) * f;
 }
 
+// The vibrational grid XVE is strictly increasing (checked when the tables are built), so
+// the reference's linear scans for the first i with XVE[i] <= E < XVE[i+1]
+// (helpers.wgsl:135-170) find the bin a bisection finds: the largest i with XVE[i] <= E,
+// when it is below the last point.  Returns DNA_N_VIB - 1 when there is no such bin.
+__device__ __forceinline__ uint32_t vib_bin(const SmallTab &S, float E)
+{
+    if (!(E >= S.XVE[0])) return DNA_N_VIB - 1u;
+    uint32_t a = 0, b = DNA_N_VIB;                  // XVE[a] <= E < XVE[b] (XVE[N] = +inf)
+    while (b - a > 1u) { uint32_t m = (a + b) >> 1; if (S.XVE[m] <= E) a = m; else b = m; }
+    return a;
+}
+
 __device__ __forceinline__ float xs_vib_total(const SmallTab &S, float E)
 {
     if (E < 2.0f || E > 100.0f) return 0.0f;
-    for (uint32_t i = 0; i < DNA_N_VIB - 1u; i++) {
-        if (E >= S.XVE[i] && E < S.XVE[i+1]) {
-            float f = (E - S.XVE[i]) / (S.XVE[i+1] - S.XVE[i]);
-            return fmx(0.0f, S.XVS[i] + (S.XVS[i+1] - S.XVS[i]) * f);
-        }
-    }
-    return 0.0f;
+    const uint32_t i = vib_bin(S, E);
+    if (i >= DNA_N_VIB - 1u) return 0.0f;
+    float f = (E - S.XVE[i]) / (S.XVE[i+1] - S.XVE[i]);
+    return fmx(0.0f, S.XVS[i] + (S.XVS[i+1] - S.XVS[i]) * f);
 }
 
 __device__ __forceinline__ uint32_t sample_vib_mode(const SmallTab &S, float E, float r)
 {
-    uint32_t bin = 0;
-    for (uint32_t i = 0; i < DNA_N_VIB - 1u; i++)
-        if (E >= S.XVE[i] && E < S.XVE[i+1]) { bin = i; break; }
+    uint32_t bin = vib_bin(S, E);
+    if (bin >= DNA_N_VIB - 1u) bin = 0;
     uint32_t base = bin * 9u;
     float cs[9], cum = 0.0f;
     for (uint32_t k = 0; k < 9; k++) {
@@ -172,16 +188,18 @@ __device__ __forceinline__ uint32_t sample_vib_mode(const SmallTab &S, float E, 
     return 8;
 }
 
-__device__ __forceinline__ float xs_dea(float E)
+// Melton DEA (helpers.wgsl:172-186).  The 14 energies are strictly increasing, so the
+// first i with E_i <= E < E_{i+1} is the number of E_1..E_13 at or below E; the pair
+// (E_i, S_i) is then read from LDS.
+__device__ __forceinline__ float xs_dea(const SmallTab &S, float E)
 {
     if (E < 4.0f || E > 13.0f) return 0.0f;
-    for (int i = 0; i < 13; i++) {
-        if (E >= c_DEA_E[i] && E < c_DEA_E[i+1]) {
-            float f = (E - c_DEA_E[i]) / (c_DEA_E[i+1] - c_DEA_E[i]);
-            return fmx(0.0f, c_DEA_S[i] + (c_DEA_S[i+1] - c_DEA_S[i]) * f);
-        }
-    }
-    return 0.0f;
+    int i = 0;
+#pragma unroll
+    for (int j = 1; j < 14; j++) i += (E >= S.DEA_E[j]) ? 1 : 0;
+    if (i >= 13) return 0.0f;
+    float f = (E - S.DEA_E[i]) / (S.DEA_E[i+1] - S.DEA_E[i]);
+    return fmx(0.0f, S.DEA_S[i] + (S.DEA_S[i+1] - S.DEA_S[i]) * f);
 }
 
 // ---------------------------------------------------------------- RNG
@@ -220,16 +238,103 @@ struct Sink {
     uint32_t *dose;
     float     box, vs;
     int       vc;
+    uint32_t  dvi, dacc;   // dose voxel being accumulated and its pending fixed-point sum
 };
 
+// Per-wavefront allocation state in LDS (4 wavefronts per workgroup at most): radical
+// slots {base, fill} and work items {next, end}.  A wavefront reserves radical slots in
+// chunks of RAD_CHUNK with one atomic and hands them out to its lanes, so emitting lanes
+// almost never wait on a device-wide atomic; slots left unused in a chunk keep the
+// invalid key written before the launch and are skipped by the canonical scatter.
+#define RAD_CHUNK 256u
+#define WORK_CHUNK 64u
+// Different lanes lead successive calls, so every access goes through a volatile view:
+// no value may be kept in a register from one call to the next.
+__shared__ uint32_t s_wrad_[8], s_wwork_[8];
+#define s_wrad ((volatile uint32_t *)s_wrad_)
+#define s_wwork ((volatile uint32_t *)s_wwork_)
+
+__device__ __forceinline__ void wave_state_init()
+{
+    if ((threadIdx.x & 63u) == 0) {
+        const uint32_t w = threadIdx.x >> 6;
+        s_wrad[2 * w] = 0u; s_wrad[2 * w + 1] = RAD_CHUNK;      // empty: the first use reserves
+        s_wwork[2 * w] = 0u; s_wwork[2 * w + 1] = 0u;
+    }
+}
+
+// Append n (1..3) radicals.  Slots of the lanes emitting together follow the lower-numbered
+// emitting lanes' (ranks from ballots per n, so lanes reaching here with different n are
+// handled too); the canonical scatter reorders slots anyway.
 __device__ __forceinline__ void emit_n(Sink &k, int n, const float4 *e)
 {
-    uint32_t base = atomicAdd(k.rad_ctr, (uint32_t)n);
+    const uint64_t m1 = __ballot(n == 1), m2 = __ballot(n == 2), m3 = __ballot(n == 3);
+    const uint32_t lane = __lane_id();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t all = m1 | m2 | m3;
+    const int leader = __ffsll((unsigned long long)all) - 1;
+    const uint32_t rank = __popcll(m1 & below) + 2u * __popcll(m2 & below) + 3u * __popcll(m3 & below);
+    uint32_t base = 0;
+    if ((int)lane == leader) {
+        const uint32_t w = threadIdx.x >> 6;
+        const uint32_t tot = (uint32_t)(__popcll(m1) + 2 * __popcll(m2) + 3 * __popcll(m3));   // <= 192
+        uint32_t f = s_wrad[2 * w + 1];
+        if (f + tot > RAD_CHUNK) { s_wrad[2 * w] = atomicAdd(k.rad_ctr, RAD_CHUNK); f = 0u; }
+        base = s_wrad[2 * w] + f;
+        s_wrad[2 * w + 1] = f + tot;
+    }
+    base = __shfl(base, leader, 64) + rank;
     for (int i = 0; i < n; i++) {
         uint32_t idx = base + (uint32_t)i;
         if (idx < k.rad_cap) { k.rad[idx] = e[i]; k.key[idx] = k.keybase | (uint64_t)(k.seq + i); }
     }
     k.seq += (uint32_t)n;
+}
+
+// Next work item for each calling lane from the wavefront's chunk of WORK_CHUNK items
+// (one atomic per chunk).  Items are handed out in increasing order, so a lane that
+// receives an item past the end knows that no earlier item of its wavefront is left.
+__device__ __forceinline__ uint32_t wave_claim_chunked(uint32_t *ctr)
+{
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    uint32_t old = 0, avail = 0, fresh = 0;
+    if ((int)lane == leader) {
+        const uint32_t w = threadIdx.x >> 6, cnt = (uint32_t)__popcll(m);
+        const uint32_t nx = s_wwork[2 * w], end = s_wwork[2 * w + 1];
+        old = nx; avail = end - nx;
+        if (avail >= cnt) { s_wwork[2 * w] = nx + cnt; }
+        else {
+            fresh = atomicAdd(ctr, WORK_CHUNK);
+            s_wwork[2 * w] = fresh + (cnt - avail);
+            s_wwork[2 * w + 1] = fresh + WORK_CHUNK;
+        }
+    }
+    old = __shfl(old, leader, 64); avail = __shfl(avail, leader, 64); fresh = __shfl(fresh, leader, 64);
+    return rank < avail ? old + rank : fresh + (rank - avail);
+}
+
+// Next work item for each calling lane; lanes calling together share one atomic.
+__device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr)
+{
+    const uint64_t m = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    return __shfl(base, leader, 64) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// The dose grid takes one atomic per voxel a lane leaves, not one per deposit: deposits
+// are truncated one by one, u32(dep * 100) (primary.wgsl:111), and summed per lane while
+// the voxel is unchanged; integer sums make the grid identical.
+__device__ __forceinline__ void flush_dose(Sink &k)
+{
+    if (k.dacc) atomicAdd(&k.dose[k.dvi], k.dacc);
+    k.dacc = 0u;
+    k.dvi = 0xFFFFFFFFu;
 }
 __device__ __forceinline__ void emit2(Sink &k, float4 a, float4 b) { float4 e[2] = {a, b}; emit_n(k, 2, e); }
 __device__ __forceinline__ void emit3(Sink &k, float4 a, float4 b, float4 c) { float4 e[3] = {a, b, c}; emit_n(k, 3, e); }
@@ -244,7 +349,8 @@ __device__ __forceinline__ void deposit(Sink &k, float px, float py, float pz, f
     int n = k.vc;
     if (ix < 0 || ix >= n || iy < 0 || iy >= n || iz < 0 || iz >= n) return;
     uint32_t vi = (uint32_t)((iz * n + iy) * n + ix);
-    atomicAdd(&k.dose[vi], (uint32_t)(dep * 100.0f));
+    if (vi != k.dvi) { flush_dose(k); k.dvi = vi; }
+    k.dacc += (uint32_t)(dep * 100.0f);
 }
 
 __device__ uint32_t dna_near(const dna_params &p, float px, float py, float pz)
@@ -468,7 +574,7 @@ __device__ bool prim_step(const TransportDev &d, const dna_params &p, const Smal
     float E = st.E;
     bool productive = E > 12.0f;
     float xi, xc, xl; xs_all(S, E, xi, xc, xl);
-    float s_vib = xs_vib_total(S, E), s_dea = xs_dea(E);
+    float s_vib = xs_vib_total(S, E), s_dea = xs_dea(S, E);
     float s_tot = xi + xc + xl + s_vib + s_dea;
     if (s_tot <= 0.0f) return true;
     float lambda = 1.0f / (NW_F * s_tot);
@@ -542,7 +648,7 @@ __device__ bool prim_step(const TransportDev &d, const dna_params &p, const Smal
                 if (pfl > 1e-10f) { st.dx = pfx / pfl; st.dy = pfy / pfl; st.dz = pfz / pfl; }
                 uint32_t r0 = s.next(), r1 = s.next(), r2 = s.next(), r3 = s.next();
                 st.nsec++;
-                uint32_t q = atomicAdd(&d.ctr[C_SEC], 1u);
+                uint32_t q = wave_claim(&d.ctr[C_SEC]);
                 if (q < d.sec_cap) {
                     SecRec rec;
                     rec.posE = make_float4(px, py, pz, W_sec);
@@ -580,15 +686,17 @@ __device__ bool prim_step(const TransportDev &d, const dna_params &p, const Smal
 __global__ void __launch_bounds__(256) k_phase_a(TransportDev d)
 {
     __shared__ SmallTab S;
-    stage_tables(S, d.tab);
+    stage_tables(S, d.tab, d.guide);
+    wave_state_init();
     const dna_params p = d.p;
     Sink k;
     k.rad = d.rad_raw; k.key = d.rad_key; k.rad_ctr = &d.ctr[C_RAD]; k.rad_cap = d.rad_cap;
     k.dose = d.dose; k.box = p.box; k.vs = 2.0f * p.box / (float)p.vc; k.vc = (int)p.vc;
+    k.dvi = 0xFFFFFFFFu; k.dacc = 0u;
     for (int i = 0; i < 6; i++) k.cnt[i] = 0;
     uint32_t dbg[5] = {0, 0, 0, 0, 0};
     PrimState st;
-    uint32_t idx = atomicAdd(&d.ctr[C_PNEXT], 1u);
+    uint32_t idx = wave_claim(&d.ctr[C_PNEXT]);
     bool active = idx < p.n;
     if (active) prim_init(d, p, idx, st, k);
     while (active) {
@@ -602,11 +710,12 @@ __global__ void __launch_bounds__(256) k_phase_a(TransportDev d)
             d.prim_nsec[idx] = st.nsec;
             d.prim_nrad[idx] = k.seq;
             dbg[0] += st.ni; dbg[1] += st.nx; dbg[2] += st.esc; dbg[3] += 1;
-            idx = atomicAdd(&d.ctr[C_PNEXT], 1u);
+            idx = wave_claim(&d.ctr[C_PNEXT]);
             active = idx < p.n;
             if (active) prim_init(d, p, idx, st, k);
         }
     }
+    flush_dose(k);
     for (int i = 0; i < 6; i++) wave_flush(&d.ctr[i], k.cnt[i]);
     for (int i = 0; i < 5; i++) wave_flush(&d.ctr[C_DBG + i], dbg[i]);
 }
@@ -642,7 +751,7 @@ __device__ bool sec_step(const TransportDev &d, const dna_params &p, const Small
     }
     if (fabsf(px) >= box || fabsf(py) >= box || fabsf(pz) >= box) { sst[2]++; return true; }
     float xi, xc, xl; xs_all(S, E, xi, xc, xl);
-    float s_vib = xs_vib_total(S, E), s_dea = xs_dea(E);
+    float s_vib = xs_vib_total(S, E), s_dea = xs_dea(S, E);
     float s_tot = xi + xc + xl + s_vib + s_dea;
     if (s_tot <= 0.0f) return true;
     float lambda = 1.0f / (NW_F * s_tot);
@@ -708,26 +817,29 @@ __device__ __forceinline__ void sec_load(const TransportDev &d, uint32_t q, SecS
 __global__ void __launch_bounds__(256) k_phase_b(TransportDev d, uint32_t sec_n)
 {
     __shared__ SmallTab S;
-    stage_tables(S, d.tab);
+    stage_tables(S, d.tab, d.guide);
+    wave_state_init();
     const dna_params p = d.p;
     Sink k;
     k.rad = d.rad_raw; k.key = d.rad_key; k.rad_ctr = &d.ctr[C_RAD]; k.rad_cap = d.rad_cap;
     k.dose = d.dose; k.box = p.box; k.vs = 2.0f * p.box / (float)p.vc; k.vc = (int)p.vc;
+    k.dvi = 0xFFFFFFFFu; k.dacc = 0u;
     for (int i = 0; i < 6; i++) k.cnt[i] = 0;
     uint32_t sst[5] = {0, 0, 0, 0, 0};
     SecState st;
-    uint32_t q = atomicAdd(&d.ctr[C_SNEXT], 1u);
+    uint32_t q = wave_claim_chunked(&d.ctr[C_SNEXT]);
     bool active = q < sec_n;
     if (active) sec_load(d, q, st, k);
     while (active) {
         bool done = sec_step(d, p, S, st, k, sst);
         if (done) {
             d.sec_nrad[q] = k.seq;
-            q = atomicAdd(&d.ctr[C_SNEXT], 1u);
+            q = wave_claim_chunked(&d.ctr[C_SNEXT]);
             active = q < sec_n;
             if (active) sec_load(d, q, st, k);
         }
     }
+    flush_dose(k);
     for (int i = 0; i < 6; i++) wave_flush(&d.ctr[i], k.cnt[i]);
     for (int i = 1; i < 5; i++) wave_flush(&d.ctr[C_SST + i], sst[i]);
 }
@@ -741,6 +853,15 @@ static int num_cu()
         g_num_cu = pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 256;
     }
     return g_num_cu;
+}
+
+size_t transport_slot_headroom(uint32_t n)
+{
+    // every wavefront of either phase may leave up to RAD_CHUNK reserved slots unused
+    const size_t lanes_max = (size_t)num_cu() * 4u * 64u * 4u;
+    const size_t waves_a = ((n < lanes_max ? n : lanes_max) + 63u) / 64u;
+    const size_t waves_b = lanes_max / 64u;
+    return (waves_a + waves_b) * RAD_CHUNK;
 }
 
 hipError_t launch_phase_a(const TransportDev &d, hipStream_t st)
