@@ -53,14 +53,13 @@ struct IrtArgs {
     const uint32_t *idx;          // bucketed entry indices
     const IrtTrack *tracks;
     float4 *pos;                  // (x, y, z, species) per particle slot
-    uint8_t *alive;
+    uint8_t *alive;               // per particle slot: species + 1 while alive, 0 once reacted
+    uint32_t *lst;                // 6 species lists of capacity cap per track (may hold reacted particles)
     double *tb;
     double *ev_t;
     uint2 *ev_ab;
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
-    uint32_t *lst;                // 6 species lists of capacity cap per track (alive particles)
-    uint32_t *slot;               // position of each particle in its species list
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
@@ -339,19 +338,30 @@ struct TrackShared {
     unsigned long long rxc[9];   // reaction counts
     unsigned long long n_re;
     uint32_t npend;              // queued stage-2 (SamplePDC) pairs
-    uint32_t nkeep;              // re-pairings that pass the first sampling stage
     uint32_t pend_next;          // drain cursor
     uint32_t long_n, long_next;  // pairs handed from the per-lane to the wavefront schedule
-    uint32_t llen[6];            // alive particles per species list
     uint32_t long_item[512];
     uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
     int      bri[9], nblk;                // channel blocks: reaction, units per row
     uint32_t bcpr[9], bu0[9], units;
-    // product re-pairing: segments (product p, partner species sj) over the species lists
+    // reaction tables by species pair (row * 6 + col) and by species, copied from c_irt
+    // so that lanes with different species read them from LDS
+    RxB      bnd[36];                     // cannot_react bounds of the pair's reaction
+    int8_t   rmap[36];                    // reaction index or -1
+    int8_t   nprod[9], prods[9][2];
+    double   Dd[6], sqD[6];
+    float    Df[6];
+    // the products of the current reaction
+    uint32_t prod_sp[2];
+    float4   prod_pos[2];
+    // species lists: length and reacted entries; products are appended, reacted particles
+    // stay until a list is compacted (when they outnumber the alive ones)
+    uint32_t llen[6], ldead[6], lcomp;
+    uint32_t lcnt;
+    // re-pairing segments (product p, partner species sj) over the lists
     int      nseg;
     uint32_t seg_u0[13], seg_ps[12];      // first flattened index; p << 4 | sj
-    float    seg_S[12], seg_sc[12], seg_R[12], seg_Dj[12];
-    float4   prod_pos[2];
+    uint32_t rq[IRT_BLOCK / 64][128];     // per wavefront: re-pairings that passed the bound
 };
 
 extern __shared__ float4 dyn_lds[];
@@ -374,11 +384,15 @@ __device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, 
     else atomicOr(A.overflow, 2u);
 }
 
+// A queued pair: indices a < b with their species in the top 4 bits (track capacities are
+// below 2^28 particles), and r0.
 __device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
-                                             uint32_t a, uint32_t b, double r0)
+                                             uint32_t a, uint32_t b, uint32_t sa, uint32_t sb, double r0)
 {
     uint32_t q = atomicAdd(&sh.npend, 1u);
-    if (q < T.ev_cap) A.pend[T.ev_off + q] = make_uint4(a, b, (uint32_t)__double2loint(r0), (uint32_t)__double2hiint(r0));
+    if (q < T.ev_cap)
+        A.pend[T.ev_off + q] = make_uint4(a | (sa << 28), b | (sb << 28), (uint32_t)__double2loint(r0),
+                                          (uint32_t)__double2hiint(r0));
     else atomicOr(A.overflow, 2u);
 }
 
@@ -394,43 +408,17 @@ __device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T
 // Called by all threads after a barrier; the caller resets the queue.
 struct PendPair { uint32_t a, b; double r0; int ri; double D; uint64_t ps; };
 
-// r0 of a product re-pairing (irt-worker.js:443-456): the partner, the one born before
-// t_base (the product itself was born at t_base), takes a Brownian step over
-// t_base - tbirth with the pair's draws 0-5, then r0 = |product - partner|; -1 beyond R_CUT.
-__device__ __forceinline__ double repair_r0(const double *tb, float4 pa, float4 pb, uint32_t a, uint32_t b,
-                                            uint64_t ps, double t_base)
-{
-    const IrtConst &C = c_irt;
-    const bool a_old = tb[a] < t_base;
-    const float4 pi = a_old ? pb : pa, pj = a_old ? pa : pb;
-    const double dt = t_base - tb[a_old ? a : b];
-    double xj = pj.x, yj = pj.y, zj = pj.z;
-    if (dt > 0) {
-        double sig = sqrt(2.0 * C.D[(int)pj.w] * dt);
-        double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
-        double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
-        double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
-        xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
-    }
-    double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
-    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-    return r2 > C.R_CUT2 ? -1.0 : sqrt(r2);
-}
-
 // A queued pair with its r0 (r0 < 0: a re-pairing beyond R_CUT, no event).
-__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, uint64_t tk, uint32_t item,
-                                                 double t_base)
+__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh,
+                                                 uint64_t tk, uint32_t item)
 {
-    const IrtConst &C = c_irt;
-    const float4 *P = A.pos + T.part_off;
     uint4 e = A.pend[T.ev_off + item];
     PendPair q;
-    q.a = e.x; q.b = e.y;
+    q.a = e.x & 0x0FFFFFFFu; q.b = e.y & 0x0FFFFFFFu;
     q.r0 = __hiloint2double((int)e.w, (int)e.z);
-    const float4 pa = P[q.a], pb = P[q.b];
-    int si = (int)pa.w, sj = (int)pb.w;
-    q.ri = C.rxnMap[si * 6 + sj];
-    q.D = C.D[si] + C.D[sj];
+    const uint32_t si = e.x >> 28, sj = e.y >> 28;
+    q.ri = sh.rmap[si * 6 + sj];
+    q.D = sh.Dd[si] + sh.Dd[sj];
     q.ps = dna_pair_state(tk, q.a, q.b);
     return q;
 }
@@ -472,7 +460,7 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
 }
 
 #define LANE_TRIALS 64u
-#define LONG_MAX 512u
+#define LONG_ITEMS 512u
 
 // per-lane schedule; a pair still running after LANE_TRIALS trials is handed to the
 // wavefront schedule (sh.long_item / sh.long_n) and resumed there at that trial
@@ -489,7 +477,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         bool done = false;
         double X = -1;
         if (need_setup) {
-            pp = load_pending(A, T, tk, item, t_base);
+            pp = load_pending(A, T, sh, tk, item);
             double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
             if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; need_setup = false; }
             else {
@@ -505,7 +493,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
                 done = true;
             } else if (trial == LANE_TRIALS) {
                 uint32_t l = atomicAdd(&sh.long_n, 1u);
-                if (l < LONG_MAX) { sh.long_item[l] = item; done = true; }
+                if (l < LONG_ITEMS) { sh.long_item[l] = item; done = true; }
             } else if (trial == 10000u) done = true;
         }
         if (done) {
@@ -521,14 +509,14 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
                                 uint32_t npend, double t_base, bool use_long)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t nlong = sh.long_n < LONG_MAX ? sh.long_n : LONG_MAX;
+    const uint32_t nlong = sh.long_n < LONG_ITEMS ? sh.long_n : LONG_ITEMS;
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(use_long ? &sh.long_next : &sh.pend_next, 1u);
         slot = __shfl(slot, 0, 64);
         if (slot >= (use_long ? nlong : npend)) break;
         uint32_t item = use_long ? sh.long_item[slot] : slot;
-        PendPair pp = load_pending(A, T, tk, item, t_base);
+        PendPair pp = load_pending(A, T, sh, tk, item);
         uint32_t start = 0;
         if (!use_long) {
             double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
@@ -552,36 +540,6 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
-// Queued re-pairings (r0 = -1) get their r0 and the exact first sampling stage in one
-// dense pass; the pairs that can react are compacted to the front of the queue (through
-// the space behind it), so the drain sees only those.  All threads, after a barrier.
-__device__ void repair_prepare(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
-{
-    const IrtConst &C = c_irt;
-    const float4 *P = A.pos + T.part_off;
-    const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
-    if (threadIdx.x == 0) sh.nkeep = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < npend; i += IRT_NT) {
-        uint4 e = A.pend[T.ev_off + i];
-        const float4 pa = P[e.x], pb = P[e.y];
-        const uint64_t ps = dna_pair_state(tk, e.x, e.y);
-        double r0 = repair_r0(A.tb + T.part_off, pa, pb, e.x, e.y, ps, t_base);
-        if (r0 < 0) continue;
-        const int si = (int)pa.w, sj = (int)pb.w;
-        if (!may_react_k(rx_constants(C.rxnMap[si * 6 + sj], C.D[si] + C.D[sj]), r0, ps)) continue;
-        uint32_t k = atomicAdd(&sh.nkeep, 1u);
-        if (npend + k < T.ev_cap)
-            A.pend[T.ev_off + npend + k] = make_uint4(e.x, e.y, (uint32_t)__double2loint(r0), (uint32_t)__double2hiint(r0));
-        else atomicOr(A.overflow, 2u);
-    }
-    __syncthreads();
-    const uint32_t nk = (npend + sh.nkeep <= T.ev_cap) ? sh.nkeep : 0u;
-    for (uint32_t k = threadIdx.x; k < nk; k += IRT_NT) A.pend[T.ev_off + k] = A.pend[T.ev_off + npend + k];
-    __syncthreads();
-    if (threadIdx.x == 0) sh.npend = nk;
-}
-
 // drain the whole queue; all threads call it after a barrier
 __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
 {
@@ -595,64 +553,131 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, Trac
     }
 }
 
+// The exact part for one staged re-pairing (product p, partner jj): the partner's
+// Brownian step, r0, the first sampling stage, and the queueing of pairs that can react.
+__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al,
+                                             uint64_t tk, double t_new, uint32_t e)
+{
+    const float4 *P = A.pos + T.part_off;
+    const double *tb = A.tb + T.part_off;
+    const uint32_t p = e >> 31, jj = e & 0x7FFFFFFFu;
+    const uint32_t sj = (uint32_t)al[jj] - 1u, sp = sh.prod_sp[p], pc = sp * 6u + sj;
+    const int rj = sh.rmap[pc];
+    const uint32_t idx = sh.prod_idx[p];
+    const float4 pi = sh.prod_pos[p], pj = P[jj];
+    const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
+    const uint64_t ps = dna_pair_state(tk, a, b);
+    const double dt = t_new - tb[jj];
+    double xj = pj.x, yj = pj.y, zj = pj.z;
+    if (dt > 0) {
+        double sig = sqrt(2.0 * sh.Dd[sj] * dt);
+        double g0 = sqrt(-2 * dna_log(dna_u01(ps, 0) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 1));
+        double g1 = sqrt(-2 * dna_log(dna_u01(ps, 2) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 3));
+        double g2 = sqrt(-2 * dna_log(dna_u01(ps, 4) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 5));
+        xj += g0 * sig; yj += g1 * sig; zj += g2 * sig;
+    }
+    double ddx = (double)pi.x - xj, ddy = (double)pi.y - yj, ddz = (double)pi.z - zj;
+    double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+    if (r2 > c_irt.R_CUT2) return;
+    const double r0 = sqrt(r2);
+    if (!may_react_k(rx_constants(rj, sh.Dd[sp] + sh.Dd[sj]), r0, ps)) return;
+    push_pending(A, T, sh, a, b, a == idx ? sp : sj, a == idx ? sj : sp, r0);
+}
+
 // Re-pair the products of the reaction at t_new with every alive particle that has a
-// reaction channel with them (irt-worker.js:430-465).  Alive particles are kept in
-// per-species lists; the (product, partner species) segments of one reaction are
-// flattened into one index range (table built by the reacting thread).  Product 0 is not
-// paired with product 1: it does not exist yet at that point of the worker's loop
-// (irt-worker.js:618-644).  Each pair first meets the necessary condition of
-// cannot_react on a lower bound of its distance after the partner's Brownian step; the
-// survivors are queued with r0 = -1 and get the step, r0 and the sampling in the drain.
-__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, uint64_t tk, double t_new,
-                              unsigned long long &wrep)
+// reaction channel with them (irt-worker.js:430-465).  The (product, partner species)
+// segments of one reaction are flattened into one index range over the species lists
+// (table built by the reacting thread); list entries that have reacted are skipped.
+// Product 0 is not paired with product 1: it does not exist yet at that point of the
+// worker's loop (irt-worker.js:618-644).  Each pair first meets the necessary condition
+// of cannot_react on a lower bound of its distance after the partner's Brownian step;
+// the survivors are staged per wavefront and finished 64 at a time (the FP64 step,
+// r0 = |product - partner| (irt-worker.js:443-456), the first sampling stage), so that
+// part runs on full wavefronts; pairs that can react are queued for the drain.
+__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al,
+                              uint64_t tk, double t_new, unsigned long long &wrep)
 {
     const float4 *P = A.pos + T.part_off;
     const double *tb = A.tb + T.part_off;
     const uint32_t *lst = A.lst + 6 * T.part_off;
     const int ns = sh.nseg, np = sh.np;
     const uint32_t tot = sh.seg_u0[ns];
-    for (uint32_t u = threadIdx.x; u < tot; u += IRT_NT) {
-        int sg = 0;
-        while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
-        const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
-        const uint32_t idx = sh.prod_idx[p];
-        const uint32_t jj = lst[(size_t)sj * T.cap + (u - sh.seg_u0[sg])];
-        if (jj == idx || (p == 0 && np == 2 && jj == sh.prod_idx[1])) continue;
-        wrep++;
-        const float4 pi = sh.prod_pos[p], pj = P[jj];
-        const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
-        const uint64_t ps = dna_pair_state(tk, a, b);
-        const double dt = t_new - tb[jj];
-        // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so the
-        // step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
-        float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-        float rmin = sqrtf(fx * fx + fy * fy + fz * fz) * 0.99999f;
-        if (dt > 0) {
-            float m2 = -2.f * (logf((float)(dna_u01(ps, 0) + 1e-30)) + logf((float)(dna_u01(ps, 2) + 1e-30)) +
-                               logf((float)(dna_u01(ps, 4) + 1e-30)));
-            float sgm = sqrtf((float)(2.0 * (double)sh.seg_Dj[sg] * dt));
-            rmin -= sgm * sqrtf(m2 * 1.0001f + 1e-4f) * 1.0001f;
+    const uint32_t pidx1 = sh.prod_idx[1];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t below = (1ull << lane) - 1ull;
+    volatile uint32_t *q = sh.rq[w];
+    uint32_t qn = 0;                                   // wavefront-uniform
+    for (uint32_t u0 = w * 64u; u0 < tot; u0 += IRT_NT) {
+        const uint32_t u = u0 + lane;
+        bool keep = false;
+        uint32_t ent = 0;
+        if (u < tot) {
+            int sg = 0;
+            while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
+            const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
+            const uint32_t jj = lst[(size_t)sj * T.cap + (u - sh.seg_u0[sg])];
+            const uint32_t idx = sh.prod_idx[p];
+            if (al[jj] && jj != idx && !(p == 0 && np == 2 && jj == pidx1)) {
+                wrep++;
+                const uint32_t pc = sh.prod_sp[p] * 6u + sj;
+                const float4 pi = sh.prod_pos[p], pj = P[jj];
+                const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
+                const uint64_t ps = dna_pair_state(tk, a, b);
+                const double dt = t_new - tb[jj];
+                // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so
+                // the step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
+                float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
+                float rmin = sqrtf(fx * fx + fy * fy + fz * fz) * 0.99999f;
+                if (dt > 0) {
+                    float m2 = -2.f * (logf((float)(dna_u01(ps, 0) + 1e-30)) + logf((float)(dna_u01(ps, 2) + 1e-30)) +
+                                       logf((float)(dna_u01(ps, 4) + 1e-30)));
+                    float sgm = sqrtf((float)(2.0 * (double)sh.Df[sj] * dt));
+                    rmin -= sgm * sqrtf(m2 * 1.0001f + 1e-4f) * 1.0001f;
+                }
+                keep = !(rmin > c_irt.R_CUT_F) && !cannot_react(sh.bnd[pc], rmin, dna_u01(ps, 6u));
+                ent = (p << 31) | jj;
+            }
         }
-        if (rmin > c_irt.R_CUT_F) continue;
-        RxB kb; kb.S = sh.seg_S[sg]; kb.sc = sh.seg_sc[sg]; kb.R = sh.seg_R[sg];
-        if (cannot_react(kb, rmin, dna_u01(ps, 6u))) continue;
-        push_pending(A, T, sh, a, b, -1.0);
+        const uint64_t m = __ballot(keep);
+        if (keep) q[qn + (uint32_t)__popcll(m & below)] = ent;
+        qn += (uint32_t)__popcll(m);
+        __builtin_amdgcn_wave_barrier();
+        if (qn >= 64u) {
+            repair_exact(A, T, sh, al, tk, t_new, q[lane]);
+            __builtin_amdgcn_wave_barrier();
+            if (lane < qn - 64u) q[lane] = q[64u + lane];
+            qn -= 64u;
+            __builtin_amdgcn_wave_barrier();
+        }
     }
+    if (lane < qn) repair_exact(A, T, sh, al, tk, t_new, q[lane]);
 }
 
-// species-list maintenance (one lane): swap-remove and append
-__device__ __forceinline__ void list_remove(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, int s, uint32_t k)
+// Drop reacted entries from the species lists flagged in sh.lcomp (all threads, after a
+// barrier; ends with one).  The order inside a list does not matter: every chunk of 256
+// entries is read before any of its slots is rewritten, and an entry is only ever moved
+// to a lower position.
+__device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al)
 {
-    uint32_t *L = A.lst + 6 * T.part_off + (size_t)s * T.cap;
-    uint32_t *slot = A.slot + T.part_off;
-    uint32_t pos = slot[k], last = L[--sh.llen[s]];
-    L[pos] = last; slot[last] = pos;
-}
-__device__ __forceinline__ void list_add(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, int s, uint32_t k)
-{
-    uint32_t *L = A.lst + 6 * T.part_off + (size_t)s * T.cap;
-    uint32_t pos = sh.llen[s]++;
-    L[pos] = k; A.slot[T.part_off + k] = pos;
+    uint32_t *lst = A.lst + 6 * T.part_off;
+    const uint32_t mask = sh.lcomp;
+    for (int s = 0; s < 6; s++) {
+        if (!(mask & (1u << s))) continue;
+        uint32_t *L = lst + (size_t)s * T.cap;
+        const uint32_t len = sh.llen[s];
+        if (threadIdx.x == 0) sh.lcnt = 0;
+        __syncthreads();
+        for (uint32_t c0 = 0; c0 < len; c0 += IRT_NT) {
+            const uint32_t i = c0 + threadIdx.x;
+            const uint32_t k = i < len ? L[i] : 0u;
+            const bool live = i < len && al[k];
+            __syncthreads();
+            if (live) L[atomicAdd(&sh.lcnt, 1u)] = k;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) { sh.llen[s] = sh.lcnt; sh.ldead[s] = 0; }
+        __syncthreads();
+    }
 }
 
 __device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh, int c,
@@ -700,6 +725,14 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     }
     if (tid == 0) { sh.n_ev = 0; sh.ev_lds = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
     for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_NT) s_loc[i] = 0;
+    if (tid < 36) {
+        const int r = C.rxnMap[tid];
+        sh.rmap[tid] = (int8_t)r;
+        RxB bz; bz.S = 0.f; bz.sc = 0.f; bz.R = 0.f;
+        sh.bnd[tid] = r >= 0 ? rx_bound(rx_constants(r, C.D[tid / 6] + C.D[tid % 6])) : bz;
+    }
+    if (tid < 9) { sh.nprod[tid] = (int8_t)C.nprod[tid]; sh.prods[tid][0] = (int8_t)C.prods[tid][0]; sh.prods[tid][1] = (int8_t)C.prods[tid][1]; }
+    if (tid < 6) { sh.Dd[tid] = C.D[tid]; sh.sqD[tid] = C.sqrtD[tid]; sh.Df[tid] = (float)C.D[tid]; }
     __syncthreads();
 
     // phase timers (100 MHz wall clock, lane 0): load, pairs, scan, react, repair, iterations
@@ -730,7 +763,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
             z += sig * ((v - 3) * 1.4142135623730951);
         }
         P[k] = make_float4((float)x, (float)y, (float)z, (float)spc);
-        al[k] = alv;
+        al[k] = alv ? (uint8_t)(spc + 1) : (uint8_t)0;
         tb[k] = 0.0;
         if (alv) { if (spc == 0) c0++; else if (spc == 1) c1++; else if (spc == 2) c2++; }
     }
@@ -831,7 +864,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
                 uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
                 const uint64_t ps = dna_pair_state(tk, a, b);
                 if (cannot_react(kb, sqrtf((float)r2) * 0.99999f, dna_u01(ps, 6u))) continue;
-                push_pending(A, T, sh, a, b, sqrt(r2));
+                push_pending(A, T, sh, a, b, oi < oj ? sa : sb, oi < oj ? sb : sa, sqrt(r2));
             }
         }
     }
@@ -839,14 +872,10 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     for (uint32_t g = tid; g < n; g += IRT_NT) {
         int spc = 0;
         while (spc < 5 && g >= sh.goff[spc + 1]) spc++;
-        if (g < sh.goff[spc] + sh.gcnt[spc]) {
-            uint32_t pos = g - sh.goff[spc];
-            uint32_t k = (uint32_t)__float_as_int(Sg[g].w);
-            A.lst[6 * T.part_off + (size_t)spc * T.cap + pos] = k;
-            A.slot[T.part_off + k] = pos;
-        }
+        if (g < sh.goff[spc] + sh.gcnt[spc])
+            A.lst[6 * T.part_off + (size_t)spc * T.cap + (g - sh.goff[spc])] = (uint32_t)__float_as_int(Sg[g].w);
     }
-    if (tid < 6) sh.llen[tid] = sh.gcnt[tid];
+    if (tid < 6) { sh.llen[tid] = sh.gcnt[tid]; sh.ldead[tid] = 0; }
     __syncthreads();
     unsigned long long tpl = wall_clock64();
     pdc_drain(A, T, tk, sh, 0.0);
@@ -918,22 +947,22 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
             double t0 = sh.wt[0]; uint64_t ab0 = sh.wab[0];
             for (int w = 1; w < (int)(IRT_NT / 64u); w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
             sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
+            sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
             if (t0 < 1e300) {
                 while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
                 uint32_t a = (uint32_t)(ab0 >> 32), b = (uint32_t)ab0;
-                int si = (int)P[a].w, sj = (int)P[b].w;
-                int ri = C.rxnMap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
+                int si = (int)al[a] - 1, sj = (int)al[b] - 1;
+                int ri = sh.rmap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
                 al[a] = 0; al[b] = 0;
-                list_remove(A, T, sh, si, a);
-                list_remove(A, T, sh, sj, b);
+                sh.ldead[si]++; sh.ldead[sj]++;
                 sh.n_re++; sh.rxc[ri]++;
                 if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
                 if (sj == 0) sh.cnt[0]--; else if (sj == 1) sh.cnt[1]--; else if (sj == 2) sh.cnt[2]--; else if (sj == 4) sh.cnt[3]--;
                 if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
-                int np = C.nprod[ri];
+                int np = sh.nprod[ri];
                 if (np > 0) {
                     float4 pa = P[a], pb = P[b];
-                    double sqDi = C.sqrtD[si], sqDj = C.sqrtD[sj];
+                    double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
                     double wsum = sqDi + sqDj;
                     double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
                     double my = (sqDj * pa.y + sqDi * pb.y) / wsum;
@@ -944,54 +973,55 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
                         uint32_t id = nt;
                         if (id >= T.cap_global) break;
                         if (id >= T.cap) { atomicOr(A.overflow, 1u); break; }
-                        int ps = C.prods[ri][pp];
+                        int ps = sh.prods[ri][pp];
                         float4 q;
                         if (np == 1) q = make_float4((float)mx, (float)my, (float)mz, (float)ps);
                         else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
                         else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
-                        P[id] = q; al[id] = 1; tb[id] = t0;
-                        list_add(A, T, sh, ps, id);
+                        P[id] = q; al[id] = (uint8_t)(ps + 1); tb[id] = t0;
+                        A.lst[6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++] = id;
                         if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
-                        sh.prod_idx[made++] = id;
+                        sh.prod_idx[made] = id; sh.prod_sp[made] = (uint32_t)ps; sh.prod_pos[made] = q;
+                        made++;
                         nt++;
                     }
                     sh.np = made;
                     sh.n_total = nt;
                 }
                 // segments of the re-pairing: (product, partner species with a channel)
-                uint32_t u = 0;
+                uint32_t u = 0, lc = 0;
                 int ns = 0;
                 for (int pp = 0; pp < sh.np; pp++) {
-                    const uint32_t id = sh.prod_idx[pp];
-                    const int sp = (int)P[id].w;
-                    sh.prod_pos[pp] = P[id];
+                    const uint32_t sp = sh.prod_sp[pp];
                     for (int sj2 = 0; sj2 < 6; sj2++) {
-                        const int rj = C.rxnMap[sp * 6 + sj2];
-                        if (rj < 0 || sh.llen[sj2] == 0) continue;
-                        const RxB bnd = rx_bound(rx_constants(rj, C.D[sp] + C.D[sj2]));
+                        if (sh.rmap[sp * 6 + sj2] < 0 || sh.llen[sj2] == 0) continue;
                         sh.seg_u0[ns] = u; sh.seg_ps[ns] = ((uint32_t)pp << 4) | (uint32_t)sj2;
-                        sh.seg_S[ns] = bnd.S; sh.seg_sc[ns] = bnd.sc; sh.seg_R[ns] = bnd.R;
-                        sh.seg_Dj[ns] = (float)C.D[sj2];
                         u += sh.llen[sj2];
                         ns++;
+                        if (2u * sh.ldead[sj2] > sh.llen[sj2] && sh.llen[sj2] >= 64u) lc |= 1u << sj2;
                     }
                 }
-                sh.seg_u0[ns] = u; sh.nseg = ns;
+                sh.seg_u0[ns] = u; sh.nseg = ns; sh.lcomp = lc;
             }
         }
         __syncthreads();
         unsigned long long q2 = wall_clock64();
         trc += q2 - q1;
         if (!(sh.win_t < 1e300)) break;
-        if (sh.np) pair_products(A, T, sh, tk, sh.win_t, wrep);
+        if (sh.lcomp) {
+            compact_lists(A, T, sh, al);
+            if (tid == 0) {                   // segment starts after the compaction
+                uint32_t u = 0;
+                for (int g = 0; g < sh.nseg; g++) { sh.seg_u0[g] = u; u += sh.llen[sh.seg_ps[g] & 15u]; }
+                sh.seg_u0[sh.nseg] = u; sh.lcomp = 0;
+            }
+            __syncthreads();
+        }
+        if (sh.np) pair_products(A, T, sh, al, tk, sh.win_t, wrep);
         __syncthreads();
         unsigned long long q3 = wall_clock64();
         if (sh.npend) {
-            repair_prepare(A, T, tk, sh, sh.win_t);
-            __syncthreads();
-            if (sh.npend) pdc_drain(A, T, tk, sh, sh.win_t);
-            __syncthreads();
-            if (tid == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
+            pdc_drain(A, T, tk, sh, sh.win_t);
             __syncthreads();
         }
         unsigned long long q4 = wall_clock64();
@@ -1201,6 +1231,10 @@ static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     G.contiguous = ntr == 0 || h_misc[1] == ntr - 1;
     G.maxn_all = 0;
     for (auto &t : G.tr) G.maxn_all = t.n > G.maxn_all ? t.n : G.maxn_all;
+    if (G.maxn_all >= (1u << 27) - 512u) {       // particle indices carry the species in bits 28-31
+        set_error("dna_run_irt: a primary has 2^27 or more entries");
+        return DNA_ERR_CAPACITY;
+    }
     if (!G.contiguous && G.maxn_all > SORT_MAX) {
         set_error("dna_run_irt: radicals of a primary are interleaved with others and the primary has more than 8192 entries; group the input by primary");
         return DNA_ERR_CAPACITY;
@@ -1234,7 +1268,7 @@ struct IrtBufs {
     size_t tally_bytes = 0;
 };
 
-static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 7 + 1) + 256, rad_n / 1024 * 4 + 64); }
+static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 6 + 1) + 256, rad_n / 1024 * 4 + 64); }
 static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
 {
     return tally_bytes + 256 + G.tr.size() * sizeof(IrtTrack) + (G.K + 1) * 4 + (size_t)G.run * 4 + tt_bytes + 1536;
@@ -1275,7 +1309,7 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     char *sp = b.scratch;
     A.rad = d_rad; A.idx = b.idx; A.tracks = b.tr;
     A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
-    A.lst = (uint32_t *)(sp + part * 40); A.slot = (uint32_t *)(sp + part * 64); A.alive = (uint8_t *)(sp + part * 68);
+    A.lst = (uint32_t *)(sp + part * 40); A.alive = (uint8_t *)(sp + part * 64);
     A.ev_t = (double *)b.events; A.ev_ab = (uint2 *)(b.events + evs * 8);
     A.pend = (uint4 *)(b.events + evs * 16);
     A.tally = b.tally; A.track_tally = b.tt; A.overflow = b.ovf;
