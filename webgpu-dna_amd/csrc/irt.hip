@@ -185,14 +185,25 @@ __device__ __forceinline__ RxB rx_bound(const RxK &k)
     return b;
 }
 
-// rmin: a lower bound of the pair distance; U: the pair's draw 6 (0 = keep: a PDC pair with
-// U = 0 passes any Winf > 0)
-__device__ __forceinline__ bool cannot_react(const RxB &b, float rmin, double U)
+// rmin: a lower bound of the pair distance; U: a lower bound of the pair's draw 6 (0 = keep:
+// a PDC pair with U = 0 passes any Winf > 0)
+__device__ __forceinline__ bool cannot_react(const RxB &b, float rmin, float U)
 {
     float a = rmin - b.R;
     if (!(a > b.sc)) return false;
-    return a * (float)U * 0.99999f > b.S;
+    return a * U * 0.99999f > b.S;
 }
+
+// Lower bound of draw n, dna_u01(ps, n) = (x >> 11) 2^-53: its top 24 bits, exact in FP32.
+__device__ __forceinline__ float u01_lo(uint64_t ps, uint32_t n)
+{
+    const uint64_t x = dna_mix64(ps + DNA_GOLDEN * (uint64_t)(n + 1u));
+    return (float)(uint32_t)(x >> 40) * 5.9604644775390625e-8f;
+}
+
+// Square root with the hardware instruction (1 ulp); only used inside bounds whose margins
+// are far above that error.
+__device__ __forceinline__ float sqrt_bound(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 // SamplePDC parameters a, b (irt-worker.js:145-160)
 __device__ void pdc_ab(double r0, int ri, double D, double &a, double &b)
@@ -627,14 +638,15 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
                 // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so
                 // the step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-                float rmin = sqrtf(fx * fx + fy * fy + fz * fz) * 0.99999f;
+                // (u01_lo <= the draw, so -ln of it is an upper bound; u01_lo = 0 gives an
+                // infinite step and keeps the pair)
+                float rmin = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
                 if (dt > 0) {
-                    float m2 = -2.f * (logf((float)(dna_u01(ps, 0) + 1e-30)) + logf((float)(dna_u01(ps, 2) + 1e-30)) +
-                                       logf((float)(dna_u01(ps, 4) + 1e-30)));
-                    float sgm = sqrtf((float)(2.0 * (double)sh.Df[sj] * dt));
-                    rmin -= sgm * sqrtf(m2 * 1.0001f + 1e-4f) * 1.0001f;
+                    float m2 = -2.f * (logf(u01_lo(ps, 0)) + logf(u01_lo(ps, 2)) + logf(u01_lo(ps, 4)));
+                    float sgm = sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt));
+                    rmin -= sgm * sqrt_bound(m2 * 1.0001f + 1e-4f) * 1.0001f;
                 }
-                keep = !(rmin > c_irt.R_CUT_F) && !cannot_react(sh.bnd[pc], rmin, dna_u01(ps, 6u));
+                keep = !(rmin > c_irt.R_CUT_F) && !cannot_react(sh.bnd[pc], rmin, u01_lo(ps, 6u));
                 ent = (p << 31) | jj;
             }
         }
@@ -684,6 +696,73 @@ __device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared 
                          unsigned long long *loc)
 {
     for (int s = 0; s < 5; s++) loc[c * 5 + s] += sh.cnt[s];
+}
+
+// Initial pairs of the channel blocks (see k_irt step 2), with the species-sorted copy in
+// LDS (SG_LDS: DS loads) or in global memory.  A pair is dropped without FP64 work when an
+// FP32 bound decides it: the FP32 squared distance is within 3.1e-7 (relative) of the FP64
+// one (five roundings of non-negative terms), so r2 above R_CUT^2 (1 + 1e-5) or below
+// R_CUT^2 (1 - 1e-5) in FP32 settles the cutoff test, and the bound of cannot_react takes
+// sqrt(r2) (1 - 1e-5) and the draw's 24-bit lower bound.  The exact FP64 distance is
+// computed near the cutoff and for the pairs that are queued.
+template <bool SG_LDS>
+__device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, uint64_t tk,
+                                            unsigned long long &wcand, unsigned long long &wsamp)
+{
+    const float4 *Sg = SG_LDS ? (const float4 *)dyn_lds : A.sorted + T.part_off;
+    // block loop outermost: the reaction and its constants are wavefront-uniform
+    // (readfirstlane -> scalar registers), so the per-pair path has no table loads
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
+    const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+    const double RC2 = c_irt.R_CUT2;
+    const float RC2_hi = (float)(RC2 * 1.00001), RC2_lo = (float)(RC2 * 0.99999);
+    for (int q = 0; q < nb; q++) {
+        const int ri = __builtin_amdgcn_readfirstlane(sh.bri[q]);
+        const int sa = BA[ri], sb = BB[ri];
+        const uint32_t cpr = __builtin_amdgcn_readfirstlane(sh.bcpr[q]);
+        const uint32_t nA = __builtin_amdgcn_readfirstlane(sh.gcnt[sa]);
+        const uint32_t nB = __builtin_amdgcn_readfirstlane(sh.gcnt[sb]);
+        const uint32_t oA = __builtin_amdgcn_readfirstlane(sh.goff[sa]);
+        const uint32_t oB = __builtin_amdgcn_readfirstlane(sh.goff[sb]);
+        const uint32_t units = nA * cpr;
+        const RxB kb = rx_bound(rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]));
+        const uint32_t h = (nA - 1) / 2;
+        for (uint32_t ul = wave; ul < units; ul += IRT_NT / 64u) {
+            const uint32_t i = ul / cpr, c = ul - i * cpr;
+            const float4 pi = Sg[oA + i];
+            uint32_t j;
+            bool valid;
+            if (sa == sb) {
+                uint32_t dmax = h + (((nA & 1u) == 0 && i < nA / 2) ? 1u : 0u);
+                uint32_t d = c * 64u + lane + 1u;
+                valid = d <= dmax;
+                j = i + d; if (j >= nA) j -= nA;
+                j += oA;
+            } else {
+                uint32_t jj = c * 64u + lane;
+                valid = jj < nB;
+                j = oB + jj;
+            }
+            if (!valid) continue;
+            wcand++;
+            const float4 pj = Sg[j];
+            const float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
+            const float r2f = fx * fx + fy * fy + fz * fz;
+            if (r2f > RC2_hi) continue;
+            if (!(r2f < RC2_lo)) {
+                double ddx = (double)pi.x - (double)pj.x, ddy = (double)pi.y - (double)pj.y, ddz = (double)pi.z - (double)pj.z;
+                if (ddx * ddx + ddy * ddy + ddz * ddz > RC2) continue;
+            }
+            wsamp++;
+            const uint32_t oi = (uint32_t)__float_as_int(pi.w), oj = (uint32_t)__float_as_int(pj.w);
+            const uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
+            const uint64_t ps = dna_pair_state(tk, a, b);
+            if (cannot_react(kb, sqrt_bound(r2f) * 0.99999f, u01_lo(ps, 6u))) continue;
+            double ddx = (double)pi.x - (double)pj.x, ddy = (double)pi.y - (double)pj.y, ddz = (double)pi.z - (double)pj.z;
+            push_pending(A, T, sh, a, b, oi < oj ? sa : sb, oi < oj ? sb : sa, sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
+        }
+    }
 }
 
 template <bool SORT>
@@ -816,58 +895,8 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         }
     }
     __syncthreads();
-    {
-        // block loop outermost: the reaction and its constants are wavefront-uniform
-        // (readfirstlane -> scalar registers), so the per-pair path has no table loads
-        const uint32_t lane = tid & 63u, wave = tid >> 6;
-        const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
-        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
-        const double RC2 = c_irt.R_CUT2;
-        for (int q = 0; q < nb; q++) {
-            const int ri = __builtin_amdgcn_readfirstlane(sh.bri[q]);
-            const int sa = BA[ri], sb = BB[ri];
-            const uint32_t cpr = __builtin_amdgcn_readfirstlane(sh.bcpr[q]);
-            const uint32_t nA = __builtin_amdgcn_readfirstlane(sh.gcnt[sa]);
-            const uint32_t nB = __builtin_amdgcn_readfirstlane(sh.gcnt[sb]);
-            const uint32_t oA = __builtin_amdgcn_readfirstlane(sh.goff[sa]);
-            const uint32_t oB = __builtin_amdgcn_readfirstlane(sh.goff[sb]);
-            const uint32_t units = nA * cpr;
-            RxK k = rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]);
-            const RxB kb = rx_bound(k);
-            const uint32_t h = (nA - 1) / 2;
-            for (uint32_t ul = wave; ul < units; ul += IRT_NT / 64u) {
-                const uint32_t i = ul / cpr, c = ul - i * cpr;
-                const float4 pi = Sg[oA + i];
-                uint32_t j;
-                bool valid;
-                if (sa == sb) {
-                    uint32_t dmax = h + (((nA & 1u) == 0 && i < nA / 2) ? 1u : 0u);
-                    uint32_t d = c * 64u + lane + 1u;
-                    valid = d <= dmax;
-                    j = i + d; if (j >= nA) j -= nA;
-                    j += oA;
-                } else {
-                    uint32_t jj = c * 64u + lane;
-                    valid = jj < nB;
-                    j = oB + jj;
-                }
-                if (!valid) continue;
-                wcand++;
-                const float4 pj = Sg[j];
-                double ddx = (double)pi.x - (double)pj.x;
-                double ddy = (double)pi.y - (double)pj.y;
-                double ddz = (double)pi.z - (double)pj.z;
-                double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (r2 > RC2) continue;
-                wsamp++;
-                uint32_t oi = (uint32_t)__float_as_int(pi.w), oj = (uint32_t)__float_as_int(pj.w);
-                uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
-                const uint64_t ps = dna_pair_state(tk, a, b);
-                if (cannot_react(kb, sqrtf((float)r2) * 0.99999f, dna_u01(ps, 6u))) continue;
-                push_pending(A, T, sh, a, b, oi < oj ? sa : sb, oi < oj ? sb : sa, sqrt(r2));
-            }
-        }
-    }
+    if (n <= A.lds_slots) pair_blocks<true>(A, T, sh, tk, wcand, wsamp);
+    else pair_blocks<false>(A, T, sh, tk, wcand, wsamp);
     // species lists of the alive initial particles = the species groups of Sg
     for (uint32_t g = tid; g < n; g += IRT_NT) {
         int spc = 0;
