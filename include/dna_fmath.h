@@ -167,6 +167,20 @@ DNA_FN double dna_exp(double x)
     return dna_scalbn(y, (int)kd);
 }
 
+/* Series coefficients 2/(2k+1), k = 11..1, of dna_log.  In device code they are read from
+   constant memory with scalar loads at the point of use: as literals, the compiler hoists
+   them out of loops into vector registers and, under register pressure, spills them to
+   scratch memory, so every call inside a loop reloaded them from memory.  Same values on
+   both sides, so the results do not change. */
+#if defined(__HIP_DEVICE_COMPILE__)
+static __constant__ double dna_log_series[11] = {
+#else
+static const double dna_log_series[11] = {
+#endif
+    0.08695652173913043478, 0.09523809523809523810, 0.10526315789473684211, 0.11764705882352941176,
+    0.13333333333333333333, 0.15384615384615384615, 0.18181818181818181818, 0.22222222222222222222,
+    0.28571428571428571429, 0.40000000000000000000, 0.66666666666666666667};
+
 DNA_FN double dna_log(double x)
 {
     const double ln2_hi = 6.93147180369123816490e-01;
@@ -185,16 +199,17 @@ DNA_FN double dna_log(double x)
     double s = f / (2.0 + f);
     double z = s * s;
     /* R = sum_{k=1..11} 2/(2k+1) z^k */
-    double R = fma(z, 0.08695652173913043478, 0.09523809523809523810);  /* 2/23, 2/21 */
-    R = fma(z, R, 0.10526315789473684211);   /* 2/19 */
-    R = fma(z, R, 0.11764705882352941176);   /* 2/17 */
-    R = fma(z, R, 0.13333333333333333333);   /* 2/15 */
-    R = fma(z, R, 0.15384615384615384615);   /* 2/13 */
-    R = fma(z, R, 0.18181818181818181818);   /* 2/11 */
-    R = fma(z, R, 0.22222222222222222222);   /* 2/9  */
-    R = fma(z, R, 0.28571428571428571429);   /* 2/7  */
-    R = fma(z, R, 0.40000000000000000000);   /* 2/5  */
-    R = fma(z, R, 0.66666666666666666667);   /* 2/3  */
+    const double *c = dna_log_series;
+    double R = fma(z, c[0], c[1]);           /* 2/23, 2/21 */
+    R = fma(z, R, c[2]);                     /* 2/19 */
+    R = fma(z, R, c[3]);                     /* 2/17 */
+    R = fma(z, R, c[4]);                     /* 2/15 */
+    R = fma(z, R, c[5]);                     /* 2/13 */
+    R = fma(z, R, c[6]);                     /* 2/11 */
+    R = fma(z, R, c[7]);                     /* 2/9  */
+    R = fma(z, R, c[8]);                     /* 2/7  */
+    R = fma(z, R, c[9]);                     /* 2/5  */
+    R = fma(z, R, c[10]);                    /* 2/3  */
     R = z * R;
     double hfsq = 0.5 * f * f;
     double ed = (double)e;
