@@ -45,6 +45,7 @@ struct IrtTrack {
     uint32_t cap;                     // particle capacity of this track (products included)
     uint32_t cap_global;              // the worker's CAP = 2*maxN + 512 of the instance
     uint32_t ev_cap;
+    uint32_t idx;                     // position in track order (per-track outputs)
     uint64_t part_off, ev_off;
 };
 
@@ -772,7 +773,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     __shared__ TrackShared sh;
     __shared__ uint32_t s_sort[SORT ? SORT_MAX : 1];
     __shared__ unsigned long long s_loc[DNA_MAX_CKPT * 5];
-    const IrtTrack T = A.tracks[blockIdx.x];
+    const IrtTrack T = A.tracks[blockIdx.x];          // launch order: largest tracks first
     const uint32_t tid = threadIdx.x;
     const uint32_t n = T.n;
     const uint32_t *idx = A.idx + T.start;
@@ -1076,7 +1077,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE;
         for (int i = 0; i < A.nc * 5; i++) if (s_loc[i]) atomicAdd(&g[i], s_loc[i]);
         if (A.track_tally)
-            for (int i = 0; i < A.nc * 5; i++) A.track_tally[(size_t)blockIdx.x * A.nc * 5 + i] = (uint32_t)s_loc[i];
+            for (int i = 0; i < A.nc * 5; i++) A.track_tally[(size_t)T.idx * A.nc * 5 + i] = (uint32_t)s_loc[i];
         for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
@@ -1178,6 +1179,7 @@ struct IrtGroup {
     uint32_t run = 0, maxn_all = 0, gb = 0;
     bool contiguous = true;
     std::vector<IrtTrack> tr;
+    std::vector<IrtTrack> launch;    // tr by decreasing particle count: the launch order
     std::vector<uint32_t> bstart, maxN;
     GroupScratch g;
 };
@@ -1285,6 +1287,10 @@ static void irt_size(IrtGroup &G, int attempt, int ev_mult, uint64_t &part, uint
         t.ev_cap = (uint32_t)std::min<uint64_t>((uint64_t)ev_mult * t.n + 1024, 0xFFFFFFF0ull);
         t.ev_off = evs; evs += t.ev_cap;
     }
+    // launch the largest tracks first, so that the last workgroups to start are short ones
+    for (size_t i = 0; i < G.tr.size(); i++) G.tr[i].idx = (uint32_t)i;
+    G.launch = G.tr;
+    std::stable_sort(G.launch.begin(), G.launch.end(), [](const IrtTrack &x, const IrtTrack &y) { return x.n > y.n; });
 }
 
 // Device buffers of one launch and their byte sizes.
@@ -1321,7 +1327,7 @@ static int irt_prepare(dna_ctx *c, const IrtGroup &G, const IrtBufs &b)
     hipStream_t st = c->stream;
     const uint32_t ntr = (uint32_t)G.tr.size();
     DNA_HIP(hipMemsetAsync(b.tally, 0, b.tally_bytes + 8, st));
-    DNA_HIP(hipMemcpyAsync(b.tr, G.tr.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(b.tr, G.launch.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemcpyAsync(b.bst, G.bstart.data(), (G.K + 1) * 4, hipMemcpyHostToDevice, st));
     const GroupScratch &g = G.g;
     hipLaunchKernelGGL(k_bucket, dim3(G.gb), dim3(256), 0, st, g.kept, g.S, g.key, G.rad_n, b.bst, g.firstj, g.cursor,
