@@ -516,9 +516,10 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
-// wavefront schedule over items [from the queue, or the long list when use_long]
+// wavefront schedule over items [from the queue, or the long list when use_long; those
+// resume at trial long_start]
 __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
-                                uint32_t npend, double t_base, bool use_long)
+                                uint32_t npend, double t_base, bool use_long, uint32_t long_start = LANE_TRIALS)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t nlong = sh.long_n < LONG_ITEMS ? sh.long_n : LONG_ITEMS;
@@ -536,7 +537,7 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
                 if (lane == 0 && t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
                 continue;
             }
-        } else start = LANE_TRIALS;
+        } else start = long_start;
         double a, b, p, q, M;
         pdc_setup(pp, a, b, p, q, M);
         double X = -1;
@@ -552,6 +553,24 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
+// small queues (fewer than 2 x workgroup size items, so at most LONG_ITEMS - 1): the first
+// stage of every pair on the first wavefront, a lane per pair; the pairs that need SamplePDC
+// trials are listed and run on the wavefront schedule from trial 0
+__device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+                                uint32_t npend, double t_base)
+{
+    if (threadIdx.x < 64u) {
+        for (uint32_t item = threadIdx.x; item < npend; item += 64u) {
+            const PendPair pp = load_pending(A, T, sh, tk, item);
+            const double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+            if (t1 == STAGE_PDC) sh.long_item[atomicAdd(&sh.long_n, 1u)] = item;
+            else if (t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
+        }
+    }
+    __syncthreads();
+    if (sh.long_n) pdc_drain_waves(A, T, tk, sh, npend, t_base, true, 0u);
+}
+
 // drain the whole queue; all threads call it after a barrier
 __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
 {
@@ -561,7 +580,7 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, Trac
         __syncthreads();
         pdc_drain_waves(A, T, tk, sh, npend, t_base, true);
     } else {
-        pdc_drain_waves(A, T, tk, sh, npend, t_base, false);
+        pdc_drain_small(A, T, tk, sh, npend, t_base);
     }
 }
 
@@ -639,11 +658,12 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
                 // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so
                 // the step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-                // (u01_lo <= the draw, so -ln of it is an upper bound; u01_lo = 0 gives an
+                // (u01_lo <= the draw, so -ln of it is an upper bound; the three logs are taken
+                // as one of the FP32 product, within 2.4e-7 of their sum; u01_lo = 0 gives an
                 // infinite step and keeps the pair)
                 float rmin = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
                 if (dt > 0) {
-                    float m2 = -2.f * (logf(u01_lo(ps, 0)) + logf(u01_lo(ps, 2)) + logf(u01_lo(ps, 4)));
+                    float m2 = -2.f * logf(u01_lo(ps, 0) * u01_lo(ps, 2) * u01_lo(ps, 4));
                     float sgm = sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt));
                     rmin -= sgm * sqrt_bound(m2 * 1.0001f + 1e-4f) * 1.0001f;
                 }
