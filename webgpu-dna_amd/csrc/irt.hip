@@ -374,6 +374,7 @@ struct TrackShared {
     int      nseg;
     uint32_t seg_u0[13], seg_ps[12];      // first flattened index; p << 4 | sj
     uint32_t rq[IRT_BLOCK / 64][128];     // per wavefront: re-pairings that passed the bound
+    uint32_t rqn[IRT_BLOCK / 64];         // and how many are left after the scan
 };
 
 extern __shared__ float4 dyn_lds[];
@@ -683,7 +684,18 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
             __builtin_amdgcn_wave_barrier();
         }
     }
-    if (lane < qn) repair_exact(A, T, sh, al, tk, t_new, q[lane]);
+    // the wavefronts' remainders (< 64 each) are finished together, 64 per wavefront, so a
+    // few survivors cost one pass instead of one per wavefront
+    if (lane == 0) sh.rqn[w] = qn;
+    __syncthreads();
+    const uint32_t nw = IRT_NT / 64u;
+    uint32_t tot_r = 0;
+    for (uint32_t k = 0; k < nw; k++) tot_r += sh.rqn[k];
+    for (uint32_t v = w * 64u + lane; v < tot_r; v += IRT_NT) {
+        uint32_t k = 0, off = v;
+        while (off >= sh.rqn[k]) { off -= sh.rqn[k]; k++; }
+        repair_exact(A, T, sh, al, tk, t_new, sh.rq[k][off]);
+    }
 }
 
 // Drop reacted entries from the species lists flagged in sh.lcomp (all threads, after a
