@@ -6,6 +6,9 @@ profiles/.
                                                    duration (ms), VGPRs, LDS, grid, block
   rocprof_summary.py pmc <results.db> [substring]  per-kernel counter totals and the value
                                                    per dispatch (FETCH_SIZE / WRITE_SIZE in KiB)
+  rocprof_summary.py traffic <fetch.db> <write.db> [f_fetch f_write]
+                                                   per-dispatch FETCH_SIZE / WRITE_SIZE and the
+                                                   bytes after copy-calibration factors
 """
 import sqlite3
 import sys
@@ -40,11 +43,31 @@ def pmc(path, sub=None):
             print(f"{n[:60]:60s} {c:16s} {v:14.6g} {k:10d} {v / max(k, 1):14.6g}")
 
 
+def traffic(fetch_db, write_db, f_fetch=1.0, f_write=1.0, top=16):
+    """Per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) of two single-counter runs of the same
+    command, and the bytes they stand for after the copy-kernel calibration factors."""
+    def per_dispatch(path, counter):
+        db = sqlite3.connect(path)
+        rows = db.execute("select dispatch_id, name, sum(counter_value) from pmc_events "
+                          "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
+        return {(d, n): v for d, n, v in rows}
+    fe, wr = per_dispatch(fetch_db, "FETCH_SIZE"), per_dispatch(write_db, "WRITE_SIZE")
+    rows = [(n, d, fe[(d, n)], wr.get((d, n), 0.0)) for (d, n) in fe]
+    rows.sort(key=lambda r: -(r[2] * f_fetch + r[3] * f_write))
+    print(f"{'kernel':34s} {'dispatch':>8s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} {'read_MB':>10s} {'write_MB':>10s}")
+    for n, d, f, w in rows[:top]:
+        short = n.split("(")[0].replace("void ", "")
+        print(f"{short[:34]:34s} {d:8d} {f:12.1f} {w:12.1f} {f * 1024 * f_fetch / 1e6:10.1f} {w * 1024 * f_write / 1e6:10.1f}")
+
+
 if __name__ == "__main__":
-    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "traffic"):
         print(__doc__)
         sys.exit(2)
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
-    else:
+    elif sys.argv[1] == "pmc":
         pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
+    else:
+        f = [float(x) for x in sys.argv[4:6]] if len(sys.argv) >= 6 else [1.0, 1.0]
+        traffic(sys.argv[2], sys.argv[3], f[0], f[1])
