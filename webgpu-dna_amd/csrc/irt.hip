@@ -600,6 +600,29 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
     const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
     const uint64_t ps = dna_pair_state(tk, a, b);
     const double dt = t_new - tb[jj];
+    {
+        // FP32 screen of the same step: each normal g = sqrt(-2 ln(u + 1e-30)) cos(6.2831853 u')
+        // is within 2e-3 + 1e-4 |g| of the FP64 one (hardware log2/cos in revolutions: the
+        // input rounding moves -2 ln u by < 2e-7, which the square root turns into < 4.5e-4 near
+        // 0; relative errors are ~1e-6), so r0 >= r0_f - M below, and a pair that the bound
+        // rejects at r0_f - M is rejected by the exact test too
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z, sg = 0.f, gsum = 0.f;
+        if (dt > 0) {
+            sg = sqrt_bound((float)(2.0 * sh.Dd[sj] * dt));
+            float g[3];
+            for (int k = 0; k < 3; k++) {
+                const float ua = (float)(dna_u01(ps, 2u * k) + 1e-30), ub = (float)dna_u01(ps, 2u * k + 1u);
+                const float L = fmaxf(0.f, -1.3862944f * __builtin_amdgcn_logf(ua));      // -2 ln ua
+                g[k] = sqrt_bound(L) * __builtin_amdgcn_cosf(ub);                        // cos(2 pi ub)
+                gsum += fabsf(g[k]);
+            }
+            dx -= g[0] * sg; dy -= g[1] * sg; dz -= g[2] * sg;
+        }
+        const float r0f = sqrt_bound(dx * dx + dy * dy + dz * dz);
+        const float M = sg * (6e-3f + 2e-4f * gsum) + 1e-5f * (r0f + sg * gsum) + 1e-6f;
+        const float r0lo = r0f - M;
+        if (r0lo > c_irt.R_CUT_F || cannot_react(sh.bnd[pc], r0lo, u01_lo(ps, 6u))) return;
+    }
     double xj = pj.x, yj = pj.y, zj = pj.z;
     if (dt > 0) {
         double sig = sqrt(2.0 * sh.Dd[sj] * dt);
