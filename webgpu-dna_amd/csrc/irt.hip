@@ -446,11 +446,15 @@ __device__ __forceinline__ void pdc_setup(const PendPair &q, double &a, double &
     pq = qq;
 }
 
-// one SamplePDC trial (irt-worker.js:95-111); returns accept.  The acceptance value
-// lam = exp(-b^2/X) (1 - a sqrt(pi X) erfcx(...)) is at most exp(-b^2/X) (a > 0 and
-// erfcx > 0), so a trial whose uniform exceeds that bound is rejected before the FP64
-// exp and erfcx: the bound is taken in FP32 with a 1e-4 relative and 1e-37 absolute
-// margin, far above its rounding, so only trials the exact test rejects are skipped.
+// one SamplePDC trial (irt-worker.js:95-111); returns accept.  The acceptance value is
+// lam = E f with E = exp(-b^2/X), f = 1 - a sqrt(pi X) erfcx(z), z = b/sqrt(X) + a sqrt(X).
+// Most trials are decided in FP32 first, with margins far above FP32 rounding, so only
+// trials whose uniform lies within the margins of lam reach the FP64 exp and erfcx:
+//  * always lam <= E (a > 0, erfcx > 0): E is taken with a 1e-4 relative and 1e-37
+//    absolute margin;
+//  * for z >= 0, 1/(z + sqrt(z^2 + 2)) < (sqrt(pi)/2) erfcx(z) <= 1/(z + sqrt(z^2 + 4/pi))
+//    (Abramowitz & Stegun 7.1.13) bracket f; z and a sqrt(pi X) are taken with 1e-5
+//    relative margins (the bounds decrease in z) and f with 1e-5 absolute ones.
 __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q, double M, uint64_t ps,
                                           uint32_t trial, double &X)
 {
@@ -461,11 +465,31 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
     U = dna_u01(ps, 8u + 2u * trial);
     {
         const bool c1 = X <= 2.0 * b / a, c2 = X >= 2.0 * b / a;
-        const float ub = expf(-(float)(b * b / X)) * 1.0001f + 1e-37f;
+        const float E = expf(-(float)(b * b / X));
+        const float Ehi = E * 1.0001f + 1e-37f;
         const float fU = (float)U;
-        const bool may1 = c1 && !(fU > ub);
-        const bool may2 = c2 && !(fU * (float)M * 0.9999f > ub * (float)X * 1.0001f);
-        if (!may1 && !may2) return false;
+        // the value compared with lam: U (X below 2b/a) or U M / X (above); X = 2b/a exactly
+        // (both tests) and NaN go to the exact path
+        const float w = c1 ? fU : fU * (float)M / (float)X;
+        const float whi = w * 1.0001f, wlo = w * 0.9999f;
+        if (c1 != c2) {
+            if (wlo > Ehi) return false;
+            const float Xf = (float)X, bf = (float)b;
+            if (bf >= 0.f) {
+                const float sq = sqrtf(Xf);
+                const float z = bf / sq + (float)a * sq;
+                const float K = (float)a * sq * 1.7724539f;                 // a sqrt(pi X)
+                const float zl = z * 0.99999f, zh = z * 1.00001f;
+                const float Lh = 1.1283792f / (zh + sqrtf(zh * zh + 2.f));   // erfcx(z) >  Lh
+                const float Ul = 1.1283792f / (zl + sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
+                const float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
+                const float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
+                const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
+                const float llo = (E * 0.9999f - 1e-37f) * fmaxf(flo, 0.f);
+                if (wlo > lhi) return false;
+                if (whi < llo && llo > 0.f) return true;
+            }
+        }
     }
     double sqX = sqrt(X);
     double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
@@ -476,43 +500,43 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
 #define LONG_ITEMS 512u
 
 // per-lane schedule; a pair still running after LANE_TRIALS trials is handed to the
-// wavefront schedule (sh.long_item / sh.long_n) and resumed there at that trial
+// wavefront schedule (sh.long_item / sh.long_n) and resumed there at that trial.  Lanes
+// take new pairs together, every REFILL trials: a lane whose pair ends waits for the next
+// refill, so the first stage and the SamplePDC setup (a global load and FP64 code) run once
+// per refill for all idle lanes instead of in almost every trial.
+#define REFILL 8u
 __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
                                 uint32_t npend, double t_base)
 {
-    uint32_t item = atomicAdd(&sh.pend_next, 1u);
-    bool act = item < npend;
     double a = 1, b = 1, p = 0, q = 0, M = 0;
     PendPair pp; pp.a = 0; pp.b = 0; pp.D = 1; pp.ps = 0; pp.r0 = 0; pp.ri = 0;
-    uint32_t trial = 0;
-    bool need_setup = true;
-    while (act) {
-        bool done = false;
-        double X = -1;
-        if (need_setup) {
-            pp = load_pending(A, T, sh, tk, item);
-            double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
-            if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; need_setup = false; }
-            else {
+    uint32_t item = 0, trial = 0;
+    bool have = false, more = true;
+    for (;;) {
+        if (!have && more) {
+            for (;;) {                                   // pairs with no trials end here
+                item = atomicAdd(&sh.pend_next, 1u);
+                if (item >= npend) { more = false; break; }
+                pp = load_pending(A, T, sh, tk, item);
+                const double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
+                if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; have = true; break; }
                 if (t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
-                done = true;
             }
         }
-        if (!done) {
-            bool acc = pdc_trial(a, b, p, q, M, pp.ps, trial, X);
+        if (!__ballot(have)) break;
+        for (uint32_t k = 0; k < REFILL; k++) {
+            if (!have) continue;
+            double X = -1;
+            const bool acc = pdc_trial(a, b, p, q, M, pp.ps, trial, X);
             trial++;
             if (acc) {
                 if (X > 0) { double ta = t_base + X / pp.D; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
-                done = true;
+                have = false;
             } else if (trial == LANE_TRIALS) {
-                uint32_t l = atomicAdd(&sh.long_n, 1u);
-                if (l < LONG_ITEMS) { sh.long_item[l] = item; done = true; }
-            } else if (trial == 10000u) done = true;
-        }
-        if (done) {
-            item = atomicAdd(&sh.pend_next, 1u);
-            act = item < npend;
-            need_setup = true;
+                // to the wavefront schedule; when its list is full the lane goes on itself
+                const uint32_t l = atomicAdd(&sh.long_n, 1u);
+                if (l < LONG_ITEMS) { sh.long_item[l] = item; have = false; }
+            } else if (trial == 10000u) have = false;
         }
     }
 }
