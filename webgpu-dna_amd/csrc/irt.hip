@@ -26,6 +26,7 @@
 #include <vector>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 namespace dna {
 
@@ -1240,8 +1241,11 @@ struct GroupScratch {
 };
 
 
+static std::mutex g_const_mu;                 // contexts on several host threads
+
 static int ensure_const(dna_ctx *c)
 {
+    std::lock_guard<std::mutex> lk(g_const_mu);
     if (!g_const_ready[c->device]) {
         IrtConst C; build_const(C);
         DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_irt), &C, sizeof(C)));

@@ -18,6 +18,7 @@
 #include "dna_ctx.h"
 #include <vector>
 #include <cmath>
+#include <mutex>
 #include <cstring>
 
 namespace dna {
@@ -227,6 +228,7 @@ template <class T> static hipError_t sgrow(void *&ptr, size_t &cap, size_t need_
 }
 
 static bool g_lcg_ready[64] = {false};
+static std::mutex g_lcg_mu;                   // contexts on several host threads
 
 }  // namespace dna
 
@@ -244,6 +246,7 @@ extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, 
     DNA_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     memset(out, 0, sizeof(*out));
+    std::unique_lock<std::mutex> lk(g_lcg_mu);
     if (!g_lcg_ready[c->device]) {
         uint32_t A[32], Cc[32];
         uint32_t a = 1664525u, cc = 1013904223u;
@@ -252,6 +255,7 @@ extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, 
         DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_c), Cc, sizeof(Cc)));
         g_lcg_ready[c->device] = true;
     }
+    lk.unlock();
     ScoreGeom G;
     G.x_half = (t->n_bp_per - 1) * t->rise * 0.5;
     G.rise_inv = 1 / t->rise;
