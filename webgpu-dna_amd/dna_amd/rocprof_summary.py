@@ -6,6 +6,9 @@ profiles/.
                                                    duration (ms), VGPRs, LDS, grid, block
   rocprof_summary.py pmc <results.db> [substring]  per-kernel counter totals and the value
                                                    per dispatch (FETCH_SIZE / WRITE_SIZE in KiB)
+  rocprof_summary.py dispatches <results.db> <substring>
+                                                   every dispatch of the matching kernels in
+                                                   start order (queue, start, duration)
   rocprof_summary.py traffic <fetch.db> <write.db> [f_fetch f_write]
                                                    per-dispatch FETCH_SIZE / WRITE_SIZE and the
                                                    bytes after copy-calibration factors
@@ -43,6 +46,20 @@ def pmc(path, sub=None):
             print(f"{n[:60]:60s} {c:16s} {v:14.6g} {k:10d} {v / max(k, 1):14.6g}")
 
 
+def dispatches(path, sub):
+    """Every dispatch of the kernels whose name contains `sub`, in start order: dispatch id,
+    queue, start offset (ms from the first one), duration (ms), grid and workgroup size."""
+    db = sqlite3.connect(path)
+    rows = db.execute("select dispatch_id, queue_id, start, duration, grid_x, workgroup_x, name "
+                      "from kernels where name like ? order by start", (f"%{sub}%",)).fetchall()
+    if not rows:
+        return
+    t0 = rows[0][2]
+    print(f"{'dispatch':>8s} {'queue':>5s} {'start_ms':>10s} {'dur_ms':>9s} {'grid':>8s} {'wg':>4s}")
+    for d, q, st, du, gx, wg, n in rows:
+        print(f"{d:8d} {q:5d} {(st - t0) / 1e6:10.3f} {du / 1e6:9.3f} {gx:8d} {wg:4d}")
+
+
 def traffic(fetch_db, write_db, f_fetch=1.0, f_write=1.0, top=16):
     """Per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) of two single-counter runs of the same
     command, and the bytes they stand for after the copy-kernel calibration factors."""
@@ -61,11 +78,13 @@ def traffic(fetch_db, write_db, f_fetch=1.0, f_write=1.0, top=16):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "traffic"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "traffic", "dispatches"):
         print(__doc__)
         sys.exit(2)
     if sys.argv[1] == "trace":
         trace(sys.argv[2])
+    elif sys.argv[1] == "dispatches":
+        dispatches(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
     elif sys.argv[1] == "pmc":
         pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
     else:
