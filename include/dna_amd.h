@@ -192,6 +192,15 @@ int dna_hash_cells(struct dna_ctx *ctx, const float *pos4, uint64_t n, float cel
                    uint32_t hash_bits, uint32_t *key_out, uint32_t *cell_start_out,
                    uint32_t *order_out);
 
+/* The step's table lookups on the GPU for given energies (host arrays): which 0 =
+ * xs_all (helpers.wgsl:15-27) -> out[3n] (sigma_ion, sigma_exc, sigma_el in nm^2),
+ * 1 = xs_shell_fracs (:29-43) -> out[5n], 2 = xs_exc_fracs (:45-64) -> out[5n],
+ * 3 = xs_vib_total (:135-156) -> out[n], 4 = xs_dea (:172-186) -> out[n],
+ * 5 = xs_el_cos (:66-82) at (E[i], r[i]) -> out[n].  The same code and LDS tables the
+ * transport kernels use; it replaces the in-shader evaluations that the reference's
+ * cross-section experiments probe (experiments/level-1, level-2). */
+int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r, uint64_t n, float *out);
+
 /* Launch options (no reference counterpart: WebGPU fixes its workgroup sizes in the
  * shaders).  DNA_OPT_IRT_BLOCK: IRT workgroup size, 0 = autotune (default: the first
  * launch of each track-size class times 256/128/64-thread workgroups on up to 3072 of

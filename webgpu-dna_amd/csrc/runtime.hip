@@ -107,6 +107,31 @@ void build_born_guide(const dna_tables &t, uint8_t *guide)
 }
 }  // namespace dna
 
+extern "C" int dna_xs_probe(dna_ctx *c, int which, const float *E, const float *r, uint64_t n, float *out)
+{
+    if (!c || !E || !out || which < 0 || which > 5 || (which == 5 && !r)) {
+        set_error("dna_xs_probe: bad argument"); return DNA_ERR_ARG;
+    }
+    if (n >= (1ull << 31)) { set_error("dna_xs_probe: at most 2^31-1 energies"); return DNA_ERR_CAPACITY; }
+    if (n == 0) return DNA_OK;
+    static const int width[6] = {3, 5, 5, 1, 1, 1};
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    float *dE = nullptr, *dr = nullptr, *dout = nullptr;
+    const size_t wout = (size_t)width[which];
+    hipError_t e = hipMalloc(&dE, n * 4);
+    if (e == hipSuccess && which == 5) e = hipMalloc(&dr, n * 4);
+    if (e == hipSuccess) e = hipMalloc(&dout, n * wout * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(dE, E, n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && dr) e = hipMemcpyAsync(dr, r, n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = launch_xs_probe(c->d_tab, c->d_guide, dE, dr, (uint32_t)n, which, dout, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * wout * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(dE); hipFree(dr); hipFree(dout);
+    DNA_HIP(e);
+    return DNA_OK;
+}
+
 extern "C" int dna_set_option(dna_ctx *c, int option, int value)
 {
     if (!c) { set_error("dna_set_option: NULL context"); return DNA_ERR_ARG; }

@@ -877,6 +877,36 @@ hipError_t launch_phase_a(const TransportDev &d, hipStream_t st)
     return hipGetLastError();
 }
 
+// The table lookups of the step evaluated for given energies (and uniforms r), with the
+// staged LDS tables the kernels use: which 0 xs_all -> 3 values, 1 shell fractions -> 5,
+// 2 excitation-level fractions -> 5, 3 vibrational total -> 1, 4 DEA -> 1,
+// 5 elastic cos(theta) at (E, r) -> 1.
+__global__ void __launch_bounds__(256) k_xs_probe(const dna_tables *T, const uint8_t *guide, const float *E,
+                                                  const float *r, uint32_t n, int which, float *out)
+{
+    __shared__ SmallTab S;
+    stage_tables(S, T, guide);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float e = E[i];
+        if (which == 0) { float a, b, c; xs_all(S, e, a, b, c); out[3 * i] = a; out[3 * i + 1] = b; out[3 * i + 2] = c; }
+        else if (which == 1) { float f[5]; xs_shell_fracs(S, e, f); for (int k = 0; k < 5; k++) out[5 * i + k] = f[k]; }
+        else if (which == 2) { float f[5]; xs_exc_fracs(S, e, f); for (int k = 0; k < 5; k++) out[5 * i + k] = f[k]; }
+        else if (which == 3) out[i] = xs_vib_total(S, e);
+        else if (which == 4) out[i] = xs_dea(S, e);
+        else out[i] = xs_el_cos(S, e, r[i]);
+    }
+}
+
+hipError_t launch_xs_probe(const dna_tables *T, const uint8_t *guide, const float *E, const float *r, uint32_t n,
+                           int which, float *out, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    uint32_t blocks = (n + 255u) / 256u;
+    if (blocks > 1024u) blocks = 1024u;
+    hipLaunchKernelGGL(k_xs_probe, dim3(blocks), dim3(256), 0, st, T, guide, E, r, n, which, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_phase_b(const TransportDev &d, uint32_t sec_n, hipStream_t st)
 {
     uint32_t lanes_max = (uint32_t)num_cu() * 4u * 64u * 4u;

@@ -48,6 +48,7 @@ def lib():
     L.dna_irt_plan_destroy.restype = None
     L.dna_set_option.argtypes = [vp, C.c_int, C.c_int]
     L.dna_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    L.dna_xs_probe.argtypes = [vp, C.c_int, vp, vp, u64, vp]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
                                      C.POINTER(DnaChemOut)]
     ip = C.POINTER(C.c_int)
@@ -75,7 +76,7 @@ EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
-                    "dna_surrogate_predict"]
+                    "dna_surrogate_predict", "dna_xs_probe"]
 
 
 def _check(fn, rc):
@@ -115,6 +116,20 @@ class Context:
         v = C.c_int()
         _check("dna_get_option", lib().dna_get_option(self.h, option, C.byref(v)))
         return v.value
+
+    # ---- table lookups of the transport step (dna_xs_probe)
+    XS_WIDTH = {0: 3, 1: 5, 2: 5, 3: 1, 4: 1, 5: 1}
+
+    def xs_probe(self, which, E, r=None):
+        """Evaluate a transport lookup on the GPU: 0 xs_all -> [n,3] (ion, exc, el nm^2),
+        1 shell fractions [n,5], 2 excitation fractions [n,5], 3 vibrational total [n,1],
+        4 DEA [n,1], 5 elastic cos(theta) at (E, r) [n,1]."""
+        E = np.ascontiguousarray(E, np.float32).ravel()
+        rr = None if r is None else np.ascontiguousarray(r, np.float32).ravel()
+        out = np.empty((E.size, self.XS_WIDTH[which]), np.float32)
+        _check("dna_xs_probe", lib().dna_xs_probe(self.h, which, _ptr(E), None if rr is None else _ptr(rr),
+                                                  E.size, _ptr(out)))
+        return out
 
     # ---- Phase A + B
     def transport(self, params, energies=None, copy=DNA_COPY_ALL):
