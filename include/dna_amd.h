@@ -146,6 +146,11 @@ typedef struct dna_score_out {
     double   t_ms;
 } dna_score_out;
 
+/* scoreIndirectSSB result (src/physics/types.ts:122-128) without the hit array */
+typedef struct dna_indirect_out {
+    uint64_t ssb0, ssb1, candidates, in_reach;
+} dna_indirect_out;
+
 /* ------------------------------------------------------------ entry points */
 
 /* Create a context on HIP device `device`, loading the cross-section table file

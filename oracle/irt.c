@@ -266,9 +266,9 @@ static void snapshot(const work *w, uint32_t n_total, uint64_t h2, uint64_t *tal
 }
 
 /* Phase 3 body for one primary, irt-worker.js:470-660 */
-static void run_track(work *w, const float *rad, const uint64_t *idx, uint32_t n, uint64_t tk,
-                      uint64_t init_h2, int64_t CAP, const double *tc, int nc,
-                      uint64_t (*tally)[5], uint64_t *rxn_counts, uint64_t *n_reacted, uint64_t *n_events)
+static uint32_t run_track(work *w, const float *rad, const uint64_t *idx, uint32_t n, uint64_t tk,
+                          uint64_t init_h2, int64_t CAP, const double *tc, int nc,
+                          uint64_t (*tally)[5], uint64_t *rxn_counts, uint64_t *n_reacted, uint64_t *n_events)
 {
     work_reserve(w, CAP > n ? CAP : n);
     for (uint32_t k = 0; k < n; k++) {
@@ -350,6 +350,36 @@ static void run_track(work *w, const float *rad, const uint64_t *idx, uint32_t n
         }
     }
     while (cp < nc) { snapshot(w, n_total, h2, tally[cp]); cp++; }
+    return n_total;
+}
+
+/* The OH alive at the end of a track (1 us), in particle order, each moved by one Brownian
+ * step over t_end - t_birth with draws 32..37 of its own stream (tk, k, k): Box-Muller as in
+ * the re-pairing step (irt-worker.js:443-456).  The worker keeps no positions at 1 us
+ * (chem_pos_final = null, worker.ts:69-70); this is the definition the indirect-SSB
+ * scoring (ssb-dsb.ts:43-121) is given here. */
+static void collect_oh(const work *w, uint32_t n_total, uint64_t tk, float **buf, uint64_t *n)
+{
+    uint64_t cnt = 0;
+    for (uint32_t k = 0; k < n_total; k++) if (w->alive[k] && w->sp[k] == 0) cnt++;
+    *buf = (float *)malloc(16 * (cnt ? cnt : 1));
+    *n = cnt;
+    uint64_t o = 0;
+    for (uint32_t k = 0; k < n_total; k++) {
+        if (!w->alive[k] || w->sp[k] != 0) continue;
+        double x = w->x[k], y = w->y[k], z = w->z[k];
+        double dt = DNA_T_END_NS - w->tb[k];
+        if (dt > 0) {
+            uint64_t ps = dna_pair_state(tk, k, k);
+            double sig = sqrt(2.0 * IRT_D[0] * dt);
+            double g0 = sqrt(-2 * dna_log(dna_u01(ps, 32) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 33));
+            double g1 = sqrt(-2 * dna_log(dna_u01(ps, 34) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 35));
+            double g2 = sqrt(-2 * dna_log(dna_u01(ps, 36) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 37));
+            x += g0 * sig; y += g1 * sig; z += g2 * sig;
+        }
+        float *q = *buf + 4 * o++;
+        q[0] = (float)x; q[1] = (float)y; q[2] = (float)z; q[3] = 0.0f;
+    }
 }
 
 typedef struct { int64_t pid; uint64_t first, n, h2; } trk;
@@ -364,7 +394,7 @@ static int cmp_u64pair(const void *a, const void *b)
 /* One worker invocation (irt-worker.js:370-699) over rad[0..rad_n); instance keys the RNG. */
 static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
                         const double *tc, int nc, uint64_t seed, uint64_t instance, int n_threads,
-                        dna_chem_out *out)
+                        dna_chem_out *out, float **oh_out, uint64_t *oh_n)
 {
     init_params();
     memset(out, 0, sizeof(*out));
@@ -411,6 +441,8 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
     uint64_t (*tally)[DNA_MAX_CKPT][5] = calloc((size_t)n_threads, sizeof(*tally));
     uint64_t (*rxc)[DNA_N_RXN] = calloc((size_t)n_threads, sizeof(*rxc));
     uint64_t *nre = calloc((size_t)n_threads, 8), *nev = calloc((size_t)n_threads, 8);
+    float **oh_tr = oh_out ? (float **)calloc(ntr ? ntr : 1, sizeof(float *)) : NULL;
+    uint64_t *oh_tn = oh_out ? (uint64_t *)calloc(ntr ? ntr : 1, 8) : NULL;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(n_threads)
 #endif
@@ -425,8 +457,9 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
 #endif
         for (int64_t t = 0; t < (int64_t)ntr; t++) {
             uint64_t tk = dna_track_key(seed, instance, (uint64_t)tr[t].pid);
-            run_track(&w, rad, idx + tr[t].first, (uint32_t)tr[t].n, tk, tr[t].h2, CAP, tc, nc,
-                      tally[tid], rxc[tid], &nre[tid], &nev[tid]);
+            uint32_t nt = run_track(&w, rad, idx + tr[t].first, (uint32_t)tr[t].n, tk, tr[t].h2, CAP, tc, nc,
+                                    tally[tid], rxc[tid], &nre[tid], &nev[tid]);
+            if (oh_tr) collect_oh(&w, nt, tk, &oh_tr[t], &oh_tn[t]);
         }
         free(w.x); free(w.y); free(w.z); free(w.sp); free(w.alive); free(w.tb); free(w.h.v);
     }
@@ -438,6 +471,19 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
     double per100 = (double)n_therm * (double)E_eV / 100.0;
     for (int c = 0; c < nc; c++) for (int s = 0; s < 5; s++)
         out->G[c][s] = per100 > 0 ? (double)out->tally[c][s] / per100 : 0.0;
+    if (oh_out) {                         /* concatenate in track order */
+        uint64_t tot = 0;
+        for (uint64_t t = 0; t < ntr; t++) tot += oh_tn[t];
+        *oh_out = (float *)malloc(16 * (tot ? tot : 1));
+        uint64_t o = 0;
+        for (uint64_t t = 0; t < ntr; t++) {
+            memcpy(*oh_out + 4 * o, oh_tr[t], 16 * oh_tn[t]);
+            o += oh_tn[t];
+            free(oh_tr[t]);
+        }
+        *oh_n = tot;
+        free(oh_tr); free(oh_tn);
+    }
     free(tally); free(rxc); free(nre); free(nev); free(tr); free(idx); free(pairs);
     return 0;
 }
@@ -446,7 +492,17 @@ int oracle_irt(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
                const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out)
 {
     if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
-    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out);
+    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, NULL, NULL);
+}
+
+/* oracle_irt, then the OH alive at 1 us (collect_oh) of every track in track order;
+ * *oh_out is malloc'ed (oracle_free). */
+int oracle_irt_oh(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                  const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out,
+                  float **oh_out, uint64_t *oh_n)
+{
+    if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
+    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, oh_out, oh_n);
 }
 
 int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, const uint32_t *n_therm,
@@ -456,7 +512,7 @@ int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, con
     if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
     for (uint32_t i = 0; i < n_inst; i++) {
         int rc = irt_instance(rad + 4 * off[i], off[i + 1] - off[i], n_therm[i], E[i], t_ckpt, n_ckpt,
-                              seed, i, n_threads, &out[i]);
+                              seed, i, n_threads, &out[i], NULL, NULL);
         if (rc) return rc;
     }
     return 0;

@@ -67,6 +67,11 @@ int oracle_irt(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
 int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, const uint32_t *n_therm,
                      const float *E, const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads,
                      dna_chem_out *out);
+/* oracle_irt plus the OH alive at 1 us (positions after one Brownian step to 1 us) of every
+ * track in track order; *oh_out is malloc'ed, release with oracle_free */
+int oracle_irt_oh(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                  const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out,
+                  float **oh_out, uint64_t *oh_n);
 /* reaction parameter table (irt-worker.js:240-279): 9 x {sigma, sigma_eff, rc, kdif, kact, prob} */
 void oracle_irt_params(double *out54);
 /* one IRT reaction-time draw for reaction ri at distance r0 with the RNG stream of
@@ -79,6 +84,9 @@ int oracle_score(const dna_dna_target *tgt, const float *rad, uint64_t rad_n,
                  dna_score_out *out);
 
 void oracle_cluster(const dna_dna_target *tgt, const uint8_t *hits, dna_score_out *out);
+/* indirect SSB from alive-OH positions first, then direct, combined hits, clustering */
+int oracle_score_oh(const dna_dna_target *tgt, const float *rad, uint64_t rad_n, const float *oh, uint64_t n_oh,
+                    uint32_t *lcg_state, uint8_t *hits_out, dna_score_out *out, dna_indirect_out *ind);
 /* Spatial hash (chemistry.wgsl:66-80,110-122) */
 void oracle_hash_cells(const float *pos4, uint64_t n, float cell_nm, uint32_t hash_bits,
                        uint32_t *key_out, uint32_t *cell_start_out, uint32_t *order_out);

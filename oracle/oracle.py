@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "webgpu-dna_amd"))
-from dna_amd.abi import DnaChemOut, DnaParams, DnaResult, DnaScoreOut, DnaTarget  # noqa: E402
+from dna_amd.abi import DnaChemOut, DnaIndirectOut, DnaParams, DnaResult, DnaScoreOut, DnaTarget  # noqa: E402
 
 TABLES = os.path.join(ROOT, "webgpu-dna_amd", "data", "cross_sections.bin")
 LIB = os.path.join(HERE, "liboracle.so")
@@ -68,6 +68,12 @@ def lib():
         L.oracle_score.argtypes = [C.POINTER(DnaTarget), C.c_void_p, C.c_uint64,
                                    C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(DnaScoreOut)]
         L.oracle_cluster.argtypes = [C.POINTER(DnaTarget), C.c_void_p, C.POINTER(DnaScoreOut)]
+        L.oracle_irt_oh.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_float, C.c_void_p, C.c_int,
+                                    C.c_uint64, C.c_int, C.POINTER(DnaChemOut), C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_uint64)]
+        L.oracle_score_oh.argtypes = [C.POINTER(DnaTarget), C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                      C.POINTER(C.c_uint32), C.c_void_p, C.POINTER(DnaScoreOut),
+                                      C.POINTER(DnaIndirectOut)]
         L.oracle_hash_cells.argtypes = [C.c_void_p, C.c_uint64, C.c_float, C.c_uint32,
                                         C.c_void_p, C.c_void_p, C.c_void_p]
         rc = L.oracle_load_tables(TABLES.encode())
@@ -152,6 +158,22 @@ def irt(rad, n_therm, E_eV, t_ckpt, seed=1, n_threads=0):
     return o
 
 
+def irt_oh(rad, n_therm, E_eV, t_ckpt, seed=1, n_threads=0):
+    """oracle_irt plus the OH alive at 1 us ([n, 4] float32, track order)."""
+    rad = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+    tc = np.ascontiguousarray(t_ckpt, np.float64)
+    o = DnaChemOut()
+    p, n = C.c_void_p(), C.c_uint64()
+    rc = lib().oracle_irt_oh(_ptr(rad), rad.shape[0], n_therm, E_eV, _ptr(tc), tc.size, seed,
+                             n_threads, C.byref(o), C.byref(p), C.byref(n))
+    if rc:
+        raise RuntimeError(f"oracle_irt_oh rc={rc}")
+    oh = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(max(n.value, 1) * 4,))[:n.value * 4]
+    oh = oh.reshape(-1, 4).copy()
+    lib().oracle_free(p)
+    return o, oh
+
+
 def irt_batch(rad, inst_off, n_therm, E, t_ckpt, seed=1, n_threads=0):
     rad = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
     off = np.ascontiguousarray(inst_off, np.uint64)
@@ -186,6 +208,24 @@ def score(target, rad, lcg_state=0x12345678, want_hits=False):
     lib().oracle_score(C.byref(t), _ptr(rad), rad.shape[0], C.byref(st), _ptr(hits), C.byref(o))
     d = dict(ssb_count=o.ssb_count, candidates=o.candidates, in_reach=o.in_reach, dsb=o.dsb,
              ssb0=o.ssb0, ssb1=o.ssb1, dsb_expected=o.dsb_expected, lcg_state=st.value)
+    if want_hits:
+        d["hits"] = hits
+    return d
+
+
+def score_oh(target, rad, oh, lcg_state=0x12345678, want_hits=False):
+    """Indirect SSB from the OH list, then direct, combined hits and clustering."""
+    rad = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+    oh = np.ascontiguousarray(oh, np.float32).reshape(-1, 4)
+    t = target.to_c()
+    st = C.c_uint32(lcg_state)
+    hits = np.zeros(2 * target.n_bp, np.uint8) if want_hits else None
+    o, ind = DnaScoreOut(), DnaIndirectOut()
+    lib().oracle_score_oh(C.byref(t), _ptr(rad), rad.shape[0], _ptr(oh), oh.shape[0], C.byref(st),
+                          _ptr(hits), C.byref(o), C.byref(ind))
+    d = dict(ssb_count=o.ssb_count, candidates=o.candidates, in_reach=o.in_reach, dsb=o.dsb,
+             ssb0=o.ssb0, ssb1=o.ssb1, dsb_expected=o.dsb_expected, lcg_state=st.value,
+             indirect=dict(ssb0=ind.ssb0, ssb1=ind.ssb1, candidates=ind.candidates, in_reach=ind.in_reach))
     if want_hits:
         d["hits"] = hits
     return d

@@ -63,6 +63,54 @@ static void direct_events(const dna_dna_target *d, const float *rad, uint64_t ra
     }
 }
 
+/* scoreIndirectSSB, ssb-dsb.ts:43-121, on a list of alive OH positions: the same geometry
+ * with the OH reaction radius 0.29 nm and p = 0.4 (constants.ts:122-123), no duplicate
+ * skipping, one LCG draw per in-reach OH in list order, hits in its own array */
+static void indirect_oh(const dna_dna_target *d, const float *oh, uint64_t n, uint32_t *lcg, uint8_t *hits,
+                        dna_indirect_out *o)
+{
+    const double r_damage = 0.29, r2d = r_damage * r_damage, p_ssb = 0.4;
+    double x_half = (d->n_bp_per - 1) * d->rise * 0.5;
+    double rise_inv = 1 / d->rise;
+    double grid_off = -((d->grid_N - 1) * d->spacing_nm) * 0.5;
+    double inv_spacing = 1 / d->spacing_nm;
+    for (uint64_t i = 0; i < n; i++) {
+        double x = oh[4*i], y = oh[4*i+1], z = oh[4*i+2];
+        if (x < -x_half - r_damage || x > x_half + r_damage) continue;
+        double fi = js_round((y - grid_off) * inv_spacing);
+        double fj = js_round((z - grid_off) * inv_spacing);
+        if (fi < 0 || fi >= d->grid_N || fj < 0 || fj >= d->grid_N) continue;
+        int64_t fiber = (int64_t)fi * d->grid_N + (int64_t)fj;
+        double y_rel = y - d->fy[fiber], z_rel = z - d->fz[fiber];
+        double r2 = y_rel * y_rel + z_rel * z_rel;
+        double outer = d->r_bb + r_damage;
+        if (r2 > outer * outer) continue;
+        o->candidates++;
+        double bp_est = js_round((x + x_half) * rise_inv);
+        int64_t b0 = (int64_t)(bp_est - 2 > 0 ? bp_est - 2 : 0);
+        int64_t b1 = (int64_t)(bp_est + 2 < d->n_bp_per - 1 ? bp_est + 2 : d->n_bp_per - 1);
+        double best = INFINITY; int64_t best_bp = -1; int best_strand = -1;
+        for (int64_t b = b0; b <= b1; b++) {
+            double dx = x - (d->x0 + b * d->rise);
+            double dy0 = y_rel - d->rbb0[2*b], dz0 = z_rel - d->rbb0[2*b+1];
+            double d20 = dx * dx + dy0 * dy0 + dz0 * dz0;
+            if (d20 < best) { best = d20; best_bp = b; best_strand = 0; }
+            double dy1 = y_rel - d->rbb1[2*b], dz1 = z_rel - d->rbb1[2*b+1];
+            double d21 = dx * dx + dy1 * dy1 + dz1 * dz1;
+            if (d21 < best) { best = d21; best_bp = b; best_strand = 1; }
+        }
+        if (best < r2d) {
+            o->in_reach++;
+            *lcg = *lcg * 1664525u + 1013904223u;
+            double u = (double)*lcg / 4294967296.0;
+            if (u < p_ssb) {
+                uint64_t idx = (uint64_t)(fiber * d->n_bp_per + best_bp) + (uint64_t)best_strand * d->n_bp;
+                if (hits[idx] == 0) { hits[idx] = 1; if (best_strand == 0) o->ssb0++; else o->ssb1++; }
+            }
+        }
+    }
+}
+
 /* clusterDSB, ssb-dsb.ts:348-407 */
 static void cluster(const dna_dna_target *d, const uint8_t *hits, dna_score_out *o)
 {
@@ -109,6 +157,25 @@ int oracle_score(const dna_dna_target *tgt, const float *rad, uint64_t rad_n,
     cluster(tgt, hits, out);
     if (hits_out) memcpy(hits_out, hits, 2 * (size_t)tgt->n_bp);
     free(hits);
+    return 0;
+}
+
+/* scoreDamageAt10keV with chemistry positions (app.ts:176-186): indirect SSB from the OH
+ * list first, then direct from rad_buf with the same LCG, combineHits (ssb-dsb.ts:413-418)
+ * as the OR of the two hit arrays, clusterDSB on the combination */
+int oracle_score_oh(const dna_dna_target *tgt, const float *rad, uint64_t rad_n, const float *oh, uint64_t n_oh,
+                    uint32_t *lcg_state, uint8_t *hits_out, dna_score_out *out, dna_indirect_out *ind)
+{
+    memset(out, 0, sizeof(*out));
+    memset(ind, 0, sizeof(*ind));
+    size_t nh = 2 * (size_t)tgt->n_bp;
+    uint8_t *hi = (uint8_t *)calloc(nh, 1), *hd = (uint8_t *)calloc(nh, 1);
+    indirect_oh(tgt, oh, n_oh, lcg_state, hi, ind);
+    direct_events(tgt, rad, rad_n, lcg_state, hd, out);
+    for (size_t k = 0; k < nh; k++) hd[k] |= hi[k];
+    cluster(tgt, hd, out);
+    if (hits_out) memcpy(hits_out, hd, nh);
+    free(hi); free(hd);
     return 0;
 }
 

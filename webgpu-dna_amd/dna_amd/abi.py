@@ -67,6 +67,11 @@ class DnaScoreOut(C.Structure):
     ]
 
 
+class DnaIndirectOut(C.Structure):
+    _fields_ = [("ssb0", C.c_uint64), ("ssb1", C.c_uint64), ("candidates", C.c_uint64),
+                ("in_reach", C.c_uint64)]
+
+
 def chem_out_to_dict(o: DnaChemOut) -> dict:
     nc = o.n_ckpt
     return {
