@@ -189,6 +189,16 @@ int dna_run_irt_batch(struct dna_ctx *ctx, const float *rad, const uint64_t *ins
 int dna_score(struct dna_ctx *ctx, const dna_dna_target *tgt, const float *rad,
               uint64_t rad_n, uint32_t *lcg_state, int copy_hits, dna_score_out *out);
 
+/* dna_score with the indirect-SSB pass first, as scoreDamageAt10keV does when chemistry
+ * positions exist (src/app.ts:176-186): scoreIndirectSSB (src/scoring/ssb-dsb.ts:43-121)
+ * on the n_oh alive-OH positions oh (host float4, or NULL for the list left on the device
+ * by dna_irt_oh_final), the LCG continuing into the direct pass, and combineHits
+ * (ssb-dsb.ts:413-418) before clusterDSB.  ind receives IndirectSSBResult; out the direct
+ * counts and the clustering of the combined hits. */
+int dna_score_oh(struct dna_ctx *ctx, const dna_dna_target *target, const float *rad, uint64_t rad_n,
+                 const float *oh, uint64_t n_oh, uint32_t *lcg_state, int copy_hits,
+                 dna_score_out *out, dna_indirect_out *ind);
+
 /* Spatial-hash binning: for n points (x,y,z,w float4, host pointer) compute
  * cell = floor(p / cell_nm) (f32 division) and key = Teschner hash mod 2^hash_bits;
  * then counting-sort the points by key: cell_start[2^hash_bits + 1] (CSR offsets) and
@@ -213,8 +223,21 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * only): the size the last IRT launch used.  Results do not depend on these options. */
 #define DNA_OPT_IRT_BLOCK       1
 #define DNA_OPT_IRT_BLOCK_LAST  2
+/* DNA_OPT_IRT_FINAL: 1 = dna_run_irt keeps each track's final state on the device for
+ * dna_irt_oh_final (default 0). */
+#define DNA_OPT_IRT_FINAL       3
 int dna_set_option(struct dna_ctx *ctx, int option, int value);
 int dna_get_option(struct dna_ctx *ctx, int option, int *value);
+
+/* The OH alive at 1 us after the last dna_run_irt made with DNA_OPT_IRT_FINAL = 1, as
+ * (x, y, z, 0) float4 in track order (primaries ascending) and particle order within a
+ * track, each moved by one Brownian step over 1 us - t_birth (sigma = sqrt(2 D_OH dt) per
+ * axis, Box-Muller on draws 32-37 of the particle's own counter-based stream).  The worker
+ * has no positions at 1 us (chem_pos_final = null, src/chemistry/worker.ts:69-70); this is
+ * the input this library gives to the indirect-SSB scoring (dna_score_oh).  The list stays
+ * on the device (oh == NULL in dna_score_oh); out_pos4 (host, may be NULL) receives it when
+ * cap >= *n_out, else DNA_ERR_CAPACITY with *n_out set. */
+int dna_irt_oh_final(struct dna_ctx *ctx, float *out_pos4, uint64_t cap, uint64_t *n_out);
 
 /* Captured batches: the instances of dna_run_irt_batch (rad == NULL: entries
  * [inst_off[0], inst_off[n_inst]) of the radicals resident from the last transport),

@@ -63,4 +63,16 @@ struct dna_ctx {
     int irt_block = 0;
     int irt_last_block = 0;            // workgroup size of the last IRT launch
     std::map<int, int> irt_tuned;      // log2(mean particles per track) -> workgroup size
+    // final IRT state (DNA_OPT_IRT_FINAL) and the alive-OH list made from it
+    int irt_final = 0;
+    bool final_ready = false;
+    uint32_t final_ntr = 0;
+    uint64_t final_seed = 0;
+    const void *final_tracks = nullptr;          // device IrtTrack[final_ntr], launch order
+    const float4 *final_pos = nullptr;
+    const uint8_t *final_alive = nullptr;
+    const double *final_tb = nullptr;
+    uint32_t *d_ntot = nullptr;        size_t cap_ntot = 0;    // particles per track at the end
+    float4 *d_oh = nullptr;            size_t cap_oh = 0;      // alive OH at 1 us
+    uint64_t oh_n = 0;
 };

@@ -65,6 +65,8 @@ struct IrtArgs {
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
+    uint32_t *final_ntot;         // optional: particles per track at the end (track order); the
+                                  // final alive bytes are then left in alive[]
     uint32_t *overflow;
     double tc[DNA_MAX_CKPT];
     int nc;
@@ -996,6 +998,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     int cp = 0;
     // the event loop reads alive flags at random: keep them in LDS, where the sorted
     // copy of the pair phase lived
+    uint8_t *const al_global = al;
     if ((size_t)T.cap <= (size_t)A.lds_slots * 16u) {
         uint8_t *la = (uint8_t *)dyn_lds;
         for (uint32_t k = tid; k < T.cap; k += IRT_NT) la[k] = k < n ? al[k] : 0;
@@ -1138,6 +1141,11 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         trp += q3 - q2;
         trd += q4 - q3;
     }
+    if (A.final_ntot) {                       // the final state for dna_irt_oh_final
+        const uint32_t nt = sh.n_total;
+        if (al != al_global) for (uint32_t k = tid; k < nt; k += IRT_NT) al_global[k] = al[k];
+        if (tid == 0) A.final_ntot[T.idx] = nt;
+    }
     for (int o = 32; o > 0; o >>= 1) {
         wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); wrep += __shfl_xor(wrep, o, 64);
     }
@@ -1161,6 +1169,65 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
+    }
+}
+
+// ------------------------------------------------------------ final alive OH
+// One workgroup per track (launch order), after a k_irt launch that kept its final state.
+__global__ void __launch_bounds__(256) k_oh_count(const IrtTrack *tracks, const uint8_t *alive, const uint32_t *ntot,
+                                                  uint32_t *cnt)
+{
+    const IrtTrack T = tracks[blockIdx.x];
+    const uint8_t *al = alive + T.part_off;
+    const uint32_t nt = ntot[T.idx];
+    uint32_t c = 0;
+    for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) c += al[k] == 1u ? 1u : 0u;   // species 0 + 1
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    __shared__ uint32_t w[4];
+    if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) { uint32_t t = 0; for (uint32_t i = 0; i < (blockDim.x >> 6); i++) t += w[i]; cnt[T.idx] = t; }
+}
+
+// The alive OH of a track in particle order from off[T.idx], each moved by one Brownian
+// step over DNA_T_END_NS - t_birth with draws 32-37 of its own stream (tk, k, k).
+__global__ void __launch_bounds__(256) k_oh_write(const IrtTrack *tracks, const float4 *pos, const uint8_t *alive,
+                                                  const double *tb, const uint32_t *ntot, const uint32_t *off,
+                                                  uint64_t seed, float4 *out)
+{
+    const IrtConst &C = c_irt;
+    const IrtTrack T = tracks[blockIdx.x];
+    const uint8_t *al = alive + T.part_off;
+    const uint32_t nt = ntot[T.idx];
+    const uint64_t tk = dna_track_key(seed, T.inst, T.pid);
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __shared__ uint32_t wc[4];
+    uint32_t base = off[T.idx];
+    for (uint32_t c0 = 0; c0 < nt; c0 += blockDim.x) {
+        const uint32_t k = c0 + threadIdx.x;
+        const bool is_oh = k < nt && al[k] == 1u;
+        const uint64_t m = __ballot(is_oh);
+        if (lane == 0) wc[w] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+        for (uint32_t i = 0; i < nw; i++) { if (i < w) before += wc[i]; tot += wc[i]; }
+        if (is_oh) {
+            const float4 q = pos[T.part_off + k];
+            double x = q.x, y = q.y, z = q.z;
+            const double dt = DNA_T_END_NS - tb[T.part_off + k];
+            if (dt > 0) {
+                const uint64_t ps = dna_pair_state(tk, k, k);
+                const double sig = sqrt(2.0 * C.D[0] * dt);
+                const double g0 = sqrt(-2 * dna_log(dna_u01(ps, 32) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 33));
+                const double g1 = sqrt(-2 * dna_log(dna_u01(ps, 34) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 35));
+                const double g2 = sqrt(-2 * dna_log(dna_u01(ps, 36) + 1e-30)) * dna_cos(6.2831853 * dna_u01(ps, 37));
+                x += g0 * sig; y += g1 * sig; z += g2 * sig;
+            }
+            const uint32_t rank = before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            out[base + rank] = make_float4((float)x, (float)y, (float)z, 0.0f);
+        }
+        base += tot;
+        __syncthreads();
     }
 }
 
@@ -1532,7 +1599,9 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
     const uint32_t ntr = (uint32_t)G.tr.size();
     for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
     for (auto &t : G.tr) outs[t.inst].n_tracks++;
+    c->final_ready = false;
     if (ntr == 0) return DNA_OK;
+    if (c->irt_final) DNA_HIP(dgrow<uint32_t>(*(void **)&c->d_ntot, c->cap_ntot, ntr));
     int ev_mult = 8;
     for (int attempt = 0; attempt < 4; attempt++) {
         uint64_t part = 0, evs = 0;
@@ -1547,6 +1616,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         carve_block(G, (char *)c->irt.tally, tally_bytes, track_tally != nullptr, b);
         rc = irt_prepare(c, G, b); if (rc) return rc;
         IrtArgs A = irt_args(G, d_rad, b, part, evs, t_ckpt, nc, seed);
+        if (c->irt_final) A.final_ntot = c->d_ntot;
         int block = 256;
         rc = irt_pick_block(c, G, A, b, block); if (rc) return rc;
         c->irt_last_block = block;
@@ -1574,6 +1644,10 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
         hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
         irt_outputs(G, ht.data(), n_therm, E, nc, ms, kms, outs);
+        if (c->irt_final) {
+            c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = b.tr;
+            c->final_pos = A.pos; c->final_alive = A.alive; c->final_tb = A.tb;
+        }
         return DNA_OK;
     }
     set_error("dna_run_irt: event/particle capacity kept overflowing");
@@ -1804,4 +1878,42 @@ extern "C" int dna_irt_plan_run(dna_irt_plan *P, uint64_t seed, dna_chem_out *ou
     }
     set_error("dna_irt_plan_run: event/particle capacity kept overflowing");
     return DNA_ERR_CAPACITY;
+}
+
+extern "C" int dna_irt_oh_final(dna_ctx *c, float *out_pos4, uint64_t cap, uint64_t *n_out)
+{
+    if (!c || !n_out) { set_error("dna_irt_oh_final: NULL argument"); return DNA_ERR_ARG; }
+    if (!c->final_ready) {
+        set_error("dna_irt_oh_final: no IRT run with DNA_OPT_IRT_FINAL = 1 since the last IRT call");
+        return DNA_ERR_ARG;
+    }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint32_t ntr = c->final_ntr;
+    // counts per track, exclusive scan (total at [ntr]), then the ordered write
+    DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, 2 * (size_t)ntr + 2 + ((size_t)ntr + 1023) / 1024 + 8));
+    uint32_t *cnt = (uint32_t *)c->irt.part, *tmp = cnt + ntr + 2;
+    const IrtTrack *tr = (const IrtTrack *)c->final_tracks;
+    hipLaunchKernelGGL(k_oh_count, dim3(ntr), dim3(256), 0, st, tr, c->final_alive, c->d_ntot, cnt);
+    DNA_HIP(hipGetLastError());
+    DNA_HIP(exclusive_scan_u32(cnt, ntr, tmp, st));
+    uint32_t total = 0;
+    DNA_HIP(hipMemcpyAsync(&total, cnt + ntr, 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    DNA_HIP(dgrow<float4>(*(void **)&c->d_oh, c->cap_oh, (size_t)total + 1));
+    hipLaunchKernelGGL(k_oh_write, dim3(ntr), dim3(256), 0, st, tr, c->final_pos, c->final_alive, c->final_tb,
+                       c->d_ntot, cnt, c->final_seed, c->d_oh);
+    DNA_HIP(hipGetLastError());
+    c->oh_n = total;
+    *n_out = total;
+    if (out_pos4) {
+        if (cap < total) {
+            DNA_HIP(hipStreamSynchronize(st));
+            set_error("dna_irt_oh_final: output capacity below the OH count (see *n_out)");
+            return DNA_ERR_CAPACITY;
+        }
+        DNA_HIP(hipMemcpyAsync(out_pos4, c->d_oh, (size_t)total * 16, hipMemcpyDeviceToHost, st));
+    }
+    DNA_HIP(hipStreamSynchronize(st));
+    return DNA_OK;
 }

@@ -144,6 +144,10 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
         c->irt_block = value;
         if (value == 0) c->irt_tuned.clear();
         return DNA_OK;
+    case DNA_OPT_IRT_FINAL:
+        if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_IRT_FINAL must be 0 or 1"); return DNA_ERR_ARG; }
+        c->irt_final = value;
+        return DNA_OK;
     default:
         set_error("dna_set_option: unknown option");
         return DNA_ERR_ARG;
@@ -156,6 +160,7 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     switch (option) {
     case DNA_OPT_IRT_BLOCK: *value = c->irt_block; return DNA_OK;
     case DNA_OPT_IRT_BLOCK_LAST: *value = c->irt_last_block; return DNA_OK;
+    case DNA_OPT_IRT_FINAL: *value = c->irt_final; return DNA_OK;
     default:
         set_error("dna_get_option: unknown option");
         return DNA_ERR_ARG;
@@ -171,7 +176,7 @@ extern "C" void dna_ctx_destroy(dna_ctx *c)
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
                    c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
                    c->irt.part, c->irt.rad_in, c->irt.tally, c->sc.hitbits, c->sc.flags, c->sc.rank,
-                   c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf};
+                   c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf, c->d_ntot, c->d_oh};
     for (void *p : dev) if (p) hipFree(p);
     void *host[] = {c->h_ctr, c->h_results, c->h_dose, c->h_rad, c->sc.h_hits};
     for (void *p : host) if (p) hipHostFree(p);

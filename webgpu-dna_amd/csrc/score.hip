@@ -67,17 +67,18 @@ __device__ int classify(const ScoreGeom &G, const float *fy, const float *fz, co
     return 2;
 }
 
-// pass 1: visited/classify; flag[i] = 1 if an in-reach event; hitidx[i] = bit; counts
+// pass 1: visited/classify; flag[i] = 1 if an in-reach event; hitidx[i] = bit; counts.
+// dedupe: the direct scorer's adjacent-duplicate skipping (the indirect scorer visits all).
 __global__ void k_ssb_classify(const float4 *rad, uint64_t n, ScoreGeom G, const float *fy, const float *fz,
                                const float *rbb0, const float *rbb1, uint32_t *flag, uint64_t *hitidx,
-                               unsigned long long *cnt /* [0] candidates, [1] in reach */)
+                               unsigned long long *cnt /* [0] candidates, [1] in reach */, int dedupe)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     unsigned long long cand = 0, reach = 0;
     if (i < n) {
         float4 e = rad[i];
         uint64_t s = i;
-        while (s > 0 && same_xyz(rad[s - 1], rad[s])) s--;
+        if (dedupe) while (s > 0 && same_xyz(rad[s - 1], rad[s])) s--;
         uint32_t f = 0;
         if (((i - s) & 1ull) == 0) {
             uint64_t h = 0;
@@ -102,24 +103,36 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s0, uint32_t m)
     return s;
 }
 
-// pass 2: draws for in-reach events (rank from the exclusive scan of flag)
+// pass 2: draws for in-reach events (rank from the exclusive scan of flag); a hit counts
+// when its bit was clear, in cnt[2] (direct) or per strand in cnt[2], cnt[3] (indirect:
+// per_strand, bits at or above n_bp belong to strand 1)
 __global__ void k_ssb_hits(const uint32_t *flag, const uint32_t *rank, const uint64_t *hitidx, uint64_t n,
-                           uint32_t s0, uint32_t *bits, unsigned long long *cnt /* [2] ssb */)
+                           uint32_t s0, double p, uint64_t n_bp, int per_strand, uint32_t *bits,
+                           unsigned long long *cnt)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    unsigned long long add = 0;
+    unsigned long long add0 = 0, add1 = 0;
     if (i < n && flag[i]) {
         uint32_t s = lcg_jump(s0, rank[i] + 1u);
         double u = (double)s / 4294967296.0;
-        if (u < 0.15) {
+        if (u < p) {
             uint64_t h = hitidx[i];
             uint32_t bit = 1u << (h & 31u);
             uint32_t old = atomicOr(&bits[h >> 5], bit);
-            if (!(old & bit)) add = 1;
+            if (!(old & bit)) { if (per_strand && h >= n_bp) add1 = 1; else add0 = 1; }
         }
     }
-    for (int o = 32; o > 0; o >>= 1) add += __shfl_xor(add, o, 64);
-    if ((threadIdx.x & 63) == 0 && add) atomicAdd(&cnt[2], add);
+    for (int o = 32; o > 0; o >>= 1) { add0 += __shfl_xor(add0, o, 64); add1 += __shfl_xor(add1, o, 64); }
+    if ((threadIdx.x & 63) == 0) {
+        if (add0) atomicAdd(&cnt[2], add0);
+        if (add1) atomicAdd(&cnt[3], add1);
+    }
+}
+
+__global__ void k_or_bits(uint32_t *dst, const uint32_t *src, uint64_t nwords)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < nwords) dst[i] |= src[i];
 }
 
 __device__ __forceinline__ bool getbit(const uint32_t *bits, uint64_t i) { return (bits[i >> 5] >> (i & 31u)) & 1u; }
@@ -234,18 +247,35 @@ static std::mutex g_lcg_mu;                   // contexts on several host thread
 
 using namespace dna;
 
-extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, uint64_t rad_n,
-                         uint32_t *lcg_state, int copy_hits, dna_score_out *out)
+static uint32_t lcg_advance(uint32_t s, uint64_t m)
+{
+    uint32_t a = 1664525u, cc = 1013904223u;
+    for (int k = 0; k < 64 && m; k++) {
+        if (m & 1ull) s = a * s + cc;
+        cc = a * cc + cc; a = a * a; m >>= 1;
+    }
+    return s;
+}
+
+// Scoring with an optional indirect pass first (ind != NULL): the alive-OH list oh
+// (host, or the resident list of dna_irt_oh_final when NULL) scored by scoreIndirectSSB
+// (ssb-dsb.ts:43-121: no duplicate skipping, p = 0.4) into its own bitmap, the LCG then
+// continuing into the direct pass, the two bitmaps OR-combined (combineHits,
+// ssb-dsb.ts:413-418) before clustering (app.ts:176-186).
+static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uint64_t rad_n, const float *oh,
+                      uint64_t n_oh, uint32_t *lcg_state, int copy_hits, dna_score_out *out, dna_indirect_out *ind)
 {
     if (!c || !t || !lcg_state || !out) { set_error("dna_score: NULL argument"); return DNA_ERR_ARG; }
     if (!t->fy || !t->fz || !t->rbb0 || !t->rbb1 || t->n_bp_per == 0 || t->grid_N == 0 ||
         (uint64_t)t->n_fibers * t->n_bp_per != t->n_bp || t->n_fibers != t->grid_N * t->grid_N) {
         set_error("dna_score: inconsistent DNA target"); return DNA_ERR_ARG;
     }
-    if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_score: too many radical entries"); return DNA_ERR_CAPACITY; }
+    if (rad_n >= 0xFFFFFFF0ull || n_oh >= 0xFFFFFFF0ull) { set_error("dna_score: too many entries"); return DNA_ERR_CAPACITY; }
+    if (ind && !oh && n_oh > c->oh_n) { set_error("dna_score_oh: oh == NULL but fewer OH are resident"); return DNA_ERR_ARG; }
     DNA_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     memset(out, 0, sizeof(*out));
+    if (ind) memset(ind, 0, sizeof(*ind));
     std::unique_lock<std::mutex> lk(g_lcg_mu);
     if (!g_lcg_ready[c->device]) {
         uint32_t A[32], Cc[32];
@@ -272,8 +302,10 @@ extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, 
     DNA_HIP(hipMemcpyAsync(d_r0, t->rbb0, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemcpyAsync(d_r1, t->rbb1, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
     const float4 *d_rad;
+    // host inputs share one upload buffer: the radicals, then the OH list
+    const uint64_t n_in = (rad ? rad_n : 0) + ((ind && oh) ? n_oh : 0);
+    if (n_in) DNA_HIP(sgrow<float4>(c->sc.rad_in, c->sc.cap_rad_in, n_in + 1));
     if (rad) {
-        DNA_HIP(sgrow<float4>(c->sc.rad_in, c->sc.cap_rad_in, rad_n + 1));
         DNA_HIP(hipMemcpyAsync(c->sc.rad_in, rad, rad_n * 16, hipMemcpyHostToDevice, st));
         d_rad = (const float4 *)c->sc.rad_in;
     } else {
@@ -281,24 +313,60 @@ extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, 
         d_rad = c->d_rad;
     }
     uint64_t nwords = (2 * (uint64_t)t->n_bp + 31) / 32;
-    DNA_HIP(sgrow<uint32_t>(c->sc.hitbits, c->sc.cap_hitbits, nwords + 64 + 2 * (size_t)t->n_fibers));
+    const uint64_t nw2 = nwords + (nwords & 1);
+    // direct bitmap | counters (8 u64) | fiber counts | indirect bitmap | indirect counters (8 u64)
+    DNA_HIP(sgrow<uint32_t>(c->sc.hitbits, c->sc.cap_hitbits, 2 * nw2 + 64 + 2 * (size_t)t->n_fibers + 2));
     uint32_t *bits = (uint32_t *)c->sc.hitbits;
-    unsigned long long *cnt = (unsigned long long *)(bits + nwords + (nwords & 1));
+    unsigned long long *cnt = (unsigned long long *)(bits + nw2);
     uint32_t *fiber_k = (uint32_t *)(cnt + 8);
-    DNA_HIP(sgrow<uint32_t>(c->sc.flags, c->sc.cap_flags, 3 * (rad_n + 1) + 16));
-    uint32_t *flag = (uint32_t *)c->sc.flags, *rank = flag + rad_n + 1;
-    DNA_HIP(sgrow<uint64_t>(c->sc.rank, c->sc.cap_rank, rad_n + 1 + rad_n / 1024 + 16));
+    uint32_t *ibits = fiber_k + ((2 * (size_t)t->n_fibers + 2 + 1) & ~(size_t)1);
+    unsigned long long *icnt = (unsigned long long *)(ibits + nw2);
+    const uint64_t m = rad_n > n_oh ? rad_n : n_oh;
+    DNA_HIP(sgrow<uint32_t>(c->sc.flags, c->sc.cap_flags, 3 * (m + 1) + 16));
+    uint32_t *flag = (uint32_t *)c->sc.flags, *rank = flag + m + 1;
+    DNA_HIP(sgrow<uint64_t>(c->sc.rank, c->sc.cap_rank, m + 1 + m / 1024 + 16));
     uint64_t *hitidx = (uint64_t *)c->sc.rank;
-    uint32_t *scan_tmp = (uint32_t *)(hitidx + rad_n + 1);
+    uint32_t *scan_tmp = (uint32_t *)(hitidx + m + 1);
     DNA_HIP(hipEventRecord(c->ev[5], st));
-    DNA_HIP(hipMemsetAsync(bits, 0, (nwords + (nwords & 1)) * 4 + 8 * 8, st));
+    DNA_HIP(hipMemsetAsync(bits, 0, nw2 * 4 + 8 * 8, st));
     const uint32_t B = 256;
+    uint32_t s_dir = *lcg_state;
+    uint64_t ind_reach = 0;
+    if (ind) {
+        DNA_HIP(hipMemsetAsync(ibits, 0, nw2 * 4 + 8 * 8, st));
+        const float4 *d_oh = c->d_oh;
+        if (oh && n_oh) {
+            d_oh = (const float4 *)c->sc.rad_in + (rad ? rad_n : 0);
+            DNA_HIP(hipMemcpyAsync((void *)d_oh, oh, n_oh * 16, hipMemcpyHostToDevice, st));
+        }
+        if (n_oh) {
+            uint32_t gi = (uint32_t)((n_oh + B - 1) / B);
+            hipLaunchKernelGGL(k_ssb_classify, dim3(gi), dim3(B), 0, st, d_oh, n_oh, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx,
+                               icnt, 0);
+            DNA_HIP(hipMemcpyAsync(rank, flag, n_oh * 4, hipMemcpyDeviceToDevice, st));
+            DNA_HIP(exclusive_scan_u32(rank, (uint32_t)n_oh, scan_tmp, st));
+            hipLaunchKernelGGL(k_ssb_hits, dim3(gi), dim3(B), 0, st, flag, rank, hitidx, n_oh, *lcg_state, 0.4,
+                               (uint64_t)t->n_bp, 1, ibits, icnt);
+        }
+        unsigned long long hi[4] = {0, 0, 0, 0};
+        DNA_HIP(hipMemcpyAsync(hi, icnt, sizeof(hi), hipMemcpyDeviceToHost, st));
+        DNA_HIP(hipStreamSynchronize(st));
+        ind->candidates = hi[0]; ind->in_reach = hi[1]; ind->ssb0 = hi[2]; ind->ssb1 = hi[3];
+        ind_reach = hi[1];
+        s_dir = lcg_advance(*lcg_state, ind_reach);     // the direct pass continues the sequence
+    }
     uint32_t gb = (uint32_t)((rad_n + B - 1) / B);
     if (rad_n) {
-        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, rad_n, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx, cnt);
+        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, rad_n, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx,
+                           cnt, 1);
         DNA_HIP(hipMemcpyAsync(rank, flag, rad_n * 4, hipMemcpyDeviceToDevice, st));
         DNA_HIP(exclusive_scan_u32(rank, (uint32_t)rad_n, scan_tmp, st));
-        hipLaunchKernelGGL(k_ssb_hits, dim3(gb), dim3(B), 0, st, flag, rank, hitidx, rad_n, *lcg_state, bits, cnt);
+        hipLaunchKernelGGL(k_ssb_hits, dim3(gb), dim3(B), 0, st, flag, rank, hitidx, rad_n, s_dir, 0.15,
+                           (uint64_t)t->n_bp, 0, bits, cnt);
+    }
+    if (ind) {
+        hipLaunchKernelGGL(k_or_bits, dim3((uint32_t)((nwords + B - 1) / B)), dim3(B), 0, st, bits, ibits, nwords);
+        DNA_HIP(hipGetLastError());
     }
     uint32_t *ovf = fiber_k + 2 * t->n_fibers;
     DNA_HIP(hipMemsetAsync(ovf, 0, 4, st));
@@ -335,18 +403,25 @@ extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, 
         if (k0 > 0 && k1 > 0) ex += 1 - std::pow(1 - p_pair, (double)(k0 * k1));
     }
     out->dsb_expected = ex;
-    // advance the caller's LCG by the draws consumed
-    uint32_t s = *lcg_state;
-    uint64_t m = h_cnt[1];
-    uint32_t a = 1664525u, cc = 1013904223u;
-    for (int k = 0; k < 64 && m; k++) {
-        if (m & 1ull) s = a * s + cc;
-        cc = a * cc + cc; a = a * a; m >>= 1;
-    }
-    *lcg_state = s;
+    // advance the caller's LCG by the draws consumed (indirect, then direct)
+    *lcg_state = lcg_advance(s_dir, h_cnt[1]);
     float ms = 0; hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
     out->t_ms = ms;
     return DNA_OK;
+}
+
+extern "C" int dna_score(dna_ctx *c, const dna_dna_target *t, const float *rad, uint64_t rad_n,
+                         uint32_t *lcg_state, int copy_hits, dna_score_out *out)
+{
+    return score_impl(c, t, rad, rad_n, nullptr, 0, lcg_state, copy_hits, out, nullptr);
+}
+
+extern "C" int dna_score_oh(dna_ctx *c, const dna_dna_target *t, const float *rad, uint64_t rad_n, const float *oh,
+                            uint64_t n_oh, uint32_t *lcg_state, int copy_hits, dna_score_out *out,
+                            dna_indirect_out *ind)
+{
+    if (!ind) { set_error("dna_score_oh: NULL argument"); return DNA_ERR_ARG; }
+    return score_impl(c, t, rad, rad_n, oh, n_oh, lcg_state, copy_hits, out, ind);
 }
 
 extern "C" int dna_hash_cells(dna_ctx *c, const float *pos4, uint64_t n, float cell_nm, uint32_t bits,

@@ -7,7 +7,7 @@ import os
 
 import numpy as np
 
-from .abi import (DNA_COPY_ALL, DnaChemOut, DnaParams, DnaResult, DnaScoreOut, DnaTarget,
+from .abi import (DNA_COPY_ALL, DnaChemOut, DnaIndirectOut, DnaParams, DnaResult, DnaScoreOut, DnaTarget,
                   DnaTransportOut)
 
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -49,6 +49,9 @@ def lib():
     L.dna_set_option.argtypes = [vp, C.c_int, C.c_int]
     L.dna_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.dna_xs_probe.argtypes = [vp, C.c_int, vp, vp, u64, vp]
+    L.dna_irt_oh_final.argtypes = [vp, vp, u64, C.POINTER(u64)]
+    L.dna_score_oh.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, vp, u64, C.POINTER(u32), C.c_int,
+                               C.POINTER(DnaScoreOut), C.POINTER(DnaIndirectOut)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
                                      C.POINTER(DnaChemOut)]
     ip = C.POINTER(C.c_int)
@@ -76,7 +79,7 @@ EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
-                    "dna_surrogate_predict", "dna_xs_probe"]
+                    "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh"]
 
 
 def _check(fn, rc):
@@ -214,6 +217,40 @@ class Context:
         d = dict(ssb_count=int(o.ssb_count), candidates=int(o.candidates),
                  in_reach=int(o.in_reach), dsb=int(o.dsb), ssb0=int(o.ssb0), ssb1=int(o.ssb1),
                  dsb_expected=o.dsb_expected, lcg_state=st.value, t_ms=o.t_ms)
+        if want_hits and o.hits:
+            d["hits"] = np.ctypeslib.as_array(C.cast(o.hits, C.POINTER(C.c_uint8)),
+                                              (2 * target.n_bp,)).copy()
+        return d
+
+    def irt_oh_final(self, copy=True):
+        """The OH alive at 1 us of the last irt() made with DNA_OPT_IRT_FINAL = 1:
+        [n, 4] float32 in track order (copy=True), or just n (copy=False; the list stays
+        on the device for score_oh(oh=None))."""
+        n = C.c_uint64()
+        _check("dna_irt_oh_final", lib().dna_irt_oh_final(self.h, None, 0, C.byref(n)))
+        if not copy:
+            return n.value
+        out = np.empty((max(n.value, 1), 4), np.float32)
+        _check("dna_irt_oh_final", lib().dna_irt_oh_final(self.h, _ptr(out), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def score_oh(self, target, rad=None, rad_n=0, oh=None, n_oh=0, lcg_state=0x12345678, want_hits=False):
+        """score() with the indirect-SSB pass on the alive-OH list first (oh=None: the
+        list resident from irt_oh_final)."""
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        q = None if oh is None else np.ascontiguousarray(oh, np.float32).reshape(-1, 4)
+        t = target.to_c()
+        st = C.c_uint32(lcg_state)
+        o, ind = DnaScoreOut(), DnaIndirectOut()
+        n = rad_n if r is None else r.shape[0]
+        m = n_oh if q is None else q.shape[0]
+        _check("dna_score_oh", lib().dna_score_oh(self.h, C.byref(t), _ptr(r), n, _ptr(q), m, C.byref(st),
+                                                  1 if want_hits else 0, C.byref(o), C.byref(ind)))
+        d = dict(ssb_count=int(o.ssb_count), candidates=int(o.candidates),
+                 in_reach=int(o.in_reach), dsb=int(o.dsb), ssb0=int(o.ssb0), ssb1=int(o.ssb1),
+                 dsb_expected=o.dsb_expected, lcg_state=st.value, t_ms=o.t_ms,
+                 indirect=dict(ssb0=int(ind.ssb0), ssb1=int(ind.ssb1), candidates=int(ind.candidates),
+                               in_reach=int(ind.in_reach)))
         if want_hits and o.hits:
             d["hits"] = np.ctypeslib.as_array(C.cast(o.hits, C.POINTER(C.c_uint8)),
                                               (2 * target.n_bp,)).copy()
