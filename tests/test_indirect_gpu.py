@@ -77,3 +77,17 @@ def test_score_oh_exact(ctx):
     d = ctx.score(dna, None, t["rad_n"], lcg_state=0x12345678)
     od = O.score(dna, t["rad"], lcg_state=0x12345678)
     assert (d["ssb_count"], d["dsb"], d["lcg_state"]) == (od["ssb_count"], od["dsb"], od["lcg_state"])
+
+
+def test_run_validation_with_indirect_ssb():
+    """The host mirror's 10 keV row with the extension on: the indirect pass runs on the
+    OH kept by the chemistry and its hits join the direct ones."""
+    from dna_amd import host
+    print("runValidation at 10 keV with indirect SSB", flush=True)
+    out = host.run_validation({"np": 2048, "boxNm": 15000.0, "ceEV": 7.4, "energies": [10000],
+                               "indirectSSB": True})
+    dmg = out["rows"][0]["damage"]
+    base = host.run_validation({"np": 2048, "boxNm": 15000.0, "ceEV": 7.4, "energies": [10000]})
+    d0 = base["rows"][0]["damage"]
+    print("with indirect:", dmg, "direct only:", d0, flush=True)
+    assert d0["ssb_ind"] == 0 and dmg["ssb_ind"] > 0

@@ -118,16 +118,28 @@ def _chem_result(o, t_wall_ms):
             "chem_alive_final": None, "n_reacted": d["n_reacted"], "rxn_info": rxn_info}
 
 
-def run_chemistry_worker(rad_buf_f32, rad_n, n_therm, E_eV, log=None, max_n=None, seed=1):
+def run_chemistry_worker(rad_buf_f32, rad_n, n_therm, E_eV, log=None, max_n=None, seed=1, oh_final=False):
     """runChemistryWorker (src/chemistry/worker.ts:32-87): the 9-reaction IRT to 1 us on
-    the GPU.  rad_buf_f32 is a (rad_n, 4) float32 array or a flat Float32Array-like."""
+    the GPU.  rad_buf_f32 is a (rad_n, 4) float32 array or a flat Float32Array-like.
+    oh_final: also keep the OH alive at 1 us on the device for the indirect-SSB scoring
+    (res["chem_oh_n"]); the worker itself returns no positions (worker.ts:69-70)."""
+    from .abi import DNA_OPT_IRT_FINAL
     cc = ensure_pipelines(_cache["np"] if _cache else 0)
     if cc is None:
         raise RuntimeError("IRT worker error: no GPU device")
     rad = np.ascontiguousarray(rad_buf_f32, np.float32).reshape(-1, 4)[:rad_n]
     t0 = time.perf_counter()
-    o = cc["ctx"].irt(rad, rad_n, n_therm, float(E_eV), list(CHECKPOINTS_NS), seed=seed)
+    ctx = cc["ctx"]
+    if oh_final:
+        ctx.set_option(DNA_OPT_IRT_FINAL, 1)
+    try:
+        o = ctx.irt(rad, rad_n, n_therm, float(E_eV), list(CHECKPOINTS_NS), seed=seed)
+    finally:
+        if oh_final:
+            ctx.set_option(DNA_OPT_IRT_FINAL, 0)
     res = _chem_result(o, (time.perf_counter() - t0) * 1e3)
+    if oh_final:
+        res["chem_oh_n"] = ctx.irt_oh_final(copy=False)
     if log:
         for cp in res["timeline"]:
             log(f"  {cp['label']} {cp['G_OH']:.3f} {cp['G_eaq']:.3f} {cp['G_H']:.3f} "
@@ -145,17 +157,25 @@ def make_ssb_rng(seed=0x12345678):
 def score_damage_at_10kev(cc, dna, chem, rad_buf, rad_n, total_deposited_eV, box_nm, rng, log=_noop,
                           kernel_dna_hits=0):
     """scoreDamageAt10keV (src/app.ts:167-207): direct-event SSB + combineHits + clusterDSB
-    on the GPU (indirect hits are empty on the worker path, app.ts:178-180)."""
-    s = cc["ctx"].score(dna, rad_buf, rad_n, lcg_state=rng["state"])
+    on the GPU.  Indirect hits are empty on the worker path (app.ts:178-180) unless the
+    chemistry kept its OH at 1 us (run_chemistry_worker(oh_final=True)): then
+    scoreIndirectSSB runs first on those, as on the reference's GPU-chemistry path."""
+    oh_n = chem.get("chem_oh_n") if chem else None
+    if oh_n is not None:
+        s = cc["ctx"].score_oh(dna, rad_buf, rad_n, None, oh_n, lcg_state=rng["state"])
+        ind = s["indirect"]
+    else:
+        s = cc["ctx"].score(dna, rad_buf, rad_n, lcg_state=rng["state"])
+        ind = {"ssb0": 0, "ssb1": 0, "in_reach": 0}
     rng["state"] = s["lcg_state"]
-    ssb_dir, ssb_ind, dsb = s["ssb_count"], 0, s["dsb"]
+    ssb_dir, ssb_ind, dsb = s["ssb_count"], ind["ssb0"] + ind["ssb1"], s["dsb"]
     dose_box_Gy = (total_deposited_eV * 1.602e-19) / ((2 * box_nm * 1e-9) ** 3 * 1000)
     target_Gbp = dna.n_bp * 1e-9
     dsb_per_gy_gbp = dsb / (dose_box_Gy * target_Gbp) if dose_box_Gy > 0 else 0.0
     ssb_tot = ssb_dir + ssb_ind
     log(f"    DAMAGE: SSB_dir={ssb_dir}  SSB_ind={ssb_ind}  DSB={dsb}  "
         f"(DSB/SSB={(dsb / ssb_tot if ssb_tot else 0):.3f})  box_dose={dose_box_Gy:.3f} Gy  "
-        f"target={dna.n_bp / 1e6:.1f} Mbp  reach_dir={s['in_reach']} reach_ind=0  "
+        f"target={dna.n_bp / 1e6:.1f} Mbp  reach_dir={s['in_reach']} reach_ind={ind['in_reach']}  "
         f"kernel_hits={kernel_dna_hits}", "data")
     return {"ssb_dir": ssb_dir, "ssb_ind": ssb_ind, "dsb": dsb, "dsb_per_gy_gbp": dsb_per_gy_gbp,
             "dsb_expected": s["dsb_expected"], "in_reach": s["in_reach"]}
@@ -176,9 +196,10 @@ def run_validation(cfg):
     dna = DNATarget(DNA_LENGTH_NM, DNA_GRID_N, DNA_SPACING_NM)
     ssb_rng = make_ssb_rng()
     chem_cb = None
+    indirect = bool(cfg.get("indirectSSB", False))       # extension: score the OH at 1 us too
     if backend != "none":
         def chem_cb(rad, rad_n, n_therm, E):
-            return run_chemistry_worker(rad, rad_n, n_therm, E, log)
+            return run_chemistry_worker(rad, rad_n, n_therm, E, log, oh_final=indirect and E == 10000)
     rows, out = [], None
     for E in energies:
         ref = next((e for e in ESTAR if e["E"] == E), {"E": E, "csda": float("nan")})
