@@ -224,7 +224,8 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
 #define DNA_OPT_IRT_BLOCK       1
 #define DNA_OPT_IRT_BLOCK_LAST  2
 /* DNA_OPT_IRT_FINAL: 1 = dna_run_irt keeps each track's final state on the device for
- * dna_irt_oh_final (default 0). */
+ * dna_irt_oh_final and dna_irt_particles, and records each particle's death time
+ * (default 0). */
 #define DNA_OPT_IRT_FINAL       3
 int dna_set_option(struct dna_ctx *ctx, int option, int value);
 int dna_get_option(struct dna_ctx *ctx, int option, int *value);
@@ -238,6 +239,15 @@ int dna_get_option(struct dna_ctx *ctx, int option, int *value);
  * on the device (oh == NULL in dna_score_oh); out_pos4 (host, may be NULL) receives it when
  * cap >= *n_out, else DNA_ERR_CAPACITY with *n_out set. */
 int dna_irt_oh_final(struct dna_ctx *ctx, float *out_pos4, uint64_t cap, uint64_t *n_out);
+
+/* Every particle slot of every track at the end of the last dna_run_irt made with
+ * DNA_OPT_IRT_FINAL = 1, in track order: out6[n][6] doubles (x, y, z, pid * 8 + rad_buf
+ * code (OH- = 6), t_birth, t_death = +inf if it never reacted).  A particle is alive at a
+ * checkpoint t iff t_birth < t <= t_death, which reproduces the checkpoint tallies.  This
+ * feeds the reference's 4D snapshot export (src/splat/export.ts:32-112, which takes
+ * ChemSnapshot {label, t_ns, pos, alive}, src/physics/types.ts:57-63) from IRT runs.
+ * out6 == NULL returns the count only; cap < *n_out gives DNA_ERR_CAPACITY. */
+int dna_irt_particles(struct dna_ctx *ctx, double *out6, uint64_t cap, uint64_t *n_out);
 
 /* Captured batches: the instances of dna_run_irt_batch (rad == NULL: entries
  * [inst_off[0], inst_off[n_inst]) of the radicals resident from the last transport),

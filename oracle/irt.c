@@ -201,7 +201,7 @@ static ev hpop(heap *h)
 
 /* ------------------------------------------------------------ one track */
 typedef struct {
-    float *x, *y, *z; int8_t *sp; uint8_t *alive; double *tb; int64_t cap;
+    float *x, *y, *z; int8_t *sp; uint8_t *alive; double *tb, *td; int64_t cap;
     heap h;
 } work;
 
@@ -211,6 +211,7 @@ static void work_reserve(work *w, int64_t cap)
     w->x = (float *)realloc(w->x, cap * 4); w->y = (float *)realloc(w->y, cap * 4);
     w->z = (float *)realloc(w->z, cap * 4); w->sp = (int8_t *)realloc(w->sp, cap);
     w->alive = (uint8_t *)realloc(w->alive, cap); w->tb = (double *)realloc(w->tb, cap * 8);
+    w->td = (double *)realloc(w->td, cap * 8);
     w->cap = cap;
 }
 
@@ -275,7 +276,7 @@ static uint32_t run_track(work *w, const float *rad, const uint64_t *idx, uint32
         const float *e = rad + 4 * idx[k];
         double x = e[0], y = e[1], z = e[2];
         int s = (int)fmod(floor((double)e[3] + 0.5), 8.0);
-        w->alive[k] = 1; w->tb[k] = 0.0;
+        w->alive[k] = 1; w->tb[k] = 0.0; w->td[k] = INFINITY;
         uint64_t ps = dna_pair_state(tk, k, k);
         uint32_t dn = 0;
         double sig = 0.0;
@@ -326,6 +327,7 @@ static uint32_t run_track(work *w, const float *rad, const uint64_t *idx, uint32
         int ri = rxnMap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
         if (ri < 0) continue;
         w->alive[e.a] = 0; w->alive[e.b] = 0;
+        w->td[e.a] = e.t; w->td[e.b] = e.t;
         (*n_reacted)++; rxn_counts[ri]++;
         if (ri == 3 || ri == 4 || ri == 6) h2++;
         int np = RXN[ri].np;
@@ -341,7 +343,7 @@ static uint32_t run_track(work *w, const float *rad, const uint64_t *idx, uint32
             work_reserve(w, id + 1);
             w->sp[id] = (int8_t)RXN[ri].prods[pp];
             w->alive[id] = 1;
-            w->tb[id] = e.t;
+            w->tb[id] = e.t; w->td[id] = INFINITY;
             if (np == 1) { w->x[id] = (float)mx; w->y[id] = (float)my; w->z[id] = (float)mz; }
             else if (pp == 0) { w->x[id] = w->x[e.a]; w->y[id] = w->y[e.a]; w->z[id] = w->z[e.a]; }
             else { w->x[id] = w->x[e.b]; w->y[id] = w->y[e.b]; w->z[id] = w->z[e.b]; }
@@ -382,6 +384,23 @@ static void collect_oh(const work *w, uint32_t n_total, uint64_t tk, float **buf
     }
 }
 
+/* Every particle slot of a track at the end: (x, y, z, pid * 8 + rad_buf code) with the
+ * species mapped back to the rad_buf codes (OH- is 6), birth and death time (infinite if it
+ * never reacted).  6 doubles per slot: x, y, z, w, t_birth, t_death. */
+static void collect_particles(const work *w, uint32_t n_total, uint64_t pid, double **buf, uint64_t *n)
+{
+    static const int code[6] = {0, 1, 2, 3, 4, 6};
+    *buf = (double *)malloc(6 * 8 * (n_total ? n_total : 1));
+    *n = n_total;
+    for (uint32_t k = 0; k < n_total; k++) {
+        double *q = *buf + 6 * k;
+        int s = w->sp[k];
+        q[0] = w->x[k]; q[1] = w->y[k]; q[2] = w->z[k];
+        q[3] = (double)(pid * 8 + (uint64_t)(s >= 0 && s < 6 ? code[s] : s));
+        q[4] = w->tb[k]; q[5] = w->td[k];
+    }
+}
+
 typedef struct { int64_t pid; uint64_t first, n, h2; } trk;
 
 static int cmp_u64pair(const void *a, const void *b)
@@ -394,7 +413,7 @@ static int cmp_u64pair(const void *a, const void *b)
 /* One worker invocation (irt-worker.js:370-699) over rad[0..rad_n); instance keys the RNG. */
 static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
                         const double *tc, int nc, uint64_t seed, uint64_t instance, int n_threads,
-                        dna_chem_out *out, float **oh_out, uint64_t *oh_n)
+                        dna_chem_out *out, float **oh_out, uint64_t *oh_n, double **pt_out, uint64_t *pt_n)
 {
     init_params();
     memset(out, 0, sizeof(*out));
@@ -443,6 +462,8 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
     uint64_t *nre = calloc((size_t)n_threads, 8), *nev = calloc((size_t)n_threads, 8);
     float **oh_tr = oh_out ? (float **)calloc(ntr ? ntr : 1, sizeof(float *)) : NULL;
     uint64_t *oh_tn = oh_out ? (uint64_t *)calloc(ntr ? ntr : 1, 8) : NULL;
+    double **pt_tr = pt_out ? (double **)calloc(ntr ? ntr : 1, sizeof(double *)) : NULL;
+    uint64_t *pt_tn = pt_out ? (uint64_t *)calloc(ntr ? ntr : 1, 8) : NULL;
 #ifdef _OPENMP
 #pragma omp parallel num_threads(n_threads)
 #endif
@@ -460,8 +481,9 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
             uint32_t nt = run_track(&w, rad, idx + tr[t].first, (uint32_t)tr[t].n, tk, tr[t].h2, CAP, tc, nc,
                                     tally[tid], rxc[tid], &nre[tid], &nev[tid]);
             if (oh_tr) collect_oh(&w, nt, tk, &oh_tr[t], &oh_tn[t]);
+            if (pt_tr) collect_particles(&w, nt, (uint64_t)tr[t].pid, &pt_tr[t], &pt_tn[t]);
         }
-        free(w.x); free(w.y); free(w.z); free(w.sp); free(w.alive); free(w.tb); free(w.h.v);
+        free(w.x); free(w.y); free(w.z); free(w.sp); free(w.alive); free(w.tb); free(w.td); free(w.h.v);
     }
     for (int th = 0; th < n_threads; th++) {
         for (int c = 0; c < nc; c++) for (int s = 0; s < 5; s++) out->tally[c][s] += tally[th][c][s];
@@ -484,6 +506,19 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
         *oh_n = tot;
         free(oh_tr); free(oh_tn);
     }
+    if (pt_out) {
+        uint64_t tot = 0;
+        for (uint64_t t = 0; t < ntr; t++) tot += pt_tn[t];
+        *pt_out = (double *)malloc(6 * 8 * (tot ? tot : 1));
+        uint64_t o = 0;
+        for (uint64_t t = 0; t < ntr; t++) {
+            memcpy(*pt_out + 6 * o, pt_tr[t], 6 * 8 * pt_tn[t]);
+            o += pt_tn[t];
+            free(pt_tr[t]);
+        }
+        *pt_n = tot;
+        free(pt_tr); free(pt_tn);
+    }
     free(tally); free(rxc); free(nre); free(nev); free(tr); free(idx); free(pairs);
     return 0;
 }
@@ -492,7 +527,7 @@ int oracle_irt(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
                const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out)
 {
     if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
-    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, NULL, NULL);
+    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, NULL, NULL, NULL, NULL);
 }
 
 /* oracle_irt, then the OH alive at 1 us (collect_oh) of every track in track order;
@@ -502,7 +537,17 @@ int oracle_irt_oh(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV
                   float **oh_out, uint64_t *oh_n)
 {
     if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
-    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, oh_out, oh_n);
+    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, oh_out, oh_n, NULL, NULL);
+}
+
+/* oracle_irt plus every particle slot of every track at the end, in track order:
+ * [n][6] doubles (x, y, z, pid * 8 + code, t_birth, t_death); *pt_out is malloc'ed */
+int oracle_irt_particles(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                         const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out,
+                         double **pt_out, uint64_t *pt_n)
+{
+    if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
+    return irt_instance(rad, rad_n, n_therm, E_eV, t_ckpt, n_ckpt, seed, 0, n_threads, out, NULL, NULL, pt_out, pt_n);
 }
 
 int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, const uint32_t *n_therm,
@@ -512,7 +557,7 @@ int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, con
     if (n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) return 1;
     for (uint32_t i = 0; i < n_inst; i++) {
         int rc = irt_instance(rad + 4 * off[i], off[i + 1] - off[i], n_therm[i], E[i], t_ckpt, n_ckpt,
-                              seed, i, n_threads, &out[i], NULL, NULL);
+                              seed, i, n_threads, &out[i], NULL, NULL, NULL, NULL);
         if (rc) return rc;
     }
     return 0;

@@ -72,6 +72,11 @@ int oracle_irt_batch(const float *rad, const uint64_t *off, uint32_t n_inst, con
 int oracle_irt_oh(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
                   const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out,
                   float **oh_out, uint64_t *oh_n);
+/* oracle_irt plus every particle slot of every track at the end (track order): [n][6]
+ * doubles (x, y, z, pid * 8 + code, t_birth, t_death = inf if it never reacted) */
+int oracle_irt_particles(const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                         const double *t_ckpt, int n_ckpt, uint64_t seed, int n_threads, dna_chem_out *out,
+                         double **pt_out, uint64_t *pt_n);
 /* reaction parameter table (irt-worker.js:240-279): 9 x {sigma, sigma_eff, rc, kdif, kact, prob} */
 void oracle_irt_params(double *out54);
 /* one IRT reaction-time draw for reaction ri at distance r0 with the RNG stream of

@@ -174,6 +174,27 @@ def irt_oh(rad, n_therm, E_eV, t_ckpt, seed=1, n_threads=0):
     return o, oh
 
 
+def irt_particles(rad, n_therm, E_eV, t_ckpt, seed=1, n_threads=0):
+    """oracle_irt plus every particle slot at the end: [n, 6] float64 (x, y, z,
+    pid*8+code, t_birth, t_death)."""
+    rad = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+    tc = np.ascontiguousarray(t_ckpt, np.float64)
+    o = DnaChemOut()
+    p, n = C.c_void_p(), C.c_uint64()
+    L = lib()
+    L.oracle_irt_particles.argtypes = [C.c_void_p, C.c_uint64, C.c_uint32, C.c_float, C.c_void_p, C.c_int,
+                                       C.c_uint64, C.c_int, C.POINTER(DnaChemOut), C.POINTER(C.c_void_p),
+                                       C.POINTER(C.c_uint64)]
+    rc = L.oracle_irt_particles(_ptr(rad), rad.shape[0], n_therm, E_eV, _ptr(tc), tc.size, seed,
+                                n_threads, C.byref(o), C.byref(p), C.byref(n))
+    if rc:
+        raise RuntimeError(f"oracle_irt_particles rc={rc}")
+    a = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(max(n.value, 1) * 6,))[:n.value * 6]
+    a = a.reshape(-1, 6).copy()
+    L.oracle_free(p)
+    return o, a
+
+
 def irt_batch(rad, inst_off, n_therm, E, t_ckpt, seed=1, n_threads=0):
     rad = np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
     off = np.ascontiguousarray(inst_off, np.uint64)

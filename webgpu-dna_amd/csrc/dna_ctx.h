@@ -72,6 +72,7 @@ struct dna_ctx {
     const float4 *final_pos = nullptr;
     const uint8_t *final_alive = nullptr;
     const double *final_tb = nullptr;
+    const double *final_td = nullptr;
     uint32_t *d_ntot = nullptr;        size_t cap_ntot = 0;    // particles per track at the end
     float4 *d_oh = nullptr;            size_t cap_oh = 0;      // alive OH at 1 us
     uint64_t oh_n = 0;

@@ -67,6 +67,7 @@ struct IrtArgs {
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
     uint32_t *final_ntot;         // optional: particles per track at the end (track order); the
                                   // final alive bytes are then left in alive[]
+    double *tdeath;               // optional (with final_ntot): reaction time per particle slot
     uint32_t *overflow;
     double tc[DNA_MAX_CKPT];
     int nc;
@@ -848,7 +849,7 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
     }
 }
 
-template <bool SORT>
+template <bool SORT, bool DEATH>
 __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
 {
     const IrtConst &C = c_irt;
@@ -926,6 +927,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         }
         P[k] = make_float4((float)x, (float)y, (float)z, (float)spc);
         al[k] = alv ? (uint8_t)(spc + 1) : (uint8_t)0;
+        if (DEATH) A.tdeath[T.part_off + k] = __longlong_as_double(0x7FF0000000000000ll);   // +inf
         tb[k] = 0.0;
         if (alv) { if (spc == 0) c0++; else if (spc == 1) c1++; else if (spc == 2) c2++; }
     }
@@ -1067,6 +1069,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
                 int si = (int)al[a] - 1, sj = (int)al[b] - 1;
                 int ri = sh.rmap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
                 al[a] = 0; al[b] = 0;
+                if (DEATH) { A.tdeath[T.part_off + a] = t0; A.tdeath[T.part_off + b] = t0; }
                 sh.ldead[si]++; sh.ldead[sj]++;
                 sh.n_re++; sh.rxc[ri]++;
                 if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
@@ -1092,6 +1095,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
                         else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
                         else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
                         P[id] = q; al[id] = (uint8_t)(ps + 1); tb[id] = t0;
+                        if (DEATH) A.tdeath[T.part_off + id] = __longlong_as_double(0x7FF0000000000000ll);
                         A.lst[6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++] = id;
                         if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
                         sh.prod_idx[made] = id; sh.prod_sp[made] = (uint32_t)ps; sh.prod_pos[made] = q;
@@ -1228,6 +1232,25 @@ __global__ void __launch_bounds__(256) k_oh_write(const IrtTrack *tracks, const 
         }
         base += tot;
         __syncthreads();
+    }
+}
+
+// Every particle slot of a track at the end, from off[T.idx]: (x, y, z, pid * 8 + rad_buf
+// code, t_birth, t_death) as doubles; species index 5 (OH-) is rad_buf code 6.
+__global__ void __launch_bounds__(256) k_particles_write(const IrtTrack *tracks, const float4 *pos, const double *tb,
+                                                         const double *td, const uint32_t *ntot, const uint32_t *off,
+                                                         double *out)
+{
+    const IrtTrack T = tracks[blockIdx.x];
+    const uint32_t nt = ntot[T.idx];
+    double *o = out + 6 * (size_t)off[T.idx];
+    for (uint32_t k = threadIdx.x; k < nt; k += blockDim.x) {
+        const float4 q = pos[T.part_off + k];
+        const int s = (int)q.w;
+        const int code = s == 5 ? 6 : s;
+        o[6 * k] = q.x; o[6 * k + 1] = q.y; o[6 * k + 2] = q.z;
+        o[6 * k + 3] = (double)((uint64_t)T.pid * 8u + (uint64_t)code);
+        o[6 * k + 4] = tb[T.part_off + k]; o[6 * k + 5] = td[T.part_off + k];
     }
 }
 
@@ -1453,7 +1476,7 @@ struct IrtBufs {
     size_t tally_bytes = 0;
 };
 
-static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 6 + 1) + 256, rad_n / 1024 * 4 + 64); }
+static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 6 + 1 + 8) + 256, rad_n / 1024 * 4 + 64); }
 static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
 {
     return tally_bytes + 256 + G.tr.size() * sizeof(IrtTrack) + (G.K + 1) * 4 + (size_t)G.run * 4 + tt_bytes + 1536;
@@ -1495,6 +1518,7 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     A.rad = d_rad; A.idx = b.idx; A.tracks = b.tr;
     A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
     A.lst = (uint32_t *)(sp + part * 40); A.alive = (uint8_t *)(sp + part * 64);
+    A.tdeath = nullptr;                         // set with final_ntot, in the space behind alive[]
     A.ev_t = (double *)b.events; A.ev_ab = (uint2 *)(b.events + evs * 8);
     A.pend = (uint4 *)(b.events + evs * 16);
     A.tally = b.tally; A.track_tally = b.tt; A.overflow = b.ovf;
@@ -1509,8 +1533,12 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
 static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
 {
     const size_t shmem = (size_t)A.lds_slots * 16;
-    if (G.contiguous) hipLaunchKernelGGL(k_irt<false>, dim3(grid), dim3(block), shmem, st, A);
-    else hipLaunchKernelGGL(k_irt<true>, dim3(grid), dim3(block), shmem, st, A);
+    // the death-time stores are compiled only into the variants that keep the final state
+    if (A.tdeath) {
+        if (G.contiguous) hipLaunchKernelGGL((k_irt<false, true>), dim3(grid), dim3(block), shmem, st, A);
+        else hipLaunchKernelGGL((k_irt<true, true>), dim3(grid), dim3(block), shmem, st, A);
+    } else if (G.contiguous) hipLaunchKernelGGL((k_irt<false, false>), dim3(grid), dim3(block), shmem, st, A);
+    else hipLaunchKernelGGL((k_irt<true, false>), dim3(grid), dim3(block), shmem, st, A);
 }
 
 // Workgroup size: the option, or autotuned per track-size class (log2 of the mean
@@ -1616,7 +1644,10 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         carve_block(G, (char *)c->irt.tally, tally_bytes, track_tally != nullptr, b);
         rc = irt_prepare(c, G, b); if (rc) return rc;
         IrtArgs A = irt_args(G, d_rad, b, part, evs, t_ckpt, nc, seed);
-        if (c->irt_final) A.final_ntot = c->d_ntot;
+        if (c->irt_final) {
+            A.final_ntot = c->d_ntot;
+            A.tdeath = (double *)((char *)c->irt.scratch + ((part * 65 + 7) & ~(uint64_t)7));
+        }
         int block = 256;
         rc = irt_pick_block(c, G, A, b, block); if (rc) return rc;
         c->irt_last_block = block;
@@ -1646,7 +1677,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         irt_outputs(G, ht.data(), n_therm, E, nc, ms, kms, outs);
         if (c->irt_final) {
             c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = b.tr;
-            c->final_pos = A.pos; c->final_alive = A.alive; c->final_tb = A.tb;
+            c->final_pos = A.pos; c->final_alive = A.alive; c->final_tb = A.tb; c->final_td = A.tdeath;
         }
         return DNA_OK;
     }
@@ -1915,5 +1946,37 @@ extern "C" int dna_irt_oh_final(dna_ctx *c, float *out_pos4, uint64_t cap, uint6
         DNA_HIP(hipMemcpyAsync(out_pos4, c->d_oh, (size_t)total * 16, hipMemcpyDeviceToHost, st));
     }
     DNA_HIP(hipStreamSynchronize(st));
+    return DNA_OK;
+}
+
+extern "C" int dna_irt_particles(dna_ctx *c, double *out6, uint64_t cap, uint64_t *n_out)
+{
+    if (!c || !n_out) { set_error("dna_irt_particles: NULL argument"); return DNA_ERR_ARG; }
+    if (!c->final_ready || !c->final_td) {
+        set_error("dna_irt_particles: no IRT run with DNA_OPT_IRT_FINAL = 1 since the last IRT call");
+        return DNA_ERR_ARG;
+    }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    const uint32_t ntr = c->final_ntr;
+    DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, 2 * (size_t)ntr + 2 + ((size_t)ntr + 1023) / 1024 + 8));
+    uint32_t *off = (uint32_t *)c->irt.part, *tmp = off + ntr + 2;
+    DNA_HIP(hipMemcpyAsync(off, c->d_ntot, (size_t)ntr * 4, hipMemcpyDeviceToDevice, st));
+    DNA_HIP(exclusive_scan_u32(off, ntr, tmp, st));
+    uint32_t total = 0;
+    DNA_HIP(hipMemcpyAsync(&total, off + ntr, 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    *n_out = total;
+    if (!out6) return DNA_OK;
+    if (cap < total) { set_error("dna_irt_particles: output capacity below the particle count (see *n_out)"); return DNA_ERR_CAPACITY; }
+    double *d = nullptr;
+    DNA_HIP(hipMalloc(&d, (size_t)total * 48 + 48));
+    hipLaunchKernelGGL(k_particles_write, dim3(ntr), dim3(256), 0, st, (const IrtTrack *)c->final_tracks, c->final_pos,
+                       c->final_tb, c->final_td, c->d_ntot, off, d);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out6, d, (size_t)total * 48, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    hipFree(d);
+    DNA_HIP(e);
     return DNA_OK;
 }

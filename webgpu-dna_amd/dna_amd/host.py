@@ -118,28 +118,38 @@ def _chem_result(o, t_wall_ms):
             "chem_alive_final": None, "n_reacted": d["n_reacted"], "rxn_info": rxn_info}
 
 
-def run_chemistry_worker(rad_buf_f32, rad_n, n_therm, E_eV, log=None, max_n=None, seed=1, oh_final=False):
+def run_chemistry_worker(rad_buf_f32, rad_n, n_therm, E_eV, log=None, max_n=None, seed=1, oh_final=False,
+                         options=None):
     """runChemistryWorker (src/chemistry/worker.ts:32-87): the 9-reaction IRT to 1 us on
     the GPU.  rad_buf_f32 is a (rad_n, 4) float32 array or a flat Float32Array-like.
     oh_final: also keep the OH alive at 1 us on the device for the indirect-SSB scoring
-    (res["chem_oh_n"]); the worker itself returns no positions (worker.ts:69-70)."""
+    (res["chem_oh_n"]); the worker itself returns no positions (worker.ts:69-70).
+    options: {"dump_snapshots": bool, "snapshot_n": int} as runChemistry's
+    (schedule.ts:16-27): res["snapshots"] then holds t = 0 and the seven checkpoints of the
+    leading snapshot_n particle slots (default 50 000), for the WGDNA4D blob."""
     from .abi import DNA_OPT_IRT_FINAL
     cc = ensure_pipelines(_cache["np"] if _cache else 0)
     if cc is None:
         raise RuntimeError("IRT worker error: no GPU device")
     rad = np.ascontiguousarray(rad_buf_f32, np.float32).reshape(-1, 4)[:rad_n]
+    dump = bool(options and options.get("dump_snapshots"))
+    keep = oh_final or dump
     t0 = time.perf_counter()
     ctx = cc["ctx"]
-    if oh_final:
+    if keep:
         ctx.set_option(DNA_OPT_IRT_FINAL, 1)
     try:
         o = ctx.irt(rad, rad_n, n_therm, float(E_eV), list(CHECKPOINTS_NS), seed=seed)
     finally:
-        if oh_final:
+        if keep:
             ctx.set_option(DNA_OPT_IRT_FINAL, 0)
     res = _chem_result(o, (time.perf_counter() - t0) * 1e3)
     if oh_final:
         res["chem_oh_n"] = ctx.irt_oh_final(copy=False)
+    if dump:
+        from .splat import DEFAULT_SNAPSHOT_N, snapshots_from_particles
+        res["snapshots"] = snapshots_from_particles(ctx.irt_particles(),
+                                                    options.get("snapshot_n", DEFAULT_SNAPSHOT_N))
     if log:
         for cp in res["timeline"]:
             log(f"  {cp['label']} {cp['G_OH']:.3f} {cp['G_eaq']:.3f} {cp['G_H']:.3f} "

@@ -50,6 +50,7 @@ def lib():
     L.dna_get_option.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.dna_xs_probe.argtypes = [vp, C.c_int, vp, vp, u64, vp]
     L.dna_irt_oh_final.argtypes = [vp, vp, u64, C.POINTER(u64)]
+    L.dna_irt_particles.argtypes = [vp, vp, u64, C.POINTER(u64)]
     L.dna_score_oh.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, vp, u64, C.POINTER(u32), C.c_int,
                                C.POINTER(DnaScoreOut), C.POINTER(DnaIndirectOut)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
@@ -79,7 +80,7 @@ EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
-                    "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh"]
+                    "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh", "dna_irt_particles"]
 
 
 def _check(fn, rc):
@@ -232,6 +233,15 @@ class Context:
             return n.value
         out = np.empty((max(n.value, 1), 4), np.float32)
         _check("dna_irt_oh_final", lib().dna_irt_oh_final(self.h, _ptr(out), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def irt_particles(self):
+        """Every particle slot of the last irt() made with DNA_OPT_IRT_FINAL = 1: [n, 6]
+        float64 (x, y, z, pid*8+code, t_birth, t_death), track order."""
+        n = C.c_uint64()
+        _check("dna_irt_particles", lib().dna_irt_particles(self.h, None, 0, C.byref(n)))
+        out = np.empty((max(n.value, 1), 6), np.float64)
+        _check("dna_irt_particles", lib().dna_irt_particles(self.h, _ptr(out), n.value, C.byref(n)))
         return out[:n.value]
 
     def score_oh(self, target, rad=None, rad_n=0, oh=None, n_oh=0, lcg_state=0x12345678, want_hits=False):
