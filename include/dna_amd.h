@@ -227,6 +227,11 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * dna_irt_oh_final and dna_irt_particles, and records each particle's death time
  * (default 0). */
 #define DNA_OPT_IRT_FINAL       3
+/* DNA_OPT_PA_LANES: active lanes per 64-lane wavefront of the primary-transport kernel,
+ * 0 = automatic (the widest power of two that still gives every CU one wavefront: 16 for
+ * 4096 primaries on 256 CUs, 64 from 16384 primaries), or 1, 2, 4, ..., 64.  Results do
+ * not depend on it. */
+#define DNA_OPT_PA_LANES        4
 int dna_set_option(struct dna_ctx *ctx, int option, int value);
 int dna_get_option(struct dna_ctx *ctx, int option, int *value);
 

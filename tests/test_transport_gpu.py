@@ -42,3 +42,21 @@ def test_transport_start_cube_and_energies(ctx):
     g = ctx.transport(p, energies=E)
     o = O.transport(p, energies=E)
     _cmp(g, o, n)
+
+
+@pytest.mark.parametrize("lanes", [1, 4, 16, 64])
+def test_transport_lanes_per_wavefront(ctx, lanes):
+    """The primary kernel's launch shape (active lanes per wavefront, DNA_OPT_PA_LANES)
+    leaves every result unchanged: RNG streams are keyed by primary and the radical list is
+    put in canonical order."""
+    from dna_amd.abi import DNA_OPT_PA_LANES
+    p = default_params(3000.0, 200)
+    print(f"transport 3 keV x 200 with {lanes} lanes per wavefront", flush=True)
+    ctx.set_option(DNA_OPT_PA_LANES, lanes)
+    try:
+        g = ctx.transport(p)
+    finally:
+        ctx.set_option(DNA_OPT_PA_LANES, 0)
+    _cmp(g, O.transport(p), 200)
+    with pytest.raises(Exception, match="DNA_OPT_PA_LANES"):
+        ctx.set_option(DNA_OPT_PA_LANES, 3)

@@ -65,6 +65,7 @@ struct dna_ctx {
     std::map<int, int> irt_tuned;      // log2(mean particles per track) -> workgroup size
     // final IRT state (DNA_OPT_IRT_FINAL) and the alive-OH list made from it
     int irt_final = 0;
+    int pa_lanes = 0;                 // DNA_OPT_PA_LANES (0: chosen from the batch size)
     bool final_ready = false;
     uint32_t final_ntr = 0;
     uint64_t final_seed = 0;

@@ -55,8 +55,9 @@ enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, 
        C_SST = 8 /* 8..15 sec_stats */, C_DBG = 16 /* 16..23 dbg */, C_PNEXT = 24, C_SNEXT = 25,
        C_NCTR = 32 };
 
-size_t transport_slot_headroom(uint32_t n);
-hipError_t launch_phase_a(const TransportDev &d, hipStream_t st);
+uint32_t phase_a_lanes(uint32_t n, int opt);
+size_t transport_slot_headroom(uint32_t n, uint32_t lanes);
+hipError_t launch_phase_a(const TransportDev &d, uint32_t lanes, hipStream_t st);
 hipError_t launch_phase_b(const TransportDev &d, uint32_t sec_n, hipStream_t st);
 hipError_t launch_xs_probe(const dna_tables *T, const uint8_t *guide, const float *E, const float *r, uint32_t n,
                            int which, float *out, hipStream_t st);

@@ -148,6 +148,13 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
         if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_IRT_FINAL must be 0 or 1"); return DNA_ERR_ARG; }
         c->irt_final = value;
         return DNA_OK;
+    case DNA_OPT_PA_LANES:
+        if (value < 0 || value > 64 || (value & (value - 1))) {
+            set_error("dna_set_option: DNA_OPT_PA_LANES must be 0 (automatic) or a power of two up to 64");
+            return DNA_ERR_ARG;
+        }
+        c->pa_lanes = value;
+        return DNA_OK;
     default:
         set_error("dna_set_option: unknown option");
         return DNA_ERR_ARG;
@@ -161,6 +168,7 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     case DNA_OPT_IRT_BLOCK: *value = c->irt_block; return DNA_OK;
     case DNA_OPT_IRT_BLOCK_LAST: *value = c->irt_last_block; return DNA_OK;
     case DNA_OPT_IRT_FINAL: *value = c->irt_final; return DNA_OK;
+    case DNA_OPT_PA_LANES: *value = c->pa_lanes; return DNA_OK;
     default:
         set_error("dna_get_option: unknown option");
         return DNA_ERR_ARG;
@@ -230,7 +238,8 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
     // capacity: reference yields are ~0.13 radical entries and ~0.0145 secondaries per eV;
     // grow and repeat the (deterministic) run if a count overflows
     size_t rad_cap = p.max_rad ? p.max_rad : (size_t)(0.2 * sumE) + 65536;
-    rad_cap += transport_slot_headroom(p.n);       // chunk tails left unused by each wavefront
+    const uint32_t pa_lanes = phase_a_lanes(p.n, c->pa_lanes);
+    rad_cap += transport_slot_headroom(p.n, pa_lanes);   // chunk tails left unused by each wavefront
     size_t sec_cap = p.max_sec ? p.max_sec : (size_t)(0.025 * sumE) + 4096;
     const size_t HARD = (size_t)0xFFFFFFF0u;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -250,7 +259,7 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
         d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
         DNA_HIP(hipEventRecord(c->ev[0], st));
-        DNA_HIP(launch_phase_a(d, st));
+        DNA_HIP(launch_phase_a(d, pa_lanes, st));
         DNA_HIP(hipEventRecord(c->ev[1], st));
         DNA_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, C_NCTR * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         DNA_HIP(hipStreamSynchronize(st));

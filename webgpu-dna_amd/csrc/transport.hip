@@ -683,7 +683,10 @@ __device__ bool prim_step(const TransportDev &d, const dna_params &p, const Smal
     return false;
 }
 
-__global__ void __launch_bounds__(256) k_phase_a(TransportDev d)
+// lanes: the active lanes of each 64-lane wavefront (1..64); the others only help stage
+// the tables.  Fewer lanes per wavefront means more wavefronts for a small batch, fewer
+// branch paths per wavefront and less waiting behind the longest history in it.
+__global__ void __launch_bounds__(256) k_phase_a(TransportDev d, uint32_t lanes)
 {
     __shared__ SmallTab S;
     stage_tables(S, d.tab, d.guide);
@@ -696,8 +699,9 @@ __global__ void __launch_bounds__(256) k_phase_a(TransportDev d)
     for (int i = 0; i < 6; i++) k.cnt[i] = 0;
     uint32_t dbg[5] = {0, 0, 0, 0, 0};
     PrimState st;
-    uint32_t idx = wave_claim(&d.ctr[C_PNEXT]);
-    bool active = idx < p.n;
+    const bool lane_on = (threadIdx.x & 63u) < lanes;
+    uint32_t idx = lane_on ? wave_claim(&d.ctr[C_PNEXT]) : 0xFFFFFFFFu;
+    bool active = lane_on && idx < p.n;
     if (active) prim_init(d, p, idx, st, k);
     while (active) {
         bool done = prim_step(d, p, S, idx, st, k, &dbg[4]);
@@ -855,25 +859,41 @@ static int num_cu()
     return g_num_cu;
 }
 
-size_t transport_slot_headroom(uint32_t n)
+uint32_t phase_a_lanes(uint32_t n, int opt)
+{
+    if (opt > 0) return (uint32_t)opt;
+    // the widest wavefronts that still give every CU one (measured on MI355X at 1024, 4096
+    // and 16384 primaries: one wavefront per CU beats both fewer, wider wavefronts and one
+    // per SIMD)
+    const uint32_t target = (uint32_t)num_cu();
+    uint32_t L = 64u;
+    while (L > 1u && (n + L - 1u) / L < target) L >>= 1;
+    return L;
+}
+
+// One 64-lane block per wavefront, `lanes` of them active, one primary per active lane up
+// to the device's resident capacity (further primaries are claimed as lanes finish).
+static uint32_t phase_a_blocks(uint32_t n, uint32_t lanes)
+{
+    const uint32_t lanes_max = (uint32_t)num_cu() * 4u * 64u * 4u;
+    const uint32_t m = n < lanes_max ? n : lanes_max;
+    return (m + lanes - 1u) / lanes;
+}
+
+size_t transport_slot_headroom(uint32_t n, uint32_t lanes)
 {
     // every wavefront of either phase may leave up to RAD_CHUNK reserved slots unused
-    const size_t lanes_max = (size_t)num_cu() * 4u * 64u * 4u;
-    const size_t waves_a = ((n < lanes_max ? n : lanes_max) + 63u) / 64u;
-    const size_t waves_b = lanes_max / 64u;
+    const size_t waves_a = phase_a_blocks(n, lanes);
+    const size_t waves_b = (size_t)num_cu() * 4u * 4u;
     return (waves_a + waves_b) * RAD_CHUNK;
 }
 
-hipError_t launch_phase_a(const TransportDev &d, hipStream_t st)
+hipError_t launch_phase_a(const TransportDev &d, uint32_t lanes, hipStream_t st)
 {
-    // one lane per primary up to the device's resident capacity; 64-lane blocks spread
-    // small batches over all CUs
-    uint32_t n = d.p.n;
-    uint32_t lanes_max = (uint32_t)num_cu() * 4u * 64u * 4u;
-    uint32_t lanes = n < lanes_max ? n : lanes_max;
-    uint32_t blocks = (lanes + 63u) / 64u;
+    if (lanes < 1u || lanes > 64u) return hipErrorInvalidValue;
+    const uint32_t blocks = phase_a_blocks(d.p.n, lanes);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_phase_a, dim3(blocks), dim3(64), 0, st, d);
+    hipLaunchKernelGGL(k_phase_a, dim3(blocks), dim3(64), 0, st, d, lanes);
     return hipGetLastError();
 }
 

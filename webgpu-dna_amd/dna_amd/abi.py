@@ -5,7 +5,7 @@ DNA_OK = 0
 DNA_MAX_CKPT = 32
 DNA_N_RXN = 9
 DNA_COPY_RESULTS, DNA_COPY_DOSE, DNA_COPY_RAD, DNA_COPY_ALL = 1, 2, 4, 7
-DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST, DNA_OPT_IRT_FINAL = 1, 2, 3
+DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST, DNA_OPT_IRT_FINAL, DNA_OPT_PA_LANES = 1, 2, 3, 4
 
 
 class DnaParams(C.Structure):

@@ -12,6 +12,11 @@ profiles/.
   rocprof_summary.py traffic <fetch.db> <write.db> [f_fetch f_write]
                                                    per-dispatch FETCH_SIZE / WRITE_SIZE and the
                                                    bytes after copy-calibration factors
+  rocprof_summary.py utilization <util.db> [<fetch.db> <write.db> f_fetch f_write]
+                                                   per kernel and launch shape: average duration,
+                                                   the derived counters of util.db (VALUBusy,
+                                                   OccupancyPercent, ...) averaged over dispatches,
+                                                   and HBM GB/s from the traffic runs
 """
 import sqlite3
 import sys
@@ -77,8 +82,57 @@ def traffic(fetch_db, write_db, f_fetch=1.0, f_write=1.0, top=16):
         print(f"{short[:34]:34s} {d:8d} {f:12.1f} {w:12.1f} {f * 1024 * f_fetch / 1e6:10.1f} {w * 1024 * f_write / 1e6:10.1f}")
 
 
+def _short(n):
+    return n.split("(")[0].replace("void ", "")
+
+
+def utilization(util_db, fetch_db=None, write_db=None, f_fetch=1.0, f_write=1.0):
+    """Per (kernel, grid, workgroup): dispatches, mean duration (ms) and the mean per-dispatch
+    value of every counter in util_db; with the traffic runs, mean HBM bytes per dispatch
+    (copy-calibrated) over the mean duration of those same dispatches, in GB/s."""
+    def per_dispatch(path, counter=None):
+        db = sqlite3.connect(path)
+        q = ("select p.dispatch_id, p.counter_name, sum(p.counter_value) from pmc_events p "
+             + ("where p.counter_name = ? " if counter else "") + "group by p.dispatch_id, p.counter_name")
+        vals = defaultdict(dict)
+        for d, c, v in db.execute(q, (counter,) if counter else ()):
+            vals[d][c] = v
+        shape = {d: (_short(n), gx, wg, du) for d, n, gx, wg, du in
+                 db.execute("select dispatch_id, name, grid_x, workgroup_x, duration from kernels")}
+        return vals, shape
+    vals, shape = per_dispatch(util_db)
+    counters = sorted({c for v in vals.values() for c in v})
+    groups = defaultdict(list)
+    for d, (n, gx, wg, du) in shape.items():
+        groups[(n, gx, wg)].append((du, vals.get(d, {})))
+    hbm = {}
+    if fetch_db and write_db:
+        fv, fs = per_dispatch(fetch_db, "FETCH_SIZE")
+        wv, _ = per_dispatch(write_db, "WRITE_SIZE")
+        acc = defaultdict(lambda: [0.0, 0.0])
+        for d, (n, gx, wg, du) in fs.items():
+            if d in fv and d in wv:
+                b = (fv[d]["FETCH_SIZE"] * f_fetch + wv[d]["WRITE_SIZE"] * f_write) * 1024
+                acc[(n, gx, wg)][0] += b
+                acc[(n, gx, wg)][1] += du
+        hbm = {k: (b / t if t else 0.0) for k, (b, t) in acc.items()}
+    head = f"{'kernel':26s} {'grid':>8s} {'wg':>4s} {'calls':>5s} {'avg_ms':>9s}"
+    head += "".join(f" {c[:16]:>16s}" for c in counters) + (f" {'HBM_GB/s':>9s}" if hbm else "")
+    print(head)
+    order = sorted(groups, key=lambda k: -sum(du for du, _ in groups[k]))
+    for k in order:
+        g = groups[k]
+        line = f"{k[0][:26]:26s} {k[1]:8d} {k[2]:4d} {len(g):5d} {sum(du for du, _ in g) / len(g) / 1e6:9.4f}"
+        for c in counters:
+            xs = [v[c] for _, v in g if c in v]
+            line += f" {sum(xs) / len(xs):16.3f}" if xs else f" {'-':>16s}"
+        if hbm:
+            line += f" {hbm[k]:9.1f}" if k in hbm else f" {'-':>9s}"
+        print(line)
+
+
 if __name__ == "__main__":
-    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "traffic", "dispatches"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("trace", "pmc", "traffic", "dispatches", "utilization"):
         print(__doc__)
         sys.exit(2)
     if sys.argv[1] == "trace":
@@ -87,6 +141,9 @@ if __name__ == "__main__":
         dispatches(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else "")
     elif sys.argv[1] == "pmc":
         pmc(sys.argv[2], sys.argv[3] if len(sys.argv) > 3 else None)
+    elif sys.argv[1] == "utilization":
+        a = sys.argv[2:]
+        utilization(a[0], *(a[1:3] if len(a) >= 3 else []), *[float(x) for x in a[3:5]])
     else:
         f = [float(x) for x in sys.argv[4:6]] if len(sys.argv) >= 6 else [1.0, 1.0]
         traffic(sys.argv[2], sys.argv[3], f[0], f[1])
