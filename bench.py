@@ -157,8 +157,9 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
 
 
 def cpu_baseline(sample_prims, threads):
-    """The CPU restatement (oracle/) timed on this host: the same chain on a bounded
-    sample of the same workload (primaries 0..sample-1 of the 10 keV run)."""
+    """The CPU restatement (oracle/) timed on this host: the same chain on primaries
+    0..sample-1 of the 10 keV run (all of them by default), transport and IRT on
+    `threads` threads, scoring on one."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from dna_amd import DNATarget, default_params
@@ -186,7 +187,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-sample", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=4096,
+                    help="primaries of the CPU baseline on the host cores (default: the whole batch)")
+    ap.add_argument("--cpu-1core-sample", type=int, default=128,
+                    help="primaries of the single-thread CPU run (the reference worker is single-threaded)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--inflight", type=int, default=2,
@@ -303,6 +307,9 @@ def main():
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)
+        if args.cpu_1core_sample > 0:
+            one = cpu_baseline(args.cpu_1core_sample, 1)
+            out["cpu_baseline"]["single_core"] = {k: one[k] for k in ("value", "unit", "cores", "sample", "seconds")}
         out["gpu_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
