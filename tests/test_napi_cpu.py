@@ -37,7 +37,8 @@ def test_addon_exports():
     out = json.loads(_node(f"const a=require({json.dumps(ADDON)}); "
                            "console.log(JSON.stringify({k:Object.keys(a), v:a.version()}))"))
     assert set(out["k"]) == {"version", "lastError", "ctxCreate", "ctxDestroy", "runTransport", "runIrt",
-                             "score", "hashCells"}
+                             "score", "hashCells", "setOption", "getOption", "irtOhFinal", "irtParticles",
+                             "scoreOh"}
     assert "gfx950" in out["v"]
 
 
@@ -72,3 +73,44 @@ def test_dna_target_v8_matches_python_bits(tmp_path):
     for k in ("fy", "fz", "rbb0", "rbb1"):
         js = np.fromfile(os.path.join(base, k), np.float32)
         assert np.array_equal(js.view(np.uint32), getattr(t, k).view(np.uint32)), k
+
+
+def test_snapshot_blob_js_matches_python_bytes(tmp_path):
+    """splat.js (V8) and dna_amd.splat read the same particle table into the same snapshots
+    and pack the same WGDNA4D bytes; the JS parser reads them back."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "webgpu-dna_amd"))
+    from dna_amd import DNATarget
+    from dna_amd.splat import pack_snapshot_blob, snapshots_from_particles
+    rng = np.random.default_rng(4)
+    n = 3000
+    parts = np.zeros((n, 6))
+    parts[:, :3] = rng.normal(0, 300, (n, 3)).astype(np.float32)
+    parts[:, 3] = rng.integers(0, 500, n) * 8 + rng.choice([0, 1, 2, 3, 4, 6], n)
+    born = rng.random(n) < 0.3
+    parts[:, 4] = np.where(born, 10 ** rng.uniform(-3, 3, n), 0.0)
+    die = rng.random(n) < 0.6
+    parts[:, 5] = np.where(die, parts[:, 4] + 10 ** rng.uniform(-3, 3, n), np.inf)
+    parts[:7, 4] = [0.0, 0.001, 0.01, 1.0, 0.0, 0.1, 1000.0]           # births and deaths on checkpoints
+    parts[:7, 5] = [0.001, 0.01, np.inf, 10.0, 1000.0, 0.1, np.inf]
+    rad0 = rng.normal(0, 50, (500, 4)).astype(np.float32)
+    base = str(tmp_path)
+    parts.tofile(base + "/parts.f64")
+    rad0.tofile(base + "/rad0.f32")
+    script = (f"const s=require({json.dumps(os.path.join(ROOT, 'webgpu-dna_amd', 'js', 'splat.js'))}); "
+              f"const h=require({json.dumps(HOST)}); const fs=require('fs'); "
+              f"const rd=(f,T)=>{{const b=fs.readFileSync({json.dumps(base)}+'/'+f); "
+              f"return new T(b.buffer.slice(b.byteOffset,b.byteOffset+b.length));}}; "
+              f"const snaps=s.snapshotsFromParticles(rd('parts.f64',Float64Array), 2000); "
+              f"const blob=s.packSnapshotBlob(snaps, h.buildDNATarget(), rd('rad0.f32',Float32Array), 400, 10000); "
+              f"fs.writeFileSync({json.dumps(base)}+'/blob.bin', blob); "
+              f"const b=s.parseSnapshotBlob(blob); "
+              f"console.log(JSON.stringify({{n:b.num_snaps, sn:b.snap_n, labels:b.snapshots.map(x=>x.label), "
+              f"alive:b.snapshots.map(x=>x.alive.reduce((a,c)=>a+c,0)), r0:b.rad0.rad_n_initial}}));")
+    out = json.loads(_node(script))
+    ss = snapshots_from_particles(parts, 2000)
+    ref = pack_snapshot_blob(ss, DNATarget(), rad0, 400, 10000)
+    got = open(base + "/blob.bin", "rb").read()
+    assert got == ref
+    assert out == {"n": 8, "sn": 2000, "labels": [s["label"] for s in ss],
+                   "alive": [int(s["alive"].sum()) for s in ss], "r0": 400}

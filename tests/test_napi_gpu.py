@@ -47,3 +47,28 @@ def test_js_run_validation_matches_oracle():
     res = out["results"]
     assert res["nPrimaries"] == 96 and res["primaryEnergyEv"] == 10000
     assert set(res) >= {"tracking", "preChemistry", "chemistry1Us", "dnaDamage", "karamitros2011"}
+
+
+@pytest.mark.skipif(NODE is None, reason="node is missing")
+def test_js_snapshot_export_and_indirect_ssb_match_python(tmp_path):
+    """generateSnapshotAt10keV from Node writes the same WGDNA4D bytes as the Python host,
+    and runValidation with indirectSSB gives the same damage (direct, indirect, DSB)."""
+    from dna_amd import host, splat
+    out = str(tmp_path / "js.bin")
+    print("node generateSnapshotAt10keV np=96 + runValidation indirectSSB np=256", flush=True)
+    script = (f"const s=require({json.dumps(os.path.join(ROOT, 'webgpu-dna_amd', 'js', 'splat.js'))}); "
+              f"const h=require({json.dumps(os.path.join(ROOT, 'webgpu-dna_amd', 'js', 'dna_host.js'))}); "
+              f"(async()=>{{ const p=await s.exportSnapshotAt10keV({{np:96, boxNm:15000, ceEV:7.4, snapshot_n:6000}}, "
+              f"{json.dumps(out)}); "
+              f"const v=await h.runValidation({{np:256, boxNm:15000, ceEV:7.4, energies:[10000], indirectSSB:true}}); "
+              f"console.log(JSON.stringify({{p, d:v.rows[0].damage, r:v.results.dnaDamage}})); }})()"
+              f".catch(e=>{{console.error(e); process.exit(1);}});")
+    r = subprocess.run([NODE, "-e", script], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-3000:]
+    js = json.loads(r.stdout.strip().splitlines()[-1])
+    py = splat.generate_snapshot_at_10kev({"np": 96, "boxNm": 15000.0, "ceEV": 7.4, "snapshot_n": 6000})
+    assert open(out, "rb").read() == py["blob"]
+    v = host.run_validation({"np": 256, "boxNm": 15000.0, "ceEV": 7.4, "energies": [10000], "indirectSSB": True})
+    d = v["rows"][0]["damage"]
+    assert (js["d"]["ssb_dir"], js["d"]["ssb_ind"], js["d"]["dsb"]) == (d["ssb_dir"], d["ssb_ind"], d["dsb"])
+    assert js["r"]["SSB_ind"] == d["ssb_ind"] == v["results"]["dnaDamage"]["SSB_ind"]

@@ -242,7 +242,8 @@ def results_json(r, damage, np_):
         "preChemistry": {"G_OH_init": round(r["G_OH"], 2), "G_eaq_init": round(r["G_eaq"], 2),
                          "G_H_init": round(r["G_H"], 2)},
         "chemistry1Us": {k: round(g.get(k, 0.0), 3) for k in ("G_OH", "G_eaq", "G_H", "G_H2O2", "G_H2")},
-        "dnaDamage": {"SSB_dir": damage["ssb_dir"] if damage else 0, "SSB_ind": 0,
+        "dnaDamage": {"SSB_dir": damage["ssb_dir"] if damage else 0,
+                      "SSB_ind": damage["ssb_ind"] if damage else 0,
                       "DSB": damage["dsb"] if damage else 0, "kernelHits": r["kernel_dna_hits"]},
         "karamitros2011": KARAMITROS_2011,
     }

@@ -11,11 +11,22 @@ export interface ChemCheckpoint {
   label: string; t_ns: number; G_OH: number; G_eaq: number; G_H: number; G_H2O2: number; G_H2: number;
 }
 
+// src/physics/types.ts:57-63
+export interface ChemSnapshot {
+  label: string; t_ns: number; n: number; pos: Float32Array; alive: Uint32Array;
+}
+
 export interface ChemResult {
   chem_n: number; t_wall: number; timeline: ChemCheckpoint[];
   chem_pos_final: Float32Array | null; chem_alive_final: Uint32Array | null;
   n_reacted: number; rxn_info: { label: string; count: number }[];
+  /** with options.ohFinal: the OH alive at 1 us kept on the device for scoreOh */
+  chem_oh_n?: number;
+  /** with options.dump_snapshots: t = 0 and the seven checkpoints (splat.js) */
+  snapshots?: ChemSnapshot[];
 }
+
+export interface ChemOptions { ohFinal?: boolean; dump_snapshots?: boolean; snapshot_n?: number; }
 
 export interface EnergyResult {
   E: number; n_therm: number; n_esc: number; mean_total: number; mean_prod: number; mean_sp: number;
@@ -38,6 +49,8 @@ export interface DamageResult {
 export interface ValidationConfig {
   np: number; boxNm: number; ceEV?: number; log?: LogFn; chemBackend?: 'worker' | 'none';
   energies?: number[];
+  /** extension: score the OH alive at 1 us (indirect SSB) before the direct pass */
+  indirectSSB?: boolean;
 }
 
 export function buildDNATarget(L_nm?: number, grid_N?: number, spacing_nm?: number): DNATarget;
@@ -45,7 +58,7 @@ export function ensurePipelines(np: number, log?: LogFn): PipeCache | null;
 export function runAtEnergy(cc: PipeCache, E_eV: number, np: number, boxNm: number, ceEV: number,
   dna: DNATarget | null, runChemistry?: ChemCallback | null, seed?: number): Promise<EnergyResult>;
 export function runChemistryWorker(radBuf: Float32Array, radN: number, nTherm: number, E_eV: number,
-  log?: LogFn, maxN?: number, seed?: number): Promise<ChemResult>;
+  log?: LogFn, maxN?: number, seed?: number, options?: ChemOptions | null): Promise<ChemResult>;
 export function makeSsbRng(seed?: number): SsbRng;
 export function scoreDamageAt10keV(cc: PipeCache, dna: DNATarget, chem: ChemResult | null,
   radBuf: Float32Array, radN: number, totalDepositedEV: number, boxNm: number, rng: SsbRng, log?: LogFn,

@@ -7,13 +7,17 @@
  *   ensurePipelines(np, log)                     src/app.ts:65-84
  *   runAtEnergy(cc, E_eV, np, boxNm, ceEV, dna, runChemistry?)
  *                                                src/gpu/dispatch.ts:85-285 -> EnergyResult
- *   runChemistryWorker(radBuf, radN, nTherm, E_eV, log?, maxN?)
+ *   runChemistryWorker(radBuf, radN, nTherm, E_eV, log?, maxN?, seed?, options?)
  *                                                src/chemistry/worker.ts:32-87 -> ChemResult
+ *                                                (options: {ohFinal} keeps the OH alive at 1 us for
+ *                                                the indirect-SSB scoring; {dump_snapshots, snapshot_n}
+ *                                                as runChemistry's, src/chemistry/schedule.ts:16-27)
  *   buildDNATarget(L_nm, grid_N, spacing_nm)     src/physics/dna-geometry.ts:16-68
  *   makeSsbRng(seed)                             src/scoring/ssb-dsb.ts:421-427
  *   scoreDamageAt10keV(...)                      src/app.ts:167-207
- *   runValidation(cfg)                           src/app.ts:91-161
- * A missing device gives null plus an 'err' log line (src/gpu/device.ts:12-21).
+ *   runValidation(cfg)                           src/app.ts:91-161 (cfg.indirectSSB: the extension
+ *                                                that scores the OH at 1 us first, ssb-dsb.ts:43-121)
+ * The WGDNA4D snapshot blob is in splat.js.  A missing device gives null plus an 'err' log line (src/gpu/device.ts:12-21).
  * Types: dna_host.d.ts.
  */
 const path = require('path');
@@ -42,6 +46,8 @@ const RXN_LABELS = ['OH+OH→H2O2 [PDC]', 'eaq+OH→OH- [PDC]', 'OH+H→H2O [PDC
   'eaq+H→OH-+H2 [TDC]', 'eaq+H3O+→H [PDC]', 'H+H→H2 [TDC]', 'eaq+H2O2→OH-+OH [PDC]',
   'H3O++OH-→H2O [TDC]'];
 const COPY_RESULTS = 1, COPY_DOSE = 2, COPY_RAD = 4, COPY_ALL = 7;
+// include/dna_amd.h options
+const OPT_IRT_BLOCK = 1, OPT_IRT_BLOCK_LAST = 2, OPT_IRT_FINAL = 3, OPT_PA_LANES = 4;
 
 function noop() {}
 
@@ -155,12 +161,19 @@ async function runAtEnergy(cc, E_eV, np, boxNm, ceEV, dna, runChemistry, seed) {
 }
 
 /** runChemistryWorker (worker.ts:32-87): the 9-reaction IRT to 1 us on the GPU. */
-async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 1) {
+async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 1, options = null) {
   const cc = ensurePipelines(cache ? cache.np : 0);
   if (!cc) throw new Error('IRT worker error: no GPU device');
   const t0 = Date.now();
   const rad = radBuf.length > 4 * radN ? radBuf.subarray(0, 4 * radN) : radBuf;
-  const o = await addon.runIrt(cc.ctx, rad, radN, nTherm, E_eV, Float64Array.from(CHECKPOINTS_NS), seed);
+  const ohFinal = !!(options && options.ohFinal), dump = !!(options && options.dump_snapshots);
+  if (ohFinal || dump) addon.setOption(cc.ctx, OPT_IRT_FINAL, 1);
+  let o;
+  try {
+    o = await addon.runIrt(cc.ctx, rad, radN, nTherm, E_eV, Float64Array.from(CHECKPOINTS_NS), seed);
+  } finally {
+    if (ohFinal || dump) addon.setOption(cc.ctx, OPT_IRT_FINAL, 0);
+  }
   const timeline = [];
   for (let k = 0; k < o.nCkpt; k++) {
     timeline.push({
@@ -173,6 +186,13 @@ async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 
     n_reacted: o.nReacted,
     rxn_info: RXN_LABELS.map((label, r) => ({ label, count: o.rxnCounts[r] })),
   };
+  if (ohFinal) res.chem_oh_n = (await addon.irtOhFinal(cc.ctx, false)).n;
+  if (dump) {
+    const splat = require('./splat');
+    const p = await addon.irtParticles(cc.ctx);
+    res.snapshots = splat.snapshotsFromParticles(p.particles,
+      options.snapshot_n != null ? options.snapshot_n : splat.DEFAULT_SNAPSHOT_N);
+  }
   if (log) {
     for (const cp of timeline) {
       log(`  ${cp.label} ${cp.G_OH.toFixed(3)} ${cp.G_eaq.toFixed(3)} ${cp.G_H.toFixed(3)} ` +
@@ -190,18 +210,23 @@ function makeSsbRng(seed = 0x12345678) {
 }
 
 /** scoreDamageAt10keV (app.ts:167-207): direct-event SSB + combineHits + clusterDSB on
- * the GPU; the indirect hits are empty on the worker path (app.ts:178-180). */
+ * the GPU.  The indirect hits are empty on the worker path (app.ts:178-180) unless the
+ * chemistry kept its OH at 1 us (runChemistryWorker options.ohFinal): then scoreIndirectSSB
+ * runs first on those, as on the reference's GPU-chemistry path. */
 async function scoreDamageAt10keV(cc, dna, chem, radBuf, radN, totalDepositedEV, boxNm, rng, log = noop,
   kernelHits = 0) {
-  const s = await addon.score(cc.ctx, dna, radBuf, radN, rng.state, false);
+  const withOh = chem && chem.chem_oh_n != null;
+  const s = withOh ? await addon.scoreOh(cc.ctx, dna, radBuf, radN, null, chem.chem_oh_n, rng.state, false)
+    : await addon.score(cc.ctx, dna, radBuf, radN, rng.state, false);
+  const ind = withOh ? s.indirect : { ssb0: 0, ssb1: 0, inReach: 0 };
   rng.state = s.lcgState >>> 0;
-  const ssb_dir = s.ssbCount, ssb_ind = 0, dsb = s.dsb;
+  const ssb_dir = s.ssbCount, ssb_ind = ind.ssb0 + ind.ssb1, dsb = s.dsb;
   const doseBoxGy = (totalDepositedEV * 1.602e-19) / (Math.pow(2 * boxNm * 1e-9, 3) * 1000);
   const targetGbp = dna.n_bp * 1e-9;
   const ssbTot = ssb_dir + ssb_ind;
   log(`    DAMAGE: SSB_dir=${ssb_dir}  SSB_ind=${ssb_ind}  DSB=${dsb}  ` +
       `(DSB/SSB=${(ssbTot ? dsb / ssbTot : 0).toFixed(3)})  box_dose=${doseBoxGy.toFixed(3)} Gy  ` +
-      `target=${(dna.n_bp / 1e6).toFixed(1)} Mbp  reach_dir=${s.inReach} reach_ind=0  kernel_hits=${kernelHits}`, 'data');
+      `target=${(dna.n_bp / 1e6).toFixed(1)} Mbp  reach_dir=${s.inReach} reach_ind=${ind.inReach}  kernel_hits=${kernelHits}`, 'data');
   return {
     ssb_dir, ssb_ind, dsb, dsb_per_gy_gbp: doseBoxGy > 0 ? dsb / (doseBoxGy * targetGbp) : 0,
     dsb_expected: s.dsbExpected, in_reach: s.inReach,
@@ -226,7 +251,8 @@ function resultsJson(r, damage, np) {
       G_H2: r3(g.G_H2 || 0),
     },
     dnaDamage: {
-      SSB_dir: damage ? damage.ssb_dir : 0, SSB_ind: 0, DSB: damage ? damage.dsb : 0, kernelHits: r.kernel_dna_hits,
+      SSB_dir: damage ? damage.ssb_dir : 0, SSB_ind: damage ? damage.ssb_ind : 0, DSB: damage ? damage.dsb : 0,
+      kernelHits: r.kernel_dna_hits,
     },
     karamitros2011: KARAMITROS_2011,
   };
@@ -243,8 +269,10 @@ async function runValidation(cfg) {
   log(`Running ${np} primaries at ${energies.length} energies, box=${boxNm}nm, cutoff=${ceEV}eV`, 'data');
   const dna = buildDNATarget(DNA_LENGTH_NM, DNA_GRID_N, DNA_SPACING_NM);
   const ssbRng = makeSsbRng();
+  const indirect = !!cfg.indirectSSB;     // extension: score the OH at 1 us too
   const chemCb = backend === 'none' ? null
-    : (rad, radN, nTherm, E) => runChemistryWorker(rad, radN, nTherm, E, log);
+    : (rad, radN, nTherm, E) => runChemistryWorker(rad, radN, nTherm, E, log, undefined, 1,
+      { ohFinal: indirect && E === 10000 });
   const rows = [];
   let results = null;
   for (const E of energies) {
@@ -271,7 +299,7 @@ async function runValidation(cfg) {
 
 module.exports = {
   addon, ESTAR, CHECKPOINTS_NS, CHECKPOINT_LABELS, RXN_LABELS, KARAMITROS_2011, SOLVATION_THRESHOLD,
-  COPY_RESULTS, COPY_DOSE, COPY_RAD, COPY_ALL,
+  COPY_RESULTS, COPY_DOSE, COPY_RAD, COPY_ALL, OPT_IRT_BLOCK, OPT_IRT_BLOCK_LAST, OPT_IRT_FINAL, OPT_PA_LANES,
   buildDNATarget, primaryParams, ensurePipelines, runAtEnergy, runChemistryWorker, makeSsbRng,
   scoreDamageAt10keV, resultsJson, runValidation,
 };

@@ -20,6 +20,11 @@
  *   runIrt(ctx, rad: Float32Array|null, radN, nTherm, E_eV, tCkpt: Float64Array, seed) -> Promise<object>
  *   score(ctx, target, rad: Float32Array|null, radN, lcgState, wantHits) -> Promise<object>
  *   hashCells(ctx, pos4: Float32Array, cellNm, hashBits) -> Promise<object>
+ *   setOption(ctx, option, value) / getOption(ctx, option) -> number   (dna_set_option)
+ *   irtOhFinal(ctx, copy?) -> Promise<{n, oh: Float32Array|null}>     (dna_irt_oh_final)
+ *   irtParticles(ctx) -> Promise<{n, particles: Float64Array}>         (dna_irt_particles)
+ *   scoreOh(ctx, target, rad|null, radN, oh|null, nOh, lcgState, wantHits?) -> Promise<object>
+ *                                                                       (dna_score_oh)
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -153,6 +158,14 @@ struct Req {
     dna_score_out sout;
     uint8_t *hits_copy;
     size_t hits_n;
+    /* final IRT state and indirect scoring */
+    const float *ohin;
+    uint64_t oh_count;
+    dna_indirect_out ind;
+    int want_copy;
+    float *oh_copy;
+    double *parts_copy;
+    uint64_t n_out;
     /* hash */
     const float *pos;
     uint64_t npos;
@@ -179,6 +192,7 @@ static void free_req(napi_env env, Req *r)
     for (int i = 0; i < 6; i++) if (r->keep[i]) napi_delete_reference(env, r->keep[i]);
     free(r->res_copy); free(r->dose_copy); free(r->rad_copy); free(r->hits_copy);
     free(r->key); free(r->start); free(r->order);
+    free(r->oh_copy); free(r->parts_copy);
     free(r);
 }
 
@@ -437,6 +451,32 @@ static napi_value js_run_irt(napi_env env, napi_callback_info info)
 }
 
 /* ---------------------------------------------------------------- scoring */
+/* the DNATarget object of buildDNATarget (src/physics/dna-geometry.ts) into r->tgt; its
+ * four arrays are kept alive in r->keep[0..3] */
+static int parse_target(napi_env env, napi_value T, Req *r)
+{
+    r->tgt.rise = get_num(env, T, "rise", 0.34);
+    r->tgt.r_bb = get_num(env, T, "r_bb", 1.0);
+    r->tgt.spacing_nm = get_num(env, T, "spacing_nm", 150);
+    r->tgt.x0 = get_num(env, T, "x0", 0);
+    r->tgt.L_nm = get_num(env, T, "L_nm", 3000);
+    r->tgt.n_bp_per = (uint32_t)get_num(env, T, "n_bp_per", 0);
+    r->tgt.n_fibers = (uint32_t)get_num(env, T, "n_fibers", 0);
+    r->tgt.n_bp = (uint32_t)get_num(env, T, "n_bp", 0);
+    r->tgt.grid_N = (uint32_t)get_num(env, T, "grid_N", 0);
+    const char *names[4] = {"fy", "fz", "rbb0", "rbb1"};
+    const float **dst[4] = {&r->tgt.fy, &r->tgt.fz, &r->tgt.rbb0, &r->tgt.rbb1};
+    size_t need[4] = {r->tgt.n_fibers, r->tgt.n_fibers, 2 * (size_t)r->tgt.n_bp_per, 2 * (size_t)r->tgt.n_bp_per};
+    for (int i = 0; i < 4; i++) {
+        napi_value a;
+        size_t n = 0;
+        if (napi_get_named_property(env, T, names[i], &a) != napi_ok ||
+            get_typed(env, a, napi_float32_array, (void **)dst[i], &n, &r->keep[i]) || !*dst[i] || n < need[i])
+            return -1;
+    }
+    return 0;
+}
+
 static int exec_score(Req *r)
 {
     int rc = dna_score(r->h->ctx, &r->tgt, r->rad, r->rad_count, &r->lcg, r->want_hits, &r->sout);
@@ -477,28 +517,10 @@ static napi_value js_score(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
-    napi_value T = argv[1];
-    r->tgt.rise = get_num(env, T, "rise", 0.34);
-    r->tgt.r_bb = get_num(env, T, "r_bb", 1.0);
-    r->tgt.spacing_nm = get_num(env, T, "spacing_nm", 150);
-    r->tgt.x0 = get_num(env, T, "x0", 0);
-    r->tgt.L_nm = get_num(env, T, "L_nm", 3000);
-    r->tgt.n_bp_per = (uint32_t)get_num(env, T, "n_bp_per", 0);
-    r->tgt.n_fibers = (uint32_t)get_num(env, T, "n_fibers", 0);
-    r->tgt.n_bp = (uint32_t)get_num(env, T, "n_bp", 0);
-    r->tgt.grid_N = (uint32_t)get_num(env, T, "grid_N", 0);
-    const char *names[4] = {"fy", "fz", "rbb0", "rbb1"};
-    const float **dst[4] = {&r->tgt.fy, &r->tgt.fz, &r->tgt.rbb0, &r->tgt.rbb1};
-    size_t need[4] = {r->tgt.n_fibers, r->tgt.n_fibers, 2 * (size_t)r->tgt.n_bp_per, 2 * (size_t)r->tgt.n_bp_per};
-    for (int i = 0; i < 4; i++) {
-        napi_value a;
-        size_t n = 0;
-        if (napi_get_named_property(env, T, names[i], &a) != napi_ok ||
-            get_typed(env, a, napi_float32_array, (void **)dst[i], &n, &r->keep[i]) || !*dst[i] || n < need[i]) {
-            free_req(env, r);
-            napi_throw_type_error(env, NULL, "target.fy/fz/rbb0/rbb1 must be Float32Arrays of the target's sizes");
-            return NULL;
-        }
+    if (parse_target(env, argv[1], r)) {
+        free_req(env, r);
+        napi_throw_type_error(env, NULL, "target.fy/fz/rbb0/rbb1 must be Float32Arrays of the target's sizes");
+        return NULL;
     }
     double d;
     napi_get_value_double(env, argv[3], &d); r->rad_count = (uint64_t)d;
@@ -568,6 +590,175 @@ static napi_value js_hash_cells(napi_env env, napi_callback_info info)
     return start(env, r, "dna_hash_cells");
 }
 
+/* ---------------------------------------------------------------- options */
+static napi_value js_set_option(napi_env env, napi_callback_info info)
+{
+    size_t argc = 3;
+    napi_value argv[3];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 3) { napi_throw_type_error(env, NULL, "setOption(ctx, option, value)"); return NULL; }
+    Handle *h = get_handle(env, argv[0]);
+    if (!h) return NULL;
+    int32_t opt = 0, val = 0;
+    napi_get_value_int32(env, argv[1], &opt);
+    napi_get_value_int32(env, argv[2], &val);
+    pthread_mutex_lock(&h->mu);
+    int rc = dna_set_option(h->ctx, opt, val);
+    pthread_mutex_unlock(&h->mu);
+    if (rc != DNA_OK) { napi_throw_range_error(env, NULL, dna_last_error()); return NULL; }
+    return NULL;
+}
+
+static napi_value js_get_option(napi_env env, napi_callback_info info)
+{
+    size_t argc = 2;
+    napi_value argv[2], out;
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 2) { napi_throw_type_error(env, NULL, "getOption(ctx, option)"); return NULL; }
+    Handle *h = get_handle(env, argv[0]);
+    if (!h) return NULL;
+    int32_t opt = 0;
+    int val = 0;
+    napi_get_value_int32(env, argv[1], &opt);
+    pthread_mutex_lock(&h->mu);
+    int rc = dna_get_option(h->ctx, opt, &val);
+    pthread_mutex_unlock(&h->mu);
+    if (rc != DNA_OK) { napi_throw_range_error(env, NULL, dna_last_error()); return NULL; }
+    CHECK(env, napi_create_int32(env, val, &out));
+    return out;
+}
+
+/* ---------------------------------------------------------------- final IRT state */
+static int exec_oh_final(Req *r)
+{
+    int rc = dna_irt_oh_final(r->h->ctx, NULL, 0, &r->n_out);
+    if (rc || !r->want_copy) return rc;
+    r->oh_copy = (float *)malloc(r->n_out * 16 + 16);
+    return dna_irt_oh_final(r->h->ctx, r->oh_copy, r->n_out, &r->n_out);
+}
+
+static napi_value done_oh_final(napi_env env, Req *r)
+{
+    napi_value o;
+    napi_create_object(env, &o);
+    set_num(env, o, "n", (double)r->n_out);
+    set_val(env, o, "oh", r->oh_copy ? make_ab_copy(env, r->oh_copy, r->n_out * 16, napi_float32_array, r->n_out * 4) : NULL);
+    return o;
+}
+
+static napi_value js_irt_oh_final(napi_env env, napi_callback_info info)
+{
+    size_t argc = 2;
+    napi_value argv[2];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 1) { napi_throw_type_error(env, NULL, "irtOhFinal(ctx, copy?)"); return NULL; }
+    Handle *h = get_handle(env, argv[0]);
+    if (!h) return NULL;
+    Req *r = (Req *)calloc(1, sizeof(Req));
+    r->h = h;
+    r->want_copy = 1;
+    if (argc > 1) { bool b = true; if (napi_get_value_bool(env, argv[1], &b) == napi_ok) r->want_copy = b; }
+    r->exec = exec_oh_final;
+    r->done = done_oh_final;
+    return start(env, r, "dna_irt_oh_final");
+}
+
+static int exec_particles(Req *r)
+{
+    int rc = dna_irt_particles(r->h->ctx, NULL, 0, &r->n_out);
+    if (rc) return rc;
+    r->parts_copy = (double *)malloc(r->n_out * 48 + 48);
+    return dna_irt_particles(r->h->ctx, r->parts_copy, r->n_out, &r->n_out);
+}
+
+static napi_value done_particles(napi_env env, Req *r)
+{
+    napi_value o;
+    napi_create_object(env, &o);
+    set_num(env, o, "n", (double)r->n_out);
+    set_val(env, o, "particles", make_ab_copy(env, r->parts_copy, r->n_out * 48, napi_float64_array, r->n_out * 6));
+    return o;
+}
+
+static napi_value js_irt_particles(napi_env env, napi_callback_info info)
+{
+    size_t argc = 1;
+    napi_value argv[1];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 1) { napi_throw_type_error(env, NULL, "irtParticles(ctx)"); return NULL; }
+    Handle *h = get_handle(env, argv[0]);
+    if (!h) return NULL;
+    Req *r = (Req *)calloc(1, sizeof(Req));
+    r->h = h;
+    r->exec = exec_particles;
+    r->done = done_particles;
+    return start(env, r, "dna_irt_particles");
+}
+
+/* ---------------------------------------------------------------- indirect + direct scoring */
+static int exec_score_oh(Req *r)
+{
+    int rc = dna_score_oh(r->h->ctx, &r->tgt, r->rad, r->rad_count, r->ohin, r->oh_count, &r->lcg,
+                          r->want_hits, &r->sout, &r->ind);
+    if (rc) return rc;
+    if (r->want_hits && r->sout.hits) {
+        r->hits_n = 2 * (size_t)r->tgt.n_bp;
+        r->hits_copy = (uint8_t *)malloc(r->hits_n + 1);
+        memcpy(r->hits_copy, r->sout.hits, r->hits_n);
+    }
+    return DNA_OK;
+}
+
+static napi_value done_score_oh(napi_env env, Req *r)
+{
+    napi_value o = done_score(env, r), ind;
+    napi_create_object(env, &ind);
+    set_num(env, ind, "ssb0", (double)r->ind.ssb0);
+    set_num(env, ind, "ssb1", (double)r->ind.ssb1);
+    set_num(env, ind, "candidates", (double)r->ind.candidates);
+    set_num(env, ind, "inReach", (double)r->ind.in_reach);
+    set_val(env, o, "indirect", ind);
+    return o;
+}
+
+static napi_value js_score_oh(napi_env env, napi_callback_info info)
+{
+    size_t argc = 8;
+    napi_value argv[8];
+    CHECK(env, napi_get_cb_info(env, info, &argc, argv, NULL, NULL));
+    if (argc < 7) {
+        napi_throw_type_error(env, NULL, "scoreOh(ctx, target, rad|null, radN, oh|null, nOh, lcgState, wantHits?)");
+        return NULL;
+    }
+    Handle *h = get_handle(env, argv[0]);
+    if (!h) return NULL;
+    Req *r = (Req *)calloc(1, sizeof(Req));
+    r->h = h;
+    if (parse_target(env, argv[1], r)) {
+        free_req(env, r);
+        napi_throw_type_error(env, NULL, "target.fy/fz/rbb0/rbb1 must be Float32Arrays of the target's sizes");
+        return NULL;
+    }
+    double d;
+    size_t nr = 0, no = 0;
+    napi_get_value_double(env, argv[3], &d); r->rad_count = (uint64_t)d;
+    napi_get_value_double(env, argv[5], &d); r->oh_count = (uint64_t)d;
+    if (get_typed(env, argv[2], napi_float32_array, (void **)&r->rad, &nr, &r->keep[4]) ||
+        (r->rad && r->rad_count * 4 > nr) ||
+        get_typed(env, argv[4], napi_float32_array, (void **)&r->ohin, &no, &r->keep[5]) ||
+        (r->ohin && r->oh_count * 4 > no)) {
+        free_req(env, r);
+        napi_throw_type_error(env, NULL, "rad / oh must be Float32Arrays of at least 4*radN / 4*nOh floats, or null");
+        return NULL;
+    }
+    napi_get_value_double(env, argv[6], &d); r->lcg = (uint32_t)d;
+    r->want_hits = 0;
+    if (argc > 7) { bool b = false; napi_get_value_bool(env, argv[7], &b); r->want_hits = b; }
+    r->exec = exec_score_oh;
+    r->done = done_score_oh;
+    return start(env, r, "dna_score_oh");
+}
+
 /* ---------------------------------------------------------------- module */
 static napi_value init(napi_env env, napi_value exports)
 {
@@ -580,6 +771,11 @@ static napi_value init(napi_env env, napi_value exports)
         {"runIrt", NULL, js_run_irt, NULL, NULL, NULL, napi_enumerable, NULL},
         {"score", NULL, js_score, NULL, NULL, NULL, napi_enumerable, NULL},
         {"hashCells", NULL, js_hash_cells, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"setOption", NULL, js_set_option, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"getOption", NULL, js_get_option, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"irtOhFinal", NULL, js_irt_oh_final, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"irtParticles", NULL, js_irt_particles, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"scoreOh", NULL, js_score_oh, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     CHECK(env, napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d));
     return exports;
