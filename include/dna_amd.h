@@ -254,6 +254,13 @@ int dna_irt_oh_final(struct dna_ctx *ctx, float *out_pos4, uint64_t cap, uint64_
  * out6 == NULL returns the count only; cap < *n_out gives DNA_ERR_CAPACITY. */
 int dna_irt_particles(struct dna_ctx *ctx, double *out6, uint64_t cap, uint64_t *n_out);
 
+/* The reaction table of the IRT (public/irt-worker.js:233-260, the worker's rxnSigma,
+ * rxnSigmaEff, rxnRc, rxnKdif, rxnKact, rxnProb): out[9][6] = (sigma nm, sigma_eff nm,
+ * rc nm, k_dif and k_act nm^3/ns, contact probability) per reaction in the order of
+ * dna_chem_out.rxn_counts.  The worker reports sigma and rc with each reaction count
+ * (rxn_info, irt-worker.js:690-694).  Needs no device. */
+int dna_irt_reactions(double *out);
+
 /* Captured batches: the instances of dna_run_irt_batch (rad == NULL: entries
  * [inst_off[0], inst_off[n_inst]) of the radicals resident from the last transport),
  * planned once (grouping, buffers, workgroup size) and captured as one hipGraph;

@@ -54,3 +54,16 @@ def test_run_irt_cli(tmp_path):
     assert line["type"] == "result" and len(line["timeline"]) == 7
     o = O.irt(t["rad"], 64, 1000.0, [0.001, 0.01, 0.1, 1, 10, 100, 1000], seed=1)
     assert line["timeline"][-1]["G_OH"] == o.G[6][0]
+    assert [rx["count"] for rx in line["rxn_info"]] == list(o.rxn_counts)
+    assert line["rxn_info"][3]["sigma"] == "0.1715" and line["rxn_info"][5]["rc"] == "-0.711"
+    # the Node CLI (the reference's own runtime for run_irt.cjs) prints the same line
+    import shutil
+    node = shutil.which("node")
+    if node:
+        r2 = subprocess.run([node, os.path.join(ROOT, "webgpu-dna_amd", "js", "run_irt.js"), str(dump), "64", "1000"],
+                            capture_output=True, text=True, timeout=120)
+        assert r2.returncode == 0, r2.stderr
+        js = json.loads(r2.stdout.strip().splitlines()[-1])
+        for d in (line, js):
+            d.pop("t_wall")
+        assert js == line

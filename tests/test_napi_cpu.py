@@ -38,7 +38,7 @@ def test_addon_exports():
                            "console.log(JSON.stringify({k:Object.keys(a), v:a.version()}))"))
     assert set(out["k"]) == {"version", "lastError", "ctxCreate", "ctxDestroy", "runTransport", "runIrt",
                              "score", "hashCells", "setOption", "getOption", "irtOhFinal", "irtParticles",
-                             "scoreOh"}
+                             "scoreOh", "irtReactions"}
     assert "gfx950" in out["v"]
 
 
@@ -114,3 +114,24 @@ def test_snapshot_blob_js_matches_python_bytes(tmp_path):
     assert got == ref
     assert out == {"n": 8, "sn": 2000, "labels": [s["label"] for s in ss],
                    "alive": [int(s["alive"].sum()) for s in ss], "r0": 400}
+
+
+def test_reaction_table_and_to_fixed_match_v8():
+    """The reaction table from the addon equals the CPU restatement's, and the Python host's
+    toFixed gives V8's strings (rxn_info sigma / rc, irt-worker.js:690-694) on the table and
+    on values where rounding is delicate (binary midpoints, negative zero, tiny values)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "webgpu-dna_amd"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from dna_amd.host import to_fixed
+    vals = [0.5, 1.5, 2.5, -0.0, 0.0, 1.0005, 1.0015, 0.125, 0.0625, -0.7109, 2.5e-5, 1e-9, 123.45675,
+            0.1715, 0.71, 1.0117, -0.7112]
+    out = json.loads(_node(f"const a=require({json.dumps(ADDON)}); const t=Array.from(a.irtReactions()); "
+                           f"const v={json.dumps(vals)}.concat([-0]); "
+                           "console.log(JSON.stringify({t, f3:v.map(x=>x.toFixed(3)), f4:v.map(x=>x.toFixed(4))}))"))
+    tab = np.array(out["t"]).reshape(9, 6)
+    assert np.array_equal(tab, O.irt_params())
+    vals = vals + [-0.0]
+    assert [to_fixed(x, 3) for x in vals] == out["f3"]
+    assert [to_fixed(x, 4) for x in vals] == out["f4"]

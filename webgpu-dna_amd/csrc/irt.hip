@@ -1949,6 +1949,20 @@ extern "C" int dna_irt_oh_final(dna_ctx *c, float *out_pos4, uint64_t cap, uint6
     return DNA_OK;
 }
 
+// The reaction table every IRT run uses (the worker's rxnSigma / rxnSigmaEff / rxnRc /
+// rxnKdif / rxnKact / rxnProb, public/irt-worker.js:233-260): out[9][6].  Host only.
+extern "C" int dna_irt_reactions(double *out)
+{
+    if (!out) { set_error("dna_irt_reactions: NULL argument"); return DNA_ERR_ARG; }
+    IrtConst C;
+    build_const(C);
+    for (int r = 0; r < 9; r++) {
+        double *o = out + 6 * r;
+        o[0] = C.sigma[r]; o[1] = C.sigma_eff[r]; o[2] = C.rc[r]; o[3] = C.kdif[r]; o[4] = C.kact[r]; o[5] = C.prob[r];
+    }
+    return DNA_OK;
+}
+
 extern "C" int dna_irt_particles(dna_ctx *c, double *out6, uint64_t cap, uint64_t *n_out)
 {
     if (!c || !n_out) { set_error("dna_irt_particles: NULL argument"); return DNA_ERR_ARG; }

@@ -104,6 +104,27 @@ def run_at_energy(cc, E_eV, np_, box_nm, ce_eV, dna, run_chemistry=None, seed=No
     }
 
 
+def to_fixed(x, digits):
+    """Number.prototype.toFixed: the exact binary value rounded half away from zero to
+    `digits` decimals; zero of either sign prints without a sign."""
+    from decimal import ROUND_HALF_UP, Decimal
+    x = float(x)
+    if x == 0.0:
+        x = 0.0
+    return str(Decimal(x).quantize(Decimal(1).scaleb(-digits), rounding=ROUND_HALF_UP))
+
+
+_RXN_TABLE = None
+
+
+def _rxn_table():
+    global _RXN_TABLE
+    if _RXN_TABLE is None:
+        from .lib import irt_reactions
+        _RXN_TABLE = irt_reactions()
+    return _RXN_TABLE
+
+
 def _chem_result(o, t_wall_ms):
     d = chem_out_to_dict(o)
     timeline = []
@@ -113,7 +134,9 @@ def _chem_result(o, t_wall_ms):
         g = d["G"][k]
         timeline.append({"label": lab, "t_ns": tns, "G_OH": g[0], "G_eaq": g[1], "G_H": g[2],
                          "G_H2O2": g[3], "G_H2": g[4]})
-    rxn_info = [{"label": RXN_LABELS[r], "count": d["rxn_counts"][r]} for r in range(9)]
+    tab = _rxn_table()                        # sigma and rc as the worker reports them (irt-worker.js:690-694)
+    rxn_info = [{"label": RXN_LABELS[r], "count": d["rxn_counts"][r], "sigma": to_fixed(tab[r, 0], 4),
+                 "rc": to_fixed(tab[r, 2], 3)} for r in range(9)]
     return {"chem_n": d["chem_n"], "t_wall": t_wall_ms, "timeline": timeline, "chem_pos_final": None,
             "chem_alive_final": None, "n_reacted": d["n_reacted"], "rxn_info": rxn_info}
 
@@ -156,6 +179,8 @@ def run_chemistry_worker(rad_buf_f32, rad_n, n_therm, E_eV, log=None, max_n=None
                 f"{cp['G_H2O2']:.3f} {cp['G_H2']:.3f}", "data")
         log(f"  IRT worker: {res['n_reacted']} reactions in {res['t_wall'] / 1000:.1f}s "
             f"({res['chem_n']} radicals)", "data")
+        for rx in res["rxn_info"]:                                # worker.ts:62-66
+            log(f"    rxn {rx['label']}: {rx['count']} (σ={rx['sigma']}nm rc={rx['rc']}nm)", "data")
     return res
 
 

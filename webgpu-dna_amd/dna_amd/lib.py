@@ -17,6 +17,14 @@ TABLE_PATH = os.path.join(PKG, "data", "cross_sections.bin")
 _lib = None
 
 
+def irt_reactions():
+    """The IRT's reaction table, [9, 6] float64: sigma, sigma_eff, rc (nm), k_dif, k_act
+    (nm^3/ns), contact probability (dna_irt_reactions; needs no device)."""
+    out = np.empty((9, 6), np.float64)
+    _check("dna_irt_reactions", lib().dna_irt_reactions(_ptr(out)))
+    return out
+
+
 class DnaError(RuntimeError):
     def __init__(self, fn, rc, msg):
         super().__init__(f"{fn} failed (status {rc}): {msg}")
@@ -51,6 +59,7 @@ def lib():
     L.dna_xs_probe.argtypes = [vp, C.c_int, vp, vp, u64, vp]
     L.dna_irt_oh_final.argtypes = [vp, vp, u64, C.POINTER(u64)]
     L.dna_irt_particles.argtypes = [vp, vp, u64, C.POINTER(u64)]
+    L.dna_irt_reactions.argtypes = [vp]
     L.dna_score_oh.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, vp, u64, C.POINTER(u32), C.c_int,
                                C.POINTER(DnaScoreOut), C.POINTER(DnaIndirectOut)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
@@ -80,7 +89,8 @@ EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_
                     "dna_surrogate_destroy", "dna_surrogate_shape", "dna_surrogate_get_params",
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
-                    "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh", "dna_irt_particles"]
+                    "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh", "dna_irt_particles",
+                    "dna_irt_reactions"]
 
 
 def _check(fn, rc):

@@ -160,6 +160,13 @@ async function runAtEnergy(cc, E_eV, np, boxNm, ceEV, dna, runChemistry, seed) {
   };
 }
 
+let rxnTab = null;
+/** The IRT's reaction table (dna_irt_reactions): 9 x (sigma, sigma_eff, rc, k_dif, k_act, p). */
+function rxnTable() {
+  if (!rxnTab) rxnTab = addon.irtReactions();
+  return rxnTab;
+}
+
 /** runChemistryWorker (worker.ts:32-87): the 9-reaction IRT to 1 us on the GPU. */
 async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 1, options = null) {
   const cc = ensurePipelines(cache ? cache.np : 0);
@@ -184,7 +191,10 @@ async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 
   const res = {
     chem_n: o.chemN, t_wall: Date.now() - t0, timeline, chem_pos_final: null, chem_alive_final: null,
     n_reacted: o.nReacted,
-    rxn_info: RXN_LABELS.map((label, r) => ({ label, count: o.rxnCounts[r] })),
+    // sigma and rc as the worker reports them (irt-worker.js:690-694)
+    rxn_info: RXN_LABELS.map((label, r) => ({
+      label, count: o.rxnCounts[r], sigma: rxnTable()[6 * r].toFixed(4), rc: rxnTable()[6 * r + 2].toFixed(3),
+    })),
   };
   if (ohFinal) res.chem_oh_n = (await addon.irtOhFinal(cc.ctx, false)).n;
   if (dump) {
@@ -199,6 +209,7 @@ async function runChemistryWorker(radBuf, radN, nTherm, E_eV, log, maxN, seed = 
           `${cp.G_H2O2.toFixed(3)} ${cp.G_H2.toFixed(3)}`, 'data');
     }
     log(`  IRT worker: ${res.n_reacted} reactions in ${(res.t_wall / 1000).toFixed(1)}s (${res.chem_n} radicals)`, 'data');
+    for (const rx of res.rxn_info) log(`    rxn ${rx.label}: ${rx.count} (σ=${rx.sigma}nm rc=${rx.rc}nm)`, 'data');
   }
   return res;
 }
@@ -300,6 +311,6 @@ async function runValidation(cfg) {
 module.exports = {
   addon, ESTAR, CHECKPOINTS_NS, CHECKPOINT_LABELS, RXN_LABELS, KARAMITROS_2011, SOLVATION_THRESHOLD,
   COPY_RESULTS, COPY_DOSE, COPY_RAD, COPY_ALL, OPT_IRT_BLOCK, OPT_IRT_BLOCK_LAST, OPT_IRT_FINAL, OPT_PA_LANES,
-  buildDNATarget, primaryParams, ensurePipelines, runAtEnergy, runChemistryWorker, makeSsbRng,
+  buildDNATarget, primaryParams, ensurePipelines, runAtEnergy, runChemistryWorker, makeSsbRng, rxnTable,
   scoreDamageAt10keV, resultsJson, runValidation,
 };

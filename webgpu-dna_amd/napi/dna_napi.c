@@ -25,6 +25,7 @@
  *   irtParticles(ctx) -> Promise<{n, particles: Float64Array}>         (dna_irt_particles)
  *   scoreOh(ctx, target, rad|null, radN, oh|null, nOh, lcgState, wantHits?) -> Promise<object>
  *                                                                       (dna_score_oh)
+ *   irtReactions() -> Float64Array(54)                                  (dna_irt_reactions)
  */
 #define NAPI_VERSION 6
 #include <node_api.h>
@@ -628,6 +629,14 @@ static napi_value js_get_option(napi_env env, napi_callback_info info)
     return out;
 }
 
+static napi_value js_irt_reactions(napi_env env, napi_callback_info info)
+{
+    (void)info;
+    double t[54];
+    if (dna_irt_reactions(t) != DNA_OK) { napi_throw_error(env, NULL, dna_last_error()); return NULL; }
+    return make_ab_copy(env, t, sizeof(t), napi_float64_array, 54);
+}
+
 /* ---------------------------------------------------------------- final IRT state */
 static int exec_oh_final(Req *r)
 {
@@ -776,6 +785,7 @@ static napi_value init(napi_env env, napi_value exports)
         {"irtOhFinal", NULL, js_irt_oh_final, NULL, NULL, NULL, napi_enumerable, NULL},
         {"irtParticles", NULL, js_irt_particles, NULL, NULL, NULL, napi_enumerable, NULL},
         {"scoreOh", NULL, js_score_oh, NULL, NULL, NULL, napi_enumerable, NULL},
+        {"irtReactions", NULL, js_irt_reactions, NULL, NULL, NULL, napi_enumerable, NULL},
     };
     CHECK(env, napi_define_properties(env, exports, sizeof(d) / sizeof(d[0]), d));
     return exports;

@@ -7,7 +7,8 @@ tools/run_irt.cjs (usage and output contract of run_irt.cjs:1-79).
 dump.bin: raw little-endian float32 (x, y, z, pid*8+code) per radical, as POSTed to the
 reference's dump server (tools/dump_server.cjs:38-54).  Prints one JSON line
 {"type": "result", "timeline": [...], "n_reacted", "chem_n", "t_wall", "n_repaired",
- "rxn_info"} on stdout and a human-readable summary on stderr.
+ "rxn_info": [{label, count, sigma, rc}]} on stdout (sigma and rc as the worker's toFixed(4) and
+toFixed(3) strings, irt-worker.js:690-697) and a human-readable summary on stderr.
 """
 import json
 import os
@@ -48,6 +49,9 @@ def main(argv):
     for cp in res["timeline"]:
         print(f"  {cp['label']:8s} G(OH)={cp['G_OH']:7.3f}  G(eaq)={cp['G_eaq']:7.3f}  G(H)={cp['G_H']:7.3f}  "
               f"G(H2O2)={cp['G_H2O2']:7.3f}  G(H2)={cp['G_H2']:7.3f}", file=sys.stderr)
+    print("  reactions fired:", file=sys.stderr)
+    for rx in res["rxn_info"]:
+        print(f"    {rx['label']:28s} count={rx['count']:7d}  σ={rx['sigma']}nm  rc={rx['rc']}nm", file=sys.stderr)
     return 0
 
 
