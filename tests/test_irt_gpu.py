@@ -119,3 +119,28 @@ def test_irt_autotune_picks_a_size_and_keeps_results(ctx):
     finally:
         ctx.set_option(DNA_OPT_IRT_BLOCK, 0)
     _same(g, r)
+
+
+def test_irt_empty_and_marker_only_inputs(ctx):
+    """No radicals, and a primary with only an H2 marker (not a worker track,
+    irt-worker.js:387-390, 470), as in the CPU restatement; with DNA_OPT_IRT_FINAL the
+    final OH list and particle table are empty and the indirect pass scores nothing."""
+    from dna_amd import DNATarget
+    from dna_amd.abi import DNA_OPT_IRT_FINAL
+    print("IRT on empty and marker-only inputs", flush=True)
+    for rad in (np.zeros((0, 4), np.float32), np.array([[0, 0, 0, 7]], np.float32)):
+        o = O.irt(rad, 1, 1000.0, CK)
+        ctx.set_option(DNA_OPT_IRT_FINAL, 1)
+        try:
+            g = ctx.irt(rad, rad.shape[0], 1, 1000.0, CK)
+        finally:
+            ctx.set_option(DNA_OPT_IRT_FINAL, 0)
+        assert g.n_tracks == o.n_tracks == 0
+        assert [[g.tally[c][s] for s in range(5)] for c in range(7)] == [[o.tally[c][s] for s in range(5)] for c in range(7)]
+        assert ctx.irt_oh_final().shape == (0, 4) and ctx.irt_particles().shape == (0, 6)
+    dna = DNATarget()
+    rad = np.array([[dna.x0, float(dna.fy[3]) + float(dna.rbb0[0]), float(dna.fz[3]) + float(dna.rbb0[1]), 0.0]], np.float32)
+    s = ctx.score_oh(dna, rad, 0, None, 0, lcg_state=11)
+    d = ctx.score(dna, rad, lcg_state=11)
+    assert s["indirect"] == {"ssb0": 0, "ssb1": 0, "candidates": 0, "in_reach": 0}
+    assert (s["ssb_count"], s["dsb"], s["lcg_state"]) == (d["ssb_count"], d["dsb"], d["lcg_state"])

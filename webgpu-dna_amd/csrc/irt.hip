@@ -1620,15 +1620,21 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
     const uint32_t n_inst = (uint32_t)off.size() - 1;
     int rc = ensure_const(c); if (rc) return rc;
     init_outs(n_inst, off, t_ckpt, nc, outs);
-    if (rad_n == 0) return DNA_OK;
+    c->final_ready = false;                   // this call replaces any earlier final state
+    if (rad_n == 0) {                         // nothing to react: an empty final state
+        if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = seed; }
+        return DNA_OK;
+    }
     DNA_HIP(hipEventRecord(c->ev[5], st));
     IrtGroup G;
     rc = irt_group(c, d_rad, rad_n, off, sel, n_sel, G); if (rc) return rc;
     const uint32_t ntr = (uint32_t)G.tr.size();
     for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
     for (auto &t : G.tr) outs[t.inst].n_tracks++;
-    c->final_ready = false;
-    if (ntr == 0) return DNA_OK;
+    if (ntr == 0) {
+        if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = seed; }
+        return DNA_OK;
+    }
     if (c->irt_final) DNA_HIP(dgrow<uint32_t>(*(void **)&c->d_ntot, c->cap_ntot, ntr));
     int ev_mult = 8;
     for (int attempt = 0; attempt < 4; attempt++) {
@@ -1921,6 +1927,7 @@ extern "C" int dna_irt_oh_final(dna_ctx *c, float *out_pos4, uint64_t cap, uint6
     DNA_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const uint32_t ntr = c->final_ntr;
+    if (ntr == 0) { c->oh_n = 0; *n_out = 0; return DNA_OK; }
     // counts per track, exclusive scan (total at [ntr]), then the ordered write
     DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, 2 * (size_t)ntr + 2 + ((size_t)ntr + 1023) / 1024 + 8));
     uint32_t *cnt = (uint32_t *)c->irt.part, *tmp = cnt + ntr + 2;
@@ -1966,13 +1973,14 @@ extern "C" int dna_irt_reactions(double *out)
 extern "C" int dna_irt_particles(dna_ctx *c, double *out6, uint64_t cap, uint64_t *n_out)
 {
     if (!c || !n_out) { set_error("dna_irt_particles: NULL argument"); return DNA_ERR_ARG; }
-    if (!c->final_ready || !c->final_td) {
+    if (!c->final_ready) {
         set_error("dna_irt_particles: no IRT run with DNA_OPT_IRT_FINAL = 1 since the last IRT call");
         return DNA_ERR_ARG;
     }
     DNA_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     const uint32_t ntr = c->final_ntr;
+    if (ntr == 0) { *n_out = 0; return DNA_OK; }
     DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, 2 * (size_t)ntr + 2 + ((size_t)ntr + 1023) / 1024 + 8));
     uint32_t *off = (uint32_t *)c->irt.part, *tmp = off + ntr + 2;
     DNA_HIP(hipMemcpyAsync(off, c->d_ntot, (size_t)ntr * 4, hipMemcpyDeviceToDevice, st));
