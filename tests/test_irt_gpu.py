@@ -144,3 +144,26 @@ def test_irt_empty_and_marker_only_inputs(ctx):
     d = ctx.score(dna, rad, lcg_state=11)
     assert s["indirect"] == {"ssb0": 0, "ssb1": 0, "candidates": 0, "in_reach": 0}
     assert (s["ssb_count"], s["dsb"], s["lcg_state"]) == (d["ssb_count"], d["dsb"], d["lcg_state"])
+
+
+def test_irt_interleaved_large_primary(ctx):
+    """Entries of three primaries shuffled together, one of them with 9000 entries: the
+    grouping's stable sort keeps input order within each primary at any size, that
+    primary's species-sorted copy lives in global memory (over 3072 particles) and its
+    events outgrow the LDS event list (3504 events against ~2550 LDS slots)."""
+    rng = np.random.default_rng(21)
+
+    def blob(pid, n, R):
+        r = np.zeros((n, 4), np.float32)
+        v = rng.normal(0, 1, (n, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        r[:, :3] = v * R * rng.random(n)[:, None] ** (1 / 3)
+        r[:, 3] = pid * 8 + rng.choice([0, 0, 1, 1, 2, 3, 6], n)
+        return r
+    rad = np.concatenate([blob(7, 9000, 400.0), blob(3, 300, 50.0), blob(11, 500, 80.0)])
+    rad = rad[rng.permutation(rad.shape[0])]
+    print("irt: 3 interleaved primaries, the largest with 9000 entries", flush=True)
+    g = ctx.irt(rad, rad.shape[0], 3, 10000.0, CK, seed=5)
+    o = O.irt(rad, 3, 10000.0, CK, seed=5, n_threads=4)
+    assert g.n_tracks == o.n_tracks == 3 and o.n_events > 3000
+    _same(g, o)

@@ -10,6 +10,7 @@ struct IrtBuffers {
     void   *part = nullptr;     size_t cap_part = 0;      // grouping scratch
     void   *rad_in = nullptr;   size_t cap_rad_in = 0;    // host-supplied radicals
     void   *tally = nullptr;    size_t cap_tally = 0;     // per-instance tallies
+    void   *sort = nullptr;     size_t cap_sort = 0;      // stable key sort of interleaved input
 };
 
 struct ScoreBuffers {

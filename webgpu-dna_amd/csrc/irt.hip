@@ -24,6 +24,7 @@
 #include "dna_ctx.h"
 #include "../../include/dna_irt_math.h"
 #include <vector>
+#include <rocprim/device/device_radix_sort.hpp>
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -73,7 +74,6 @@ struct IrtArgs {
     int nc;
     uint64_t seed;
     const uint64_t *seed_ptr;     // when set, the seed is read here (graph replays)
-    int sort_tracks;              // 1: bucket order is not input order -> sort per track
 };
 #define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 5 + 8)   // tallies, rxn, n_re, n_ev, 3 work counters, 8 phase timers
 
@@ -319,21 +319,31 @@ __global__ void k_changes(const uint32_t *ck, uint32_t m, uint32_t *changes)
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(changes, c);
 }
 
+// (key, index) pairs of every entry for the stable sort of interleaved input: entries that
+// are not particles take the key `past`, after every track
+__global__ void k_sort_input(const uint32_t *kept, const uint32_t *key, uint64_t n, uint32_t past, uint32_t *keys,
+                             uint32_t *vals)
+{
+    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = kept[i] ? key[i] : past;
+    vals[i] = (uint32_t)i;
+}
+
+// input grouped by primary: an entry's rank in its track is its rank among the kept
+// entries minus that of the track's first one
 __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t *key, uint64_t n,
-                         const uint32_t *bstart, const uint32_t *firstj, uint32_t *cursor, int contiguous,
-                         uint32_t *idx)
+                         const uint32_t *bstart, const uint32_t *firstj, uint32_t *idx)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n || !kept[i]) return;
     uint32_t k = key[i];
-    uint32_t r = contiguous ? S[i] - firstj[k] : atomicAdd(&cursor[k], 1u);
-    idx[bstart[k] + r] = (uint32_t)i;
+    idx[bstart[k] + S[i] - firstj[k]] = (uint32_t)i;
 }
 
 // ------------------------------------------------------------ per-track kernel
 #define IRT_BLOCK 256                // largest workgroup; the launch picks 64, 128 or 256
 #define IRT_NT (blockDim.x)
-#define SORT_MAX 8192
 
 __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint64_t ab2)
 {
@@ -849,12 +859,11 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
     }
 }
 
-template <bool SORT, bool DEATH>
+template <bool DEATH>
 __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
 {
     const IrtConst &C = c_irt;
     __shared__ TrackShared sh;
-    __shared__ uint32_t s_sort[SORT ? SORT_MAX : 1];
     __shared__ unsigned long long s_loc[DNA_MAX_CKPT * 5];
     const IrtTrack T = A.tracks[blockIdx.x];          // launch order: largest tracks first
     const uint32_t tid = threadIdx.x;
@@ -865,27 +874,8 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     double *tb = A.tb + T.part_off;
     const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
 
-    // input order of this track's entries (bucket order is already input order unless
-    // the input interleaved primaries; then sort the <= SORT_MAX indices)
+    // the track's entries in input order (the grouping keeps input order within a primary)
     const uint32_t *ord = idx;
-    if (SORT) {
-        uint32_t np2 = 1; while (np2 < n) np2 <<= 1;
-        for (uint32_t i = tid; i < np2; i += IRT_NT) s_sort[i] = i < n ? idx[i] : 0xFFFFFFFFu;
-        __syncthreads();
-        for (uint32_t k = 2; k <= np2; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = tid; i < np2; i += IRT_NT) {
-                    uint32_t l = i ^ j;
-                    if (l > i) {
-                        uint32_t x = s_sort[i], y = s_sort[l];
-                        bool up = (i & k) == 0;
-                        if ((x > y) == up) { s_sort[i] = y; s_sort[l] = x; }
-                    }
-                }
-                __syncthreads();
-            }
-        ord = s_sort;
-    }
     if (tid == 0) { sh.n_ev = 0; sh.ev_lds = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
     for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_NT) s_loc[i] = 0;
     if (tid < 36) {
@@ -1327,7 +1317,7 @@ using namespace dna;
 
 // Scratch of the grouping stage, carved out of one allocation.
 struct GroupScratch {
-    uint32_t *kept, *key, *S, *ck, *cnt, *h2, *firstj, *bstart, *cursor, *idx, *misc;
+    uint32_t *kept, *key, *S, *ck, *cnt, *h2, *firstj, *bstart, *idx, *misc;
 };
 
 
@@ -1388,13 +1378,12 @@ static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     const uint64_t K1 = (uint64_t)h_misc[0] + 1, K = K1 * n_inst;
     G.K1 = K1; G.K = K;
     if (K > (1ull << 28)) { set_error("dna_run_irt: primary-id range too large"); return DNA_ERR_CAPACITY; }
-    // stage 2 scratch: per-key arrays cnt, h2, firstj, bstart, cursor
-    DNA_HIP(dgrow<uint32_t>(c->irt.tracks, c->irt.cap_tracks, 5 * (K + 1)));
+    // stage 2 scratch: per-key arrays cnt, h2, firstj, bstart
+    DNA_HIP(dgrow<uint32_t>(c->irt.tracks, c->irt.cap_tracks, 4 * (K + 1)));
     uint32_t *kb = (uint32_t *)c->irt.tracks;
-    g.cnt = kb; g.h2 = kb + (K + 1); g.firstj = kb + 2 * (K + 1); g.bstart = kb + 3 * (K + 1); g.cursor = kb + 4 * (K + 1);
+    g.cnt = kb; g.h2 = kb + (K + 1); g.firstj = kb + 2 * (K + 1); g.bstart = kb + 3 * (K + 1);
     DNA_HIP(hipMemsetAsync(g.cnt, 0, 2 * (K + 1) * 4, st));
     DNA_HIP(hipMemsetAsync(g.firstj, 0xFF, (K + 1) * 4, st));
-    DNA_HIP(hipMemsetAsync(g.cursor, 0, (K + 1) * 4, st));
     hipLaunchKernelGGL(k_count, dim3(G.gb), dim3(B), 0, st, d_rad, rad_n, d_off, n_inst, (uint32_t)K1,
                        g.kept, g.key, g.cnt, g.h2);
     DNA_HIP(hipGetLastError());
@@ -1437,10 +1426,6 @@ static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     for (auto &t : G.tr) G.maxn_all = t.n > G.maxn_all ? t.n : G.maxn_all;
     if (G.maxn_all >= (1u << 27) - 512u) {       // particle indices carry the species in bits 28-31
         set_error("dna_run_irt: a primary has 2^27 or more entries");
-        return DNA_ERR_CAPACITY;
-    }
-    if (!G.contiguous && G.maxn_all > SORT_MAX) {
-        set_error("dna_run_irt: radicals of a primary are interleaved with others and the primary has more than 8192 entries; group the input by primary");
         return DNA_ERR_CAPACITY;
     }
     return DNA_OK;
@@ -1494,7 +1479,11 @@ static void carve_block(const IrtGroup &G, char *base, size_t tally_bytes, bool 
     b.tt = want_tt ? b.idx + (((uint64_t)G.run + 63) & ~(uint64_t)63) : nullptr;
 }
 
-// upload the track list and bucket the entries (k_bucket) into b.idx
+// upload the track list and put the entries' indices in b.idx, grouped by track (key
+// order) and in input order within a track: for input grouped by primary, k_bucket
+// places each entry at its rank among the kept entries; for interleaved primaries, a
+// stable radix sort of (key, index) over every entry (the entries that are not particles
+// take the key K, past every track) gives the same order for tracks of any size
 static int irt_prepare(dna_ctx *c, const IrtGroup &G, const IrtBufs &b)
 {
     hipStream_t st = c->stream;
@@ -1503,9 +1492,28 @@ static int irt_prepare(dna_ctx *c, const IrtGroup &G, const IrtBufs &b)
     DNA_HIP(hipMemcpyAsync(b.tr, G.launch.data(), ntr * sizeof(IrtTrack), hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemcpyAsync(b.bst, G.bstart.data(), (G.K + 1) * 4, hipMemcpyHostToDevice, st));
     const GroupScratch &g = G.g;
-    hipLaunchKernelGGL(k_bucket, dim3(G.gb), dim3(256), 0, st, g.kept, g.S, g.key, G.rad_n, b.bst, g.firstj, g.cursor,
-                       G.contiguous ? 1 : 0, b.idx);
+    if (G.contiguous) {
+        hipLaunchKernelGGL(k_bucket, dim3(G.gb), dim3(256), 0, st, g.kept, g.S, g.key, G.rad_n, b.bst, g.firstj,
+                           b.idx);
+        DNA_HIP(hipGetLastError());
+        return DNA_OK;
+    }
+    const uint32_t n = (uint32_t)G.rad_n;
+    unsigned bits = 1;
+    while (bits < 32 && (1ull << bits) <= G.K) bits++;
+    size_t tmp_bytes = 0;
+    DNA_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
+                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, n, 0u, bits, st));
+    const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+    DNA_HIP(dgrow<char>(c->irt.sort, c->irt.cap_sort, 4 * arr + tmp_bytes + 256));
+    char *base = (char *)c->irt.sort;
+    uint32_t *kin = (uint32_t *)base, *vin = (uint32_t *)(base + arr);
+    uint32_t *kout = (uint32_t *)(base + 2 * arr), *vout = (uint32_t *)(base + 3 * arr);
+    hipLaunchKernelGGL(k_sort_input, dim3(G.gb), dim3(256), 0, st, g.kept, g.key, G.rad_n, (uint32_t)G.K, kin, vin);
     DNA_HIP(hipGetLastError());
+    DNA_HIP(rocprim::radix_sort_pairs(base + 4 * arr, tmp_bytes, (const uint32_t *)kin, kout, (const uint32_t *)vin,
+                                      vout, n, 0u, bits, st));
+    DNA_HIP(hipMemcpyAsync(b.idx, vout, (size_t)G.run * 4, hipMemcpyDeviceToDevice, st));
     return DNA_OK;
 }
 
@@ -1523,7 +1531,7 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     A.pend = (uint4 *)(b.events + evs * 16);
     A.tally = b.tally; A.track_tally = b.tt; A.overflow = b.ovf;
     for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
-    A.nc = nc; A.seed = seed; A.seed_ptr = nullptr; A.sort_tracks = G.contiguous ? 0 : 1;
+    A.nc = nc; A.seed = seed; A.seed_ptr = nullptr;
     // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in global
     // memory above; the event loop reuses that LDS for alive flags and events
     A.lds_slots = std::min<uint32_t>(G.maxn_all, 3072u);
@@ -1533,12 +1541,9 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
 static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
 {
     const size_t shmem = (size_t)A.lds_slots * 16;
-    // the death-time stores are compiled only into the variants that keep the final state
-    if (A.tdeath) {
-        if (G.contiguous) hipLaunchKernelGGL((k_irt<false, true>), dim3(grid), dim3(block), shmem, st, A);
-        else hipLaunchKernelGGL((k_irt<true, true>), dim3(grid), dim3(block), shmem, st, A);
-    } else if (G.contiguous) hipLaunchKernelGGL((k_irt<false, false>), dim3(grid), dim3(block), shmem, st, A);
-    else hipLaunchKernelGGL((k_irt<true, false>), dim3(grid), dim3(block), shmem, st, A);
+    // the death-time stores are compiled only into the variant that keeps the final state
+    if (A.tdeath) hipLaunchKernelGGL((k_irt<true>), dim3(grid), dim3(block), shmem, st, A);
+    else hipLaunchKernelGGL((k_irt<false>), dim3(grid), dim3(block), shmem, st, A);
 }
 
 // Workgroup size: the option, or autotuned per track-size class (log2 of the mean

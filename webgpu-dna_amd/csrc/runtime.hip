@@ -183,7 +183,7 @@ extern "C" void dna_ctx_destroy(dna_ctx *c)
     void *dev[] = {c->d_tab, c->d_guide, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
                    c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
-                   c->irt.part, c->irt.rad_in, c->irt.tally, c->sc.hitbits, c->sc.flags, c->sc.rank,
+                   c->irt.part, c->irt.rad_in, c->irt.tally, c->irt.sort, c->sc.hitbits, c->sc.flags, c->sc.rank,
                    c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf, c->d_ntot, c->d_oh};
     for (void *p : dev) if (p) hipFree(p);
     void *host[] = {c->h_ctr, c->h_results, c->h_dose, c->h_rad, c->sc.h_hits};
