@@ -5,6 +5,7 @@
  * validation/webgpu-results.json) and logs to stderr.
  *
  *   node run_validation.js [--np N] [--energies 1000,10000] [--box NM] [--chem worker|none]
+ *                          [--indirect-ssb]   (score the OH alive at 1 us too)
  */
 const host = require('./dna_host');
 
@@ -20,6 +21,7 @@ async function main() {
     np, boxNm: parseFloat(arg('--box', '15000')), ceEV: host.SOLVATION_THRESHOLD,
     chemBackend: arg('--chem', 'worker'),
     energies: energies ? energies.split(',').map(Number) : undefined,
+    indirectSSB: process.argv.includes('--indirect-ssb'),
     log: (m, cls) => process.stderr.write(`[${cls || ''}] ${m}\n`),
   };
   const out = await host.runValidation(cfg);
