@@ -218,8 +218,8 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
 
 /* Launch options (no reference counterpart: WebGPU fixes its workgroup sizes in the
  * shaders).  DNA_OPT_IRT_BLOCK: IRT workgroup size, 0 = autotune (default: the first
- * launch of each track-size class times 256/128/64-thread workgroups on up to 3072 of
- * its tracks and keeps the fastest), or 64 / 128 / 256.  DNA_OPT_IRT_BLOCK_LAST (get
+ * launch of each track-size class times 256/128/64-thread workgroups twice on up to 3072
+ * of its tracks and keeps the fastest best-of-two), or 64 / 128 / 256.  DNA_OPT_IRT_BLOCK_LAST (get
  * only): the size the last IRT launch used.  Results do not depend on these options. */
 #define DNA_OPT_IRT_BLOCK       1
 #define DNA_OPT_IRT_BLOCK_LAST  2
