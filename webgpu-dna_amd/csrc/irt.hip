@@ -1704,8 +1704,8 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
                     memcpy(track_tally + (size_t)G.tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
         }
         float ms = 0, kms = 0;
-        hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
-        hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
+        (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);     // timings only
+        (void)hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
         irt_outputs(G, ht.data(), n_therm, E, nc, ms, kms, outs);
         if (c->irt_final) {
             c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = b.tr;
@@ -1815,9 +1815,9 @@ struct dna_irt_plan {
 
 static void plan_release_graph(dna_irt_plan *P)
 {
-    if (P->exec) { hipGraphExecDestroy(P->exec); P->exec = nullptr; }
-    if (P->graph) { hipGraphDestroy(P->graph); P->graph = nullptr; }
-    if (P->buf) { hipFree(P->buf); P->buf = nullptr; }
+    if (P->exec) { (void)hipGraphExecDestroy(P->exec); P->exec = nullptr; }
+    if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+    if (P->buf) { (void)hipFree(P->buf); P->buf = nullptr; }
 }
 
 // (re)build buffers and graph for the current ev_mult / attempt
@@ -1848,12 +1848,19 @@ static int plan_build(dna_irt_plan *P)
     // capture
     DNA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     const size_t words = P->tally_bytes / 8 + 1;
+    // the capture is always ended; a failure while recording is reported after it
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, b.tally, words);
-    hipMemcpyAsync(P->d_seed, P->h_seed, 8, hipMemcpyHostToDevice, st);
-    if (ntr) irt_launch(G, A, ntr, P->block, st);
-    hipMemcpyAsync(P->h_tally, b.tally, P->tally_bytes + 8, hipMemcpyDeviceToHost, st);
+    hipError_t ce = hipGetLastError();
+    if (ce == hipSuccess) ce = hipMemcpyAsync(P->d_seed, P->h_seed, 8, hipMemcpyHostToDevice, st);
+    if (ce == hipSuccess && ntr) { irt_launch(G, A, ntr, P->block, st); ce = hipGetLastError(); }
+    if (ce == hipSuccess) ce = hipMemcpyAsync(P->h_tally, b.tally, P->tally_bytes + 8, hipMemcpyDeviceToHost, st);
     hipError_t e = hipStreamEndCapture(st, &P->graph);
-    if (e != hipSuccess) { set_error(std::string("dna_irt_plan: capture failed: ") + hipGetErrorString(e)); return DNA_ERR_HIP; }
+    if (e == hipSuccess) e = ce;
+    if (e != hipSuccess) {
+        if (P->graph) { (void)hipGraphDestroy(P->graph); P->graph = nullptr; }
+        set_error(std::string("dna_irt_plan: capture failed: ") + hipGetErrorString(e));
+        return DNA_ERR_HIP;
+    }
     DNA_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
     return DNA_OK;
 }
@@ -1861,11 +1868,11 @@ static int plan_build(dna_irt_plan *P)
 extern "C" void dna_irt_plan_destroy(dna_irt_plan *P)
 {
     if (!P) return;
-    hipStreamSynchronize(P->c->stream);
+    (void)hipStreamSynchronize(P->c->stream);
     plan_release_graph(P);
-    if (P->rad) hipFree(P->rad);
-    if (P->h_tally) hipHostFree(P->h_tally);
-    if (P->h_seed) hipHostFree(P->h_seed);
+    if (P->rad) (void)hipFree(P->rad);
+    if (P->h_tally) (void)hipHostFree(P->h_tally);
+    if (P->h_seed) (void)hipHostFree(P->h_seed);
     delete P;
 }
 
@@ -1931,7 +1938,7 @@ extern "C" int dna_irt_plan_run(dna_irt_plan *P, uint64_t seed, dna_chem_out *ou
             continue;
         }
         float ms = 0;
-        hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+        (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
         init_outs(P->G.n_inst, P->off, P->tck.data(), P->nc, out);
         for (uint32_t i = 0; i < P->G.n_inst; i++) out[i].n_tracks = 0;
         for (auto &t : P->G.tr) out[t.inst].n_tracks++;
@@ -2024,7 +2031,7 @@ extern "C" int dna_irt_particles(dna_ctx *c, double *out6, uint64_t cap, uint64_
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out6, d, (size_t)total * 48, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(d);
+    (void)hipFree(d);
     DNA_HIP(e);
     return DNA_OK;
 }

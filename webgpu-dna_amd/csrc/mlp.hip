@@ -332,7 +332,7 @@ static int launch_gemm(hipStream_t st, GemmArgs g)
 
 template <class T> static hipError_t sg_alloc(T *&p, size_t n)
 {
-    if (p) { hipFree(p); p = nullptr; }
+    if (p) { (void)hipFree(p); p = nullptr; }
     return hipMalloc((void **)&p, n * sizeof(T) + 16);
 }
 
@@ -521,10 +521,10 @@ extern "C" int dna_surrogate_create(dna_ctx *ctx, int n_in, int n_hidden, int n_
 extern "C" void dna_surrogate_destroy(dna_surrogate *s)
 {
     if (!s) return;
-    hipStreamSynchronize(s->ctx->stream);
+    (void)hipStreamSynchronize(s->ctx->stream);
     void *ptrs[] = {s->w, s->g, s->m1, s->m2, s->wb, s->norm, s->loss, s->xb, s->acts, s->dh, s->dh2, s->y, s->tn,
                     s->xin, s->tin, s->d_off, s->d_E, s->d_F, s->d_mu, s->d_sig, s->d_fb, s->d_nfb};
-    for (void *p : ptrs) if (p) hipFree(p);
+    for (void *p : ptrs) if (p) (void)hipFree(p);
     delete s;
 }
 
@@ -666,8 +666,8 @@ extern "C" int dna_surrogate_train(dna_surrogate *s, const float *X, const float
         if (e != hipSuccess) { set_error(std::string("dna_surrogate_train: ") + hipGetErrorString(e)); rc = DNA_ERR_HIP; break; }
         last = l / ((double)nb * batch * s->n_out);
     }
-    hipStreamSynchronize(st);
-    hipFree(dX); hipFree(dT); hipFree(dP);
+    (void)hipStreamSynchronize(st);          // errors were checked at each epoch's read-back
+    (void)hipFree(dX); (void)hipFree(dT); (void)hipFree(dP);
     if (rc) return rc;
     if (mean_loss) *mean_loss = last;
     return DNA_OK;
@@ -727,7 +727,7 @@ extern "C" int dna_track_features(dna_surrogate *s, const float *rad, const uint
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) { set_error(std::string("dna_track_features: ") + hipGetErrorString(e)); rc = DNA_ERR_HIP; }
     }
-    if (tmp) { hipStreamSynchronize(st); hipFree(tmp); }
+    if (tmp) { (void)hipStreamSynchronize(st); (void)hipFree(tmp); }
     return rc;
 }
 
@@ -772,9 +772,9 @@ extern "C" int dna_surrogate_predict(dna_surrogate *s, const float *E, uint32_t 
     if (n_fallback) *n_fallback = nf;
     if (t_ms) {
         float a = 0, b = 0, w = 0;
-        hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-        hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-        hipEventElapsedTime(&w, c->ev[0], c->ev[3]);
+        (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+        (void)hipEventElapsedTime(&w, c->ev[0], c->ev[3]);
         t_ms[0] = a; t_ms[1] = b; t_ms[2] = w;
     }
     return DNA_OK;

@@ -45,7 +45,7 @@ template <class T> static hipError_t grow(T *&ptr, size_t &cap, size_t need)
 template <class T> static hipError_t grow_host(T *&ptr, size_t &cap, size_t need)
 {
     if (need <= cap && ptr) return hipSuccess;
-    if (ptr) { hipHostFree(ptr); ptr = nullptr; }
+    if (ptr) { (void)hipHostFree(ptr); ptr = nullptr; }
     size_t n = need < 16 ? 16 : need;
     hipError_t e = hipHostMalloc((void **)&ptr, n * sizeof(T), hipHostMallocDefault);
     if (e == hipSuccess) cap = n; else cap = 0;
@@ -127,7 +127,7 @@ extern "C" int dna_xs_probe(dna_ctx *c, int which, const float *E, const float *
     if (e == hipSuccess) e = launch_xs_probe(c->d_tab, c->d_guide, dE, dr, (uint32_t)n, which, dout, st);
     if (e == hipSuccess) e = hipMemcpyAsync(out, dout, n * wout * 4, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    hipFree(dE); hipFree(dr); hipFree(dout);
+    (void)hipFree(dE); (void)hipFree(dr); (void)hipFree(dout);
     DNA_HIP(e);
     return DNA_OK;
 }
@@ -178,18 +178,18 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
 extern "C" void dna_ctx_destroy(dna_ctx *c)
 {
     if (!c) return;
-    hipSetDevice(c->device);
-    hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);           // teardown: nothing to report to
+    (void)hipStreamSynchronize(c->stream);
     void *dev[] = {c->d_tab, c->d_guide, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
                    c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
                    c->irt.part, c->irt.rad_in, c->irt.tally, c->irt.sort, c->sc.hitbits, c->sc.flags, c->sc.rank,
                    c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf, c->d_ntot, c->d_oh};
-    for (void *p : dev) if (p) hipFree(p);
+    for (void *p : dev) if (p) (void)hipFree(p);
     void *host[] = {c->h_ctr, c->h_results, c->h_dose, c->h_rad, c->sc.h_hits};
-    for (void *p : host) if (p) hipHostFree(p);
-    for (int i = 0; i < 8; i++) hipEventDestroy(c->ev[i]);
-    hipStreamDestroy(c->stream);
+    for (void *p : host) if (p) (void)hipHostFree(p);
+    for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
+    (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -309,9 +309,9 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         out->rad_n_raw = rad_n; out->rad_n_stored = rad_n; out->rad_dropped = 0;
         out->sec_n_raw = sec_raw; out->sec_n = sec_n; out->sec_dropped = 0;
         float ta = 0, tb = 0, tt = 0;
-        hipEventElapsedTime(&ta, c->ev[0], c->ev[1]);
-        hipEventElapsedTime(&tb, c->ev[2], c->ev[3]);
-        hipEventElapsedTime(&tt, c->ev[0], c->ev[4]);
+        (void)hipEventElapsedTime(&ta, c->ev[0], c->ev[1]);
+        (void)hipEventElapsedTime(&tb, c->ev[2], c->ev[3]);
+        (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[4]);
         out->t_prim_ms = ta; out->t_sec_ms = tb; out->t_total_ms = tt;
         c->rad_n = rad_n;
         c->last_n = p.n;

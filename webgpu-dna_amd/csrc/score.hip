@@ -381,7 +381,7 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
         uint64_t nh = 2 * (uint64_t)t->n_bp;
         DNA_HIP(sgrow<uint8_t>(c->sc.hits_u8, c->sc.cap_hits_u8, nh));
         if (nh > c->sc.hcap_hits) {
-            if (c->sc.h_hits) hipHostFree(c->sc.h_hits);
+            if (c->sc.h_hits) (void)hipHostFree(c->sc.h_hits);
             DNA_HIP(hipHostMalloc((void **)&c->sc.h_hits, nh, hipHostMallocDefault));
             c->sc.hcap_hits = nh;
         }
@@ -405,7 +405,7 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
     out->dsb_expected = ex;
     // advance the caller's LCG by the draws consumed (indirect, then direct)
     *lcg_state = lcg_advance(s_dir, h_cnt[1]);
-    float ms = 0; hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+    float ms = 0; (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
     out->t_ms = ms;
     return DNA_OK;
 }

@@ -852,9 +852,12 @@ static int g_num_cu = 0;
 static int num_cu()
 {
     if (!g_num_cu) {
-        int dev = 0; hipGetDevice(&dev);
-        hipDeviceProp_t pr; hipGetDeviceProperties(&pr, dev);
-        g_num_cu = pr.multiProcessorCount > 0 ? pr.multiProcessorCount : 256;
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            g_num_cu = n;
+        else
+            g_num_cu = 256;                   // MI355X: 256 CUs
     }
     return g_num_cu;
 }
