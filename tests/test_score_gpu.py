@@ -65,3 +65,16 @@ def test_hash_cells_exact(ctx):
         kg, sg, og = ctx.hash_cells(pos4, cell, bits)
         ko, so, oo = O.hash_cells(pos4, cell, bits)
         assert np.array_equal(kg, ko) and np.array_equal(sg, so) and np.array_equal(og, oo)
+
+
+def test_hash_cells_empty_and_bad_arguments(ctx):
+    """No points: every bucket is empty (cell_start all zero), as the CPU restatement
+    gives; out-of-range arguments are refused before any GPU work."""
+    empty = np.zeros((0, 4), np.float32)
+    kg, sg, og = ctx.hash_cells(empty, 1.5, 10)
+    ko, so, oo = O.hash_cells(empty, 1.5, 10)
+    assert kg.size == og.size == 0 and np.array_equal(sg, so) and not sg.any()
+    with pytest.raises(Exception, match="bits"):
+        ctx.hash_cells(empty, 1.5, 0)
+    with pytest.raises(Exception, match="cell_nm"):
+        ctx.hash_cells(empty, 0.0, 10)
