@@ -259,7 +259,8 @@ def main():
     value = N_PRIM * args.steps / total
     t, c, sc, n_therm = last
     # roofline of the dominant kernel (IRT, one workgroup per track): algorithmic FP64
-    # work = candidate pair evaluations x FP64 operations per evaluation
+    # work = 9 per candidate pair + 6 per sampled pair + 167 per Brownian re-pairing
+    # (dna_chem_out.algo_flops, counted on the restated worker source)
     irt_ms = sum(k_irt) / len(k_irt)
     flops = c.algo_flops
     achieved = flops / (irt_ms * 1e-3) / 1e12

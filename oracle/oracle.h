@@ -9,7 +9,8 @@
  * functions of include/dna_fmath.h, IRT in FP64, scoring in FP64 with the reference's
  * 32-bit LCG — the same arithmetic the HIP kernels use, so results compare exactly.
  *
- * Parity status of each part (how it is pinned) is listed in tests/ and DESIGN.md.
+ * How each part is pinned (golden vectors, closed forms, transcriptions) is in the
+ * docstrings of the tests/ that check it.
  */
 #ifndef DNA_ORACLE_H
 #define DNA_ORACLE_H

@@ -5,7 +5,7 @@ No counterpart in the reference.  The network (libdna_amd.so, dna_surrogate_*) m
 70 per-track features of dna_track_features to the track's G(t) for 5 species (OH, e-aq,
 H, H2O2, H2) at 16 log-spaced times from 1 ps to 1 us (80 means + 80 log-variances).
 A track whose predicted sigma exceeds a threshold is handed to IRT, which stays the
-fallback and the judge of the surrogate's error.
+fallback and the reference against which the surrogate's error is measured.
 
 Per-track G uses the reference's normalisation with one thermalised primary:
 G = count / (E / 100) (irt-worker.js:670-698 with n_therm = 1); a set of tracks has
