@@ -79,7 +79,11 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
     losses = S.fit(s, d["X"], d["G"])
     t2 = time.perf_counter()
     log(f"surrogate: trained in {t2 - t1:.1f} s, losses {losses}")
-    frac = 0.1
+    # fallback target: the 2% most uncertain training tracks set the gate.  On a separate
+    # 65,536-track validation batch (seed stream 0x5EED0099, neither the held-out nor the
+    # inference batch) the aggregate G error stayed at 0.63-0.66% for targets from 10% to
+    # 1%, while the fallback IRT time fell from 8.5 to 5.1 ms
+    frac = 0.02
     sigma_max = S.calibrate_sigma(s, d["X"], frac)
     out.update({"train_tracks": int(d["X"].shape[0]), "gen_s": round(t1 - t0, 2),
                 "train_s": round(t2 - t1, 2), "nll": [round(x, 4) for x in losses],

@@ -1,9 +1,16 @@
 """Loader and thin object wrapper of libdna_amd.so (the HIP C ABI of include/dna_amd.h).
 
 There is no fallback: if the library or a GPU is missing, calls raise.
+
+Handles still open when the interpreter exits are closed by an atexit hook, surrogates and
+captured plans before their contexts, while this module and the HIP runtime are intact
+(at interpreter shutdown, module globals are cleared in an order that `__del__` cannot
+rely on).
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -15,6 +22,24 @@ LIB_PATH = os.path.join(PKG, "lib", "libdna_amd.so")
 TABLE_PATH = os.path.join(PKG, "data", "cross_sections.bin")
 
 _lib = None
+_open = weakref.WeakSet()         # Context / Surrogate / IrtPlan objects with a live handle
+
+
+@atexit.register
+def _close_open_handles():
+    objs = list(_open)
+    for o in objs:                 # dependants first: they use their context's stream
+        if not isinstance(o, Context):
+            try:
+                o.close()
+            except Exception:
+                pass
+    for o in objs:
+        if isinstance(o, Context):
+            try:
+                o.close()
+            except Exception:
+                pass
 
 
 def irt_reactions():
@@ -110,6 +135,7 @@ class Context:
         h = C.c_void_p()
         _check("dna_ctx_create", L.dna_ctx_create(device, table_path.encode(), C.byref(h)))
         self.h = h
+        _open.add(self)
 
     def close(self):
         if self.h:
@@ -303,6 +329,7 @@ class IrtPlan:
         _check("dna_irt_plan_create", lib().dna_irt_plan_create(ctx.h, _ptr(r), _ptr(off), self.n_inst, _ptr(nt),
                                                                 _ptr(Ee), _ptr(tc), tc.size, C.byref(h)))
         self.h, self.ctx = h, ctx
+        _open.add(self)
 
     def run(self, seed):
         outs = (DnaChemOut * self.n_inst)()
@@ -329,6 +356,7 @@ class Surrogate:
         _check("dna_surrogate_create", lib().dna_surrogate_create(ctx.h, n_in, n_hidden, n_layers,
                                                                   n_out, seed, C.byref(h)))
         self.h, self.ctx = h, ctx
+        _open.add(self)
         a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         npar = C.c_uint64()
         _check("dna_surrogate_shape", lib().dna_surrogate_shape(h, C.byref(a), C.byref(b), C.byref(c),
