@@ -56,9 +56,15 @@ typedef struct dna_params {
     float    ce;           /* tracking cut / solvation threshold (eV)              */
     uint32_t ms;           /* max primary steps (reference 65536)                  */
     float    be;           /* beam energy (eV), used when energies == NULL         */
-    uint32_t max_sec;      /* secondary capacity, 0 = sized automatically          */
+    uint32_t max_sec;      /* secondary cap, 0 = none: only the first max_sec
+                              secondaries in canonical (primary, emission) order are
+                              transported, the rest count in sec_dropped (the
+                              reference's MAX_SEC, dispatch.ts:124-126)              */
     uint32_t vc;           /* dose voxels per axis (reference 128)                 */
-    uint32_t max_rad;      /* radical capacity, 0 = sized automatically            */
+    uint32_t max_rad;      /* radical cap, 0 = none: the first max_rad entries of the
+                              canonical radical buffer are kept, the rest count in
+                              rad_dropped (MAX_RAD, dispatch.ts:190-192).  Device
+                              buffers are sized by the library either way.          */
     float    start_half;   /* start positions uniform in +-start_half (0 = origin) */
     uint32_t dna_enable;   /* kernel-level DNA proximity counter (counters[4])     */
     uint32_t dna_grid_n;   /* fibers per grid side                                 */
@@ -98,8 +104,8 @@ typedef struct dna_transport_out {
     uint32_t dbg[8];             /* [0] sum ni, [1] sum nx, [2] escaped, [3] primaries,
                                     [4] sum(u32(W_transfer*10)) over primary ionisations */
     uint64_t dose_sum;           /* sum of dose[] (exact)                                */
-    uint64_t rad_n_raw, rad_n_stored, rad_dropped;
-    uint64_t sec_n_raw, sec_n, sec_dropped;
+    uint64_t rad_n_raw, rad_n_stored, rad_dropped;  /* emitted, kept, beyond max_rad        */
+    uint64_t sec_n_raw, sec_n, sec_dropped;         /* emitted, transported, beyond max_sec */
     double   t_prim_ms, t_sec_ms, t_total_ms;   /* device time of the two phases        */
 } dna_transport_out;
 
@@ -154,7 +160,10 @@ typedef struct dna_indirect_out {
 /* ------------------------------------------------------------ entry points */
 
 /* Create a context on HIP device `device`, loading the cross-section table file
- * (NULL = the cross_sections.bin shipped next to the library). */
+ * (NULL = the cross_sections.bin shipped next to the library).  dna_ctx_destroy releases
+ * the caller's handle; surrogates and captured IRT plans made on the context keep it alive
+ * until they are destroyed too, so the handles may be destroyed in any order.  A context
+ * must not be used after dna_ctx_destroy. */
 int  dna_ctx_create(int device, const char *table_path, struct dna_ctx **out);
 void dna_ctx_destroy(struct dna_ctx *ctx);
 const char *dna_last_error(void);

@@ -53,6 +53,8 @@ typedef struct oracle_transport_out {
     uint64_t    sec_n;
     uint32_t    counters[8], sec_stats[8], dbg[8];
     uint64_t    dose_sum;
+    uint64_t    rad_n_raw;     /* emitted; rad_n = kept (first max_rad in canonical order) */
+    uint64_t    sec_n_raw;     /* emitted; sec_n = transported (first max_sec)            */
 } oracle_transport_out;
 int  oracle_transport(const dna_params *p, const float *energies, int n_threads,
                       oracle_transport_out *out);

@@ -23,7 +23,8 @@ class OracleTransportOut(C.Structure):
     _fields_ = [("results", C.c_void_p), ("dose", C.c_void_p), ("rad_buf", C.c_void_p),
                 ("rad_n", C.c_uint64), ("sec_n", C.c_uint64),
                 ("counters", C.c_uint32 * 8), ("sec_stats", C.c_uint32 * 8),
-                ("dbg", C.c_uint32 * 8), ("dose_sum", C.c_uint64)]
+                ("dbg", C.c_uint32 * 8), ("dose_sum", C.c_uint64),
+                ("rad_n_raw", C.c_uint64), ("sec_n_raw", C.c_uint64)]
 
 
 def build():
@@ -139,6 +140,7 @@ def transport(params, energies=None, first=0, count=None, n_threads=0):
     rad = np.ctypeslib.as_array(C.cast(o.rad_buf, C.POINTER(C.c_float)),
                                 (max(o.rad_n, 1) * 4,))[: o.rad_n * 4].copy().reshape(-1, 4)
     out = dict(results=res, dose=dose, rad=rad, rad_n=int(o.rad_n), sec_n=int(o.sec_n),
+               rad_n_raw=int(o.rad_n_raw), sec_n_raw=int(o.sec_n_raw),
                counters=np.array(o.counters[:], np.uint32),
                sec_stats=np.array(o.sec_stats[:], np.uint32),
                dbg=np.array(o.dbg[:], np.uint32), dose_sum=int(o.dose_sum))

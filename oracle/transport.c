@@ -672,8 +672,16 @@ int oracle_transport_range(const dna_params *p, const float *energies, uint32_t 
     oracle_seed_primary(first + count, p->seed, seeds);
     sink *sinks = (sink *)calloc(count ? count : 1, sizeof(sink));
     uint32_t *nsecs = (uint32_t *)calloc(count ? count : 1, 4);
+    uint32_t *nkeep = (uint32_t *)calloc(count ? count : 1, 4);
+    secrec **secs = (secrec **)calloc(count ? count : 1, sizeof(secrec *));
 #ifdef _OPENMP
     if (n_threads <= 0) n_threads = omp_get_max_threads();
+#endif
+    /* primaries first, then the secondaries: with a cap (p->max_sec != 0) only the first
+     * max_sec secondaries of the range in canonical (primary, emission) order are
+     * transported, as the reference drops the secondaries beyond MAX_SEC
+     * (primary.wgsl:364, dispatch.ts:124-126) */
+#ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
 #endif
     for (int64_t c = 0; c < (int64_t)count; c++) {
@@ -681,33 +689,52 @@ int oracle_transport_range(const dna_params *p, const float *energies, uint32_t 
         sink *k = &sinks[c];
         k->dose = out->dose;
         uint32_t s[4] = {seeds[4*i], seeds[4*i+1], seeds[4*i+2], seeds[4*i+3]};
-        secrec *secs = NULL; uint32_t nsec = 0, capsec = 0;
+        uint32_t capsec = 0;
         float E0 = energies ? energies[i] : p->be;
-        run_primary(p, &W, i, E0, s, k, &secs, &nsec, &capsec, &out->results[c]);
+        run_primary(p, &W, i, E0, s, k, &secs[c], &nsecs[c], &capsec, &out->results[c]);
+    }
+    uint64_t sec_raw = 0;
+    for (uint32_t c = 0; c < count; c++) {
+        uint64_t keep = nsecs[c];
+        if (p->max_sec) keep = sec_raw >= p->max_sec ? 0 : (p->max_sec - sec_raw < keep ? p->max_sec - sec_raw : keep);
+        nkeep[c] = (uint32_t)keep;
+        sec_raw += nsecs[c];
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(n_threads)
+#endif
+    for (int64_t c = 0; c < (int64_t)count; c++) {
+        uint32_t i = first + (uint32_t)c;
         float pid = (float)((uint32_t)((float)(i + 1u)) - 1u) * 8.0f;   /* secondary.wgsl:88 */
-        for (uint32_t q = 0; q < nsec; q++) run_secondary(p, &W, pid, &secs[q], k);
-        nsecs[c] = nsec;
-        free(secs);
+        for (uint32_t q = 0; q < nkeep[c]; q++) run_secondary(p, &W, pid, &secs[c][q], &sinks[c]);
+        free(secs[c]);
     }
     uint64_t tot = 0;
     for (uint32_t c = 0; c < count; c++) tot += sinks[c].n;
-    out->rad_buf = (float *)malloc(16 * (tot ? tot : 1));
+    /* with p->max_rad != 0 the first max_rad entries in canonical order are kept
+     * (MAX_RAD, dispatch.ts:190-192) */
+    const uint64_t kept = (p->max_rad && tot > p->max_rad) ? p->max_rad : tot;
+    out->rad_buf = (float *)malloc(16 * (kept ? kept : 1));
     uint64_t o = 0;
     for (uint32_t c = 0; c < count; c++) {
         sink *k = &sinks[c];
-        if (k->n) memcpy(out->rad_buf + 4 * o, k->rad, 16 * k->n);
-        o += k->n;
+        uint64_t m = k->n;
+        if (o + m > kept) m = kept - o;
+        if (m) memcpy(out->rad_buf + 4 * o, k->rad, 16 * m);
+        o += m;
         for (int j = 0; j < 8; j++) { out->counters[j] += k->cnt[j]; out->sec_stats[j] += k->sst[j]; out->dbg[j] += k->dbg[j]; }
-        out->sec_n += nsecs[c];
+        out->sec_n += nkeep[c];
         free(k->rad);
     }
-    out->rad_n = tot;
+    out->rad_n = kept;
+    out->rad_n_raw = tot;
+    out->sec_n_raw = sec_raw;
     out->counters[7] = (uint32_t)tot;
-    out->counters[6] = (uint32_t)out->sec_n;
+    out->counters[6] = (uint32_t)sec_raw;
     uint64_t ds = 0;
     for (size_t v = 0; v < nvox; v++) ds += out->dose[v];
     out->dose_sum = ds;
-    free(sinks); free(nsecs); free(seeds);
+    free(sinks); free(nsecs); free(nkeep); free(secs); free(seeds);
     return 0;
 }
 

@@ -91,3 +91,21 @@ def test_run_validation_with_indirect_ssb():
     d0 = base["rows"][0]["damage"]
     print("with indirect:", dmg, "direct only:", d0, flush=True)
     assert d0["ssb_ind"] == 0 and dmg["ssb_ind"] > 0
+
+
+def test_final_state_survives_a_plan(ctx):
+    """Grouping the input of a captured plan leaves the final state kept by
+    DNA_OPT_IRT_FINAL untouched: the OH list is the same before and after planning a batch
+    of more than 1024 entries."""
+    p = default_params(5000.0, 48)
+    print("final state, then a plan over the resident radicals", flush=True)
+    t, n_therm, c = run(ctx, p, seed=7)
+    before = ctx.irt_oh_final()
+    assert t["rad_n"] > 1024
+    plan = ctx.irt_plan(None, [0, t["rad_n"]], [n_therm], [5000.0], CK)
+    try:
+        plan.run(3)
+    finally:
+        plan.close()
+    after = ctx.irt_oh_final()
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))

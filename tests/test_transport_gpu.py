@@ -60,3 +60,26 @@ def test_transport_lanes_per_wavefront(ctx, lanes):
     _cmp(g, O.transport(p), 200)
     with pytest.raises(Exception, match="DNA_OPT_PA_LANES"):
         ctx.set_option(DNA_OPT_PA_LANES, 3)
+
+
+def test_transport_caps(ctx):
+    """max_sec / max_rad as caps with drop counts (the reference's MAX_SEC / MAX_RAD,
+    dispatch.ts:124-126,190-192): the kept secondaries and radicals are the first ones in
+    canonical order, as in the oracle, and an uncapped run afterwards is unaffected."""
+    n = 160
+    p = default_params(2000.0, n)
+    full = O.transport(p)
+    p.max_sec = full["sec_n"] // 2
+    p.max_rad = full["rad_n"] // 3
+    print(f"transport 2 keV x {n} with max_sec={p.max_sec} max_rad={p.max_rad}", flush=True)
+    g = ctx.transport(p)
+    o = O.transport(p)
+    _cmp(g, o, n)
+    assert g["sec_n_raw"] == o["sec_n_raw"] == full["sec_n"]
+    assert g["sec_dropped"] == o["sec_n_raw"] - o["sec_n"] > 0
+    assert g["rad_n_raw"] == o["rad_n_raw"]
+    assert g["rad_n"] == p.max_rad and g["rad_dropped"] == o["rad_n_raw"] - p.max_rad > 0
+    q = default_params(2000.0, n)
+    g2 = ctx.transport(q)
+    _cmp(g2, full, n)
+    assert g2["sec_dropped"] == 0 and g2["rad_dropped"] == 0

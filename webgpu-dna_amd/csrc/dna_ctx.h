@@ -1,5 +1,6 @@
 // dna_ctx.h — the opaque context behind the C ABI.
 #pragma once
+#include <atomic>
 #include <map>
 #include "dna_internal.h"
 
@@ -28,6 +29,11 @@ struct HashBuffers {
 };
 
 struct dna_ctx {
+    // Owners of the context: the handle returned by dna_ctx_create, plus one per surrogate and
+    // per captured IRT plan made on it (they use its stream and buffers).  dna_ctx_destroy
+    // drops the creator's share; the memory goes with the last share, so destroying the
+    // context before its surrogates and plans leaves them working until they are destroyed.
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     dna_tables h_tab;
@@ -51,6 +57,7 @@ struct dna_ctx {
     uint32_t *d_sec_nrad = nullptr;    size_t cap_sec_nrad = 0;
     uint32_t *d_cnt = nullptr;         size_t cap_cnt = 0;
     uint32_t *d_block = nullptr;       size_t cap_block = 0;
+    uint32_t *d_sec_base = nullptr;    size_t cap_sec_base = 0;  // canonical secondary ranks (max_sec)
     uint64_t rad_n = 0;                // canonical radicals resident on the device
     uint32_t last_n = 0;
     // host outputs (pinned)

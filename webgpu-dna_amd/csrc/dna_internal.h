@@ -13,6 +13,11 @@ namespace dna {
 
 void set_error(const std::string &msg);
 
+// Shares of a context (see dna_ctx::refs): surrogates and plans take one at creation and
+// give it back when destroyed; the last release frees the context.
+void ctx_retain(dna_ctx *c);
+void ctx_release(dna_ctx *c);
+
 // Secondary-electron record of the device work queue (64 bytes).
 struct SecRec {
     float4 posE;      // x, y, z, E
@@ -49,11 +54,15 @@ struct TransportDev {
     uint32_t *prim_nrad;        // [n] radical entries emitted by each primary
     uint32_t *sec_nrad;         // [sec_cap] radical entries emitted by each secondary
     uint32_t *ctr;              // [32] global counters, layout below
+    // secondary cap: with sec_base (exclusive scan of prim_nsec) set, Phase B transports only
+    // the secondaries whose canonical rank sec_base[pid] + k - 1 is below sec_keep
+    const uint32_t *sec_base;
+    uint32_t sec_keep;
 };
 // ctr layout
 enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, C_RAD = 7,
        C_SST = 8 /* 8..15 sec_stats */, C_DBG = 16 /* 16..23 dbg */, C_PNEXT = 24, C_SNEXT = 25,
-       C_NCTR = 32 };
+       C_WRAP = 26 /* set when the radical slot counter would pass 2^32 */, C_NCTR = 32 };
 
 uint32_t phase_a_lanes(uint32_t n, int opt);
 size_t transport_slot_headroom(uint32_t n, uint32_t lanes);

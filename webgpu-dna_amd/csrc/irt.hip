@@ -1374,13 +1374,17 @@ static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     G.n_inst = n_inst; G.rad_n = rad_n;
     if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_run_irt: too many radical entries"); return DNA_ERR_CAPACITY; }
     size_t misc_words = 64 + 2 * (n_inst + 1);
-    // stage 1 scratch: kept, key, S (rad_n + 1 each), ck (rad_n), misc
-    size_t need1 = 4 * ((size_t)rad_n + 1) + misc_words;
+    const size_t scan_words = (size_t)rad_n / 1024 + 16;
+    // stage 1 scratch: kept, key, S (rad_n + 1 each), ck (rad_n), misc, the scan's partial
+    // sums.  All of it in irt.part: irt.scratch holds the final state that
+    // DNA_OPT_IRT_FINAL keeps, which grouping for a captured plan must not touch.
+    size_t need1 = 4 * ((size_t)rad_n + 1) + misc_words + scan_words;
     DNA_HIP(dgrow<uint32_t>(c->irt.part, c->irt.cap_part, need1));
     uint32_t *base = (uint32_t *)c->irt.part;
     GroupScratch &g = G.g;
     g.kept = base; g.key = g.kept + rad_n + 1; g.S = g.key + rad_n + 1; g.ck = g.S + rad_n + 1;
     g.misc = g.ck + rad_n + 1;
+    uint32_t *scan_tmp = g.misc + misc_words;
     uint64_t *d_off = (uint64_t *)(((uintptr_t)(g.misc + 16) + 7) & ~(uintptr_t)7);
     DNA_HIP(hipMemcpyAsync(d_off, off.data(), off.size() * 8, hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemsetAsync(g.misc, 0, 16 * 4, st));
@@ -1404,8 +1408,7 @@ static int irt_group(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     DNA_HIP(hipGetLastError());
     // exclusive scan of kept flags -> S (the scan helper scans in place)
     DNA_HIP(hipMemcpyAsync(g.S, g.kept, rad_n * 4, hipMemcpyDeviceToDevice, st));
-    DNA_HIP(dgrow<uint32_t>(c->irt.scratch, c->irt.cap_scratch, rad_n / 1024 + 16));
-    DNA_HIP(exclusive_scan_u32(g.S, (uint32_t)rad_n, (uint32_t *)c->irt.scratch, st));
+    DNA_HIP(exclusive_scan_u32(g.S, (uint32_t)rad_n, scan_tmp, st));
     hipLaunchKernelGGL(k_first, dim3(G.gb), dim3(B), 0, st, g.kept, g.S, g.key, rad_n, g.ck, g.firstj);
     uint32_t m_kept = 0;
     DNA_HIP(hipMemcpyAsync(&m_kept, g.S + rad_n, 4, hipMemcpyDeviceToHost, st));
@@ -1873,7 +1876,9 @@ extern "C" void dna_irt_plan_destroy(dna_irt_plan *P)
     if (P->rad) (void)hipFree(P->rad);
     if (P->h_tally) (void)hipHostFree(P->h_tally);
     if (P->h_seed) (void)hipHostFree(P->h_seed);
+    dna_ctx *c = P->c;
     delete P;
+    ctx_release(c);
 }
 
 extern "C" int dna_irt_plan_create(dna_ctx *c, const float *rad, const uint64_t *inst_off, uint32_t n_inst,
@@ -1893,6 +1898,7 @@ extern "C" int dna_irt_plan_create(dna_ctx *c, const float *rad, const uint64_t 
     DNA_HIP(hipSetDevice(c->device));
     rc = ensure_const(c); if (rc) return rc;
     dna_irt_plan *P = new dna_irt_plan();
+    ctx_retain(c);                                // given back by dna_irt_plan_destroy
     P->c = c; P->off = off; P->nc = n_ckpt;
     P->n_therm.assign(n_therm, n_therm + n_inst);
     P->E.assign(E_eV, E_eV + n_inst);

@@ -463,6 +463,7 @@ extern "C" int dna_surrogate_create(dna_ctx *ctx, int n_in, int n_hidden, int n_
     }
     DNA_HIP(hipSetDevice(ctx->device));
     dna_surrogate *s = new dna_surrogate();
+    ctx_retain(ctx);                              // given back by dna_surrogate_destroy
     s->ctx = ctx; s->n_in = n_in; s->n_in_pad = (n_in + 31) / 32 * 32; s->n_hid = n_hidden;
     s->n_layers = n_layers; s->n_out = n_out;
     size_t off = 0;
@@ -525,7 +526,9 @@ extern "C" void dna_surrogate_destroy(dna_surrogate *s)
     void *ptrs[] = {s->w, s->g, s->m1, s->m2, s->wb, s->norm, s->loss, s->xb, s->acts, s->dh, s->dh2, s->y, s->tn,
                     s->xin, s->tin, s->d_off, s->d_E, s->d_F, s->d_mu, s->d_sig, s->d_fb, s->d_nfb};
     for (void *p : ptrs) if (p) (void)hipFree(p);
+    dna_ctx *c = s->ctx;
     delete s;
+    ctx_release(c);
 }
 
 extern "C" int dna_surrogate_shape(const dna_surrogate *s, int *n_in, int *n_hidden, int *n_layers, int *n_out,

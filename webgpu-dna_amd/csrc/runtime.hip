@@ -175,14 +175,13 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     }
 }
 
-extern "C" void dna_ctx_destroy(dna_ctx *c)
+static void ctx_free(dna_ctx *c)
 {
-    if (!c) return;
     (void)hipSetDevice(c->device);           // teardown: nothing to report to
     (void)hipStreamSynchronize(c->stream);
     void *dev[] = {c->d_tab, c->d_guide, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
-                   c->d_base_p, c->d_cnt, c->d_block, c->irt.scratch, c->irt.events, c->irt.tracks,
+                   c->d_base_p, c->d_cnt, c->d_block, c->d_sec_base, c->irt.scratch, c->irt.events, c->irt.tracks,
                    c->irt.part, c->irt.rad_in, c->irt.tally, c->irt.sort, c->sc.hitbits, c->sc.flags, c->sc.rank,
                    c->sc.hits_u8, c->sc.rad_in, c->sc.tgt, c->hs.buf, c->d_ntot, c->d_oh};
     for (void *p : dev) if (p) (void)hipFree(p);
@@ -191,6 +190,21 @@ extern "C" void dna_ctx_destroy(dna_ctx *c)
     for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+namespace dna {
+void ctx_retain(dna_ctx *c) { c->refs.fetch_add(1, std::memory_order_relaxed); }
+void ctx_release(dna_ctx *c)
+{
+    if (c->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) ctx_free(c);
+}
+}  // namespace dna
+
+// Drops the creator's share: the context is freed now, or when the last surrogate or plan
+// made on it is destroyed.
+extern "C" void dna_ctx_destroy(dna_ctx *c)
+{
+    if (c) ctx_release(c);
 }
 
 __global__ void k_dose_sum(const uint32_t *dose, size_t n, unsigned long long *out)
@@ -235,16 +249,26 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
     double sumE = 0.0;
     if (energies) for (uint32_t i = 0; i < p.n; i++) sumE += energies[i] > 0 ? energies[i] : 0.0;
     else sumE = (double)p.n * (p.be > 0 ? p.be : 0.0);
-    // capacity: reference yields are ~0.13 radical entries and ~0.0145 secondaries per eV;
-    // grow and repeat the (deterministic) run if a count overflows
-    size_t rad_cap = p.max_rad ? p.max_rad : (size_t)(0.2 * sumE) + 65536;
+    // internal capacities: reference yields are ~0.13-0.16 radical entries and ~0.0145
+    // secondaries per eV; grow and repeat the (deterministic) run if a count overflows.  The
+    // caller's max_rad / max_sec are caps on what is kept, applied in canonical order after
+    // the run (the reference's MAX_RAD / MAX_SEC with *_dropped counts, dispatch.ts:124-126,
+    // 190-192), not buffer sizes.
     const uint32_t pa_lanes = phase_a_lanes(p.n, c->pa_lanes);
-    rad_cap += transport_slot_headroom(p.n, pa_lanes);   // chunk tails left unused by each wavefront
-    size_t sec_cap = p.max_sec ? p.max_sec : (size_t)(0.025 * sumE) + 4096;
+    const size_t headroom = transport_slot_headroom(p.n, pa_lanes);   // chunk tails left unused
+    size_t rad_cap = (size_t)(0.2 * sumE) + 65536 + headroom;
+    size_t sec_cap = (size_t)(0.025 * sumE) + 4096;
+    // slot counters are 32-bit: refuse a batch that may need 2^32 or more slots
     const size_t HARD = (size_t)0xFFFFFFF0u;
+    if (0.2 * sumE + 65536.0 + (double)headroom > (double)HARD || 0.025 * sumE > (double)HARD) {
+        set_error("dna_run_transport: the batch may emit 2^32 or more radical slots; split it");
+        return DNA_ERR_CAPACITY;
+    }
     for (int attempt = 0; attempt < 4; attempt++) {
-        if (rad_cap > HARD) rad_cap = HARD;
-        if (sec_cap > HARD) sec_cap = HARD;
+        if (rad_cap > HARD || sec_cap > HARD) {
+            set_error("dna_run_transport: the batch needs 2^32 or more radical or secondary slots; split it");
+            return DNA_ERR_CAPACITY;
+        }
         int rc = ensure_transport(c, p.n, rad_cap, sec_cap, p.vc);
         if (rc) return rc;
         if (energies) DNA_HIP(hipMemcpyAsync(c->d_energies, energies, p.n * sizeof(float), hipMemcpyHostToDevice, st));
@@ -258,6 +282,7 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         d.results = c->d_results; d.dose = c->d_dose; d.rad_raw = c->d_rad_raw; d.rad_key = c->d_rad_key;
         d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
         d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
+        d.sec_base = nullptr; d.sec_keep = 0xFFFFFFFFu;
         DNA_HIP(hipEventRecord(c->ev[0], st));
         DNA_HIP(launch_phase_a(d, pa_lanes, st));
         DNA_HIP(hipEventRecord(c->ev[1], st));
@@ -265,23 +290,38 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         DNA_HIP(hipStreamSynchronize(st));
         uint32_t sec_raw = c->h_ctr[C_SEC];
         if (sec_raw > sec_cap) { sec_cap = (size_t)sec_raw * 5 / 4 + 4096; continue; }
-        uint32_t sec_n = sec_raw;
+        uint32_t sec_n = sec_raw;                   // secondary records written by Phase A
+        uint32_t sec_kept = sec_raw;                // of which Phase B transports
+        if (p.max_sec && sec_raw > p.max_sec) {
+            // the first max_sec secondaries in canonical (primary, emission) order
+            DNA_HIP(grow(c->d_sec_base, c->cap_sec_base, (size_t)p.n + 1));
+            DNA_HIP(hipMemcpyAsync(c->d_sec_base, c->d_prim_nsec, (size_t)p.n * 4, hipMemcpyDeviceToDevice, st));
+            DNA_HIP(exclusive_scan_u32(c->d_sec_base, p.n, c->d_block, st));
+            d.sec_base = c->d_sec_base; d.sec_keep = p.max_sec;
+            sec_kept = p.max_sec;
+        }
         DNA_HIP(hipEventRecord(c->ev[2], st));
         DNA_HIP(launch_phase_b(d, sec_n, st));
         DNA_HIP(hipEventRecord(c->ev[3], st));
         DNA_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, C_NCTR * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
         DNA_HIP(hipStreamSynchronize(st));
+        if (c->h_ctr[C_WRAP]) {
+            set_error("dna_run_transport: the radical slot counter passed 2^32; split the batch");
+            return DNA_ERR_CAPACITY;
+        }
         uint32_t rad_raw = c->h_ctr[C_RAD];
-        if (rad_raw > rad_cap) { rad_cap = (size_t)rad_raw * 5 / 4 + 65536; continue; }
+        if (rad_raw > rad_cap) { rad_cap = (size_t)rad_raw * 5 / 4 + 65536 + headroom; continue; }
         CanonScratch cs; cs.base_p = c->d_base_p; cs.cnt = c->d_cnt; cs.block = c->d_block; cs.block_cap = c->cap_block;
         DNA_HIP(launch_canonicalize(d, p.n, sec_n, rad_raw, cs, c->d_rad, st));
         hipLaunchKernelGGL(k_dose_sum, dim3(1024), dim3(256), 0, st, c->d_dose, v3, c->d_u64);
         DNA_HIP(hipGetLastError());
         DNA_HIP(hipEventRecord(c->ev[4], st));
-        // radicals emitted = the total of the per-emitter counts (slots include unused ones)
-        uint32_t rad_n = 0;
-        DNA_HIP(hipMemcpyAsync(&rad_n, c->d_cnt + (size_t)p.n + sec_n, 4, hipMemcpyDeviceToHost, st));
+        // radicals emitted = the total of the per-emitter counts (slots include unused ones);
+        // a caller's max_rad keeps the first max_rad of them in canonical order
+        uint32_t rad_total = 0;
+        DNA_HIP(hipMemcpyAsync(&rad_total, c->d_cnt + (size_t)p.n + sec_n, 4, hipMemcpyDeviceToHost, st));
         DNA_HIP(hipStreamSynchronize(st));
+        const uint32_t rad_n = (p.max_rad && rad_total > p.max_rad) ? p.max_rad : rad_total;
         // copies
         unsigned long long dsum = 0;
         DNA_HIP(hipMemcpyAsync(&c->h_u64_tmp, c->d_u64, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -304,10 +344,10 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         DNA_HIP(hipStreamSynchronize(st));
         dsum = c->h_u64_tmp;
         for (int i = 0; i < 8; i++) { out->counters[i] = c->h_ctr[i]; out->sec_stats[i] = c->h_ctr[C_SST + i]; out->dbg[i] = c->h_ctr[C_DBG + i]; }
-        out->counters[C_RAD] = rad_n;
+        out->counters[C_RAD] = rad_total;          // emitted, as the reference's counters[7]
         out->dose_sum = dsum;
-        out->rad_n_raw = rad_n; out->rad_n_stored = rad_n; out->rad_dropped = 0;
-        out->sec_n_raw = sec_raw; out->sec_n = sec_n; out->sec_dropped = 0;
+        out->rad_n_raw = rad_total; out->rad_n_stored = rad_n; out->rad_dropped = rad_total - rad_n;
+        out->sec_n_raw = sec_raw; out->sec_n = sec_kept; out->sec_dropped = sec_raw - sec_kept;
         float ta = 0, tb = 0, tt = 0;
         (void)hipEventElapsedTime(&ta, c->ev[0], c->ev[1]);
         (void)hipEventElapsedTime(&tb, c->ev[2], c->ev[3]);

@@ -279,7 +279,13 @@ __device__ __forceinline__ void emit_n(Sink &k, int n, const float4 *e)
         const uint32_t w = threadIdx.x >> 6;
         const uint32_t tot = (uint32_t)(__popcll(m1) + 2 * __popcll(m2) + 3 * __popcll(m3));   // <= 192
         uint32_t f = s_wrad[2 * w + 1];
-        if (f + tot > RAD_CHUNK) { s_wrad[2 * w] = atomicAdd(k.rad_ctr, RAD_CHUNK); f = 0u; }
+        if (f + tot > RAD_CHUNK) {
+            const uint32_t b = atomicAdd(k.rad_ctr, RAD_CHUNK);
+            // the next claim would wrap the 32-bit counter: flag it for the host (which reports
+            // DNA_ERR_CAPACITY); slots at or beyond rad_cap are never written
+            if (b >= 0xFFFFFFFFu - RAD_CHUNK) atomicOr(k.rad_ctr + (C_WRAP - C_RAD), 1u);
+            s_wrad[2 * w] = b; f = 0u;
+        }
         base = s_wrad[2 * w] + f;
         s_wrad[2 * w + 1] = f + tot;
     }
@@ -818,6 +824,19 @@ __device__ __forceinline__ void sec_load(const TransportDev &d, uint32_t q, SecS
     k.seq = 0;
 }
 
+// The next secondary record to transport: records beyond the secondary cap (canonical rank
+// sec_base[pid] + k - 1 >= sec_keep) are claimed, given no radicals and skipped.
+__device__ __forceinline__ uint32_t claim_kept(const TransportDev &d, uint32_t sec_n)
+{
+    for (;;) {
+        const uint32_t q = wave_claim_chunked(&d.ctr[C_SNEXT]);
+        if (q >= sec_n || !d.sec_base) return q;
+        const SecRec &r = d.sec[q];
+        if (d.sec_base[r.pid] + r.k - 1u < d.sec_keep) return q;
+        d.sec_nrad[q] = 0u;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_phase_b(TransportDev d, uint32_t sec_n)
 {
     __shared__ SmallTab S;
@@ -831,14 +850,14 @@ __global__ void __launch_bounds__(256) k_phase_b(TransportDev d, uint32_t sec_n)
     for (int i = 0; i < 6; i++) k.cnt[i] = 0;
     uint32_t sst[5] = {0, 0, 0, 0, 0};
     SecState st;
-    uint32_t q = wave_claim_chunked(&d.ctr[C_SNEXT]);
+    uint32_t q = claim_kept(d, sec_n);
     bool active = q < sec_n;
     if (active) sec_load(d, q, st, k);
     while (active) {
         bool done = sec_step(d, p, S, st, k, sst);
         if (done) {
             d.sec_nrad[q] = k.seq;
-            q = wave_claim_chunked(&d.ctr[C_SNEXT]);
+            q = claim_kept(d, sec_n);
             active = q < sec_n;
             if (active) sec_load(d, q, st, k);
         }

@@ -3,7 +3,7 @@
 Same names, argument meaning and result layouts as the reference, with the GPU work
 routed to libdna_amd.so:
   ensure_pipelines(np, log)                      src/app.ts:65-84
-  run_at_energy(cc, E_eV, np, box_nm, ce_eV, dna, run_chemistry=None)
+  run_at_energy(device, buffers, pipelines, E_eV, np, box_nm, ce_eV, dna, run_chemistry=None)
                                                  src/gpu/dispatch.ts:85-285 -> EnergyResult
   run_chemistry_worker(rad_buf, rad_n, n_therm, E_eV, log=None)
                                                  src/chemistry/worker.ts:32-87 -> ChemResult
@@ -53,17 +53,23 @@ def ensure_pipelines(np_, log=_noop):
         except (DnaError, FileNotFoundError, OSError) as e:
             log(f"GPU init failed: {e}", "err")
             return None
-        _cache = {"ctx": ctx, "np": np_}
+        # the reference's PipeCache {device, buffers, pipelines, np} (app.ts:65-84): the
+        # device is the context; the library sizes its buffers on demand and its kernels are
+        # compiled in, so buffers and pipelines carry nothing the calls need
+        _cache = {"device": {"ctx": ctx}, "buffers": {"np": np_}, "pipelines": {}, "np": np_, "ctx": ctx}
     else:
         _cache["np"] = np_
+        _cache["buffers"] = {"np": np_}
     log(f"Kernels loaded, buffers sized on demand (np={np_})", "data")
     return _cache
 
 
-def run_at_energy(cc, E_eV, np_, box_nm, ce_eV, dna, run_chemistry=None, seed=None):
-    """One energy point: Phase A + B on the GPU, then the host aggregation of
-    dispatch.ts:178-226, then the chemistry callback at 10 keV (dispatch.ts:242-246)."""
-    ctx = cc["ctx"]
+def run_at_energy(device, buffers, pipelines, E_eV, np_, box_nm, ce_eV, dna, run_chemistry=None, seed=None):
+    """runAtEnergy (dispatch.ts:85-95), with the reference's arguments: Phase A + B on the
+    GPU of `device` (from ensure_pipelines), then the host aggregation of
+    dispatch.ts:178-226, then the chemistry callback at 10 keV (dispatch.ts:242-246).
+    `seed` (not in the reference) overrides its floor(E_eV)."""
+    ctx = device["ctx"]
     p = default_params(E_eV, np_, box_nm=box_nm, ce_eV=ce_eV, dna=dna, seed=seed)
     t0 = time.perf_counter()
     t = ctx.transport(p, copy=DNA_COPY_ALL)
@@ -189,12 +195,15 @@ def make_ssb_rng(seed=0x12345678):
     return {"state": seed & 0xFFFFFFFF}
 
 
-def score_damage_at_10kev(cc, dna, chem, rad_buf, rad_n, total_deposited_eV, box_nm, rng, log=_noop,
+def score_damage_at_10kev(dna, chem, rad_buf, rad_n, total_deposited_eV, box_nm, rng, log=_noop,
                           kernel_dna_hits=0):
     """scoreDamageAt10keV (src/app.ts:167-207): direct-event SSB + combineHits + clusterDSB
     on the GPU.  Indirect hits are empty on the worker path (app.ts:178-180) unless the
     chemistry kept its OH at 1 us (run_chemistry_worker(oh_final=True)): then
     scoreIndirectSSB runs first on those, as on the reference's GPU-chemistry path."""
+    cc = ensure_pipelines(_cache["np"] if _cache else 0, log)
+    if cc is None:
+        raise DnaError("score_damage_at_10kev", 2, "no device")
     oh_n = chem.get("chem_oh_n") if chem else None
     if oh_n is not None:
         s = cc["ctx"].score_oh(dna, rad_buf, rad_n, None, oh_n, lcg_state=rng["state"])
@@ -238,10 +247,10 @@ def run_validation(cfg):
     rows, out = [], None
     for E in energies:
         ref = next((e for e in ESTAR if e["E"] == E), {"E": E, "csda": float("nan")})
-        r = run_at_energy(cc, float(E), np_, box_nm, ce_eV, dna, chem_cb)
+        r = run_at_energy(cc["device"], cc["buffers"], cc["pipelines"], float(E), np_, box_nm, ce_eV, dna, chem_cb)
         damage = None
         if r["chem_result"] and r["rad_buf_final"] is not None and E == 10000:
-            damage = score_damage_at_10kev(cc, dna, r["chem_result"], r["rad_buf_final"], r["rad_n_stored"],
+            damage = score_damage_at_10kev(dna, r["chem_result"], r["rad_buf_final"], r["rad_n_stored"],
                                            r["total_deposited_eV"], box_nm, ssb_rng, log, r["kernel_dna_hits"])
         log(f"  E={E} eV: CSDA={r['mean_total']:.1f}nm ({r['mean_total'] / ref['csda']:.2f}x) "
             f"ions/pri={r['mean_ions']:.1f} sec/pri={r['sec_per_pri']:.1f} "

@@ -147,7 +147,8 @@ def generate_snapshot_at_10kev(cfg):
         return host.run_chemistry_worker(rad, rad_n, n_therm, E_eV,
                                          options={"dump_snapshots": True, "snapshot_n": snapshot_n})
 
-    r = host.run_at_energy(cc, 10000, np_, box_nm, ce_eV, dna, run_chemistry=chem)
+    r = host.run_at_energy(cc["device"], cc["buffers"], cc["pipelines"], 10000, np_, box_nm, ce_eV, dna,
+                           run_chemistry=chem)
     chem_r = r["chem_result"]
     if not chem_r or not chem_r.get("snapshots") or r["rad_buf_final"] is None:
         log("Snapshot capture: no chemistry result returned.", "err")

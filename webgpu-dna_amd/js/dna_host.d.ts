@@ -41,7 +41,10 @@ export interface EnergyResult {
 export type LogFn = (msg: string, cls?: string) => void;
 export type ChemCallback = (radBuf: Float32Array, radN: number, nTherm: number, E_eV: number) =>
   Promise<ChemResult | null>;
-export interface PipeCache { ctx: object; np: number; }
+/** the addon context in place of a GPUDevice */
+export interface DnaDevice { ctx: object; }
+/** app.ts PipeCache {device, buffers, pipelines, np}; ctx = device.ctx */
+export interface PipeCache { device: DnaDevice; buffers: { np: number }; pipelines: object; np: number; ctx: object; }
 export interface SsbRng { state: number; }
 export interface DamageResult {
   ssb_dir: number; ssb_ind: number; dsb: number; dsb_per_gy_gbp: number; dsb_expected: number; in_reach: number;
@@ -55,12 +58,13 @@ export interface ValidationConfig {
 
 export function buildDNATarget(L_nm?: number, grid_N?: number, spacing_nm?: number): DNATarget;
 export function ensurePipelines(np: number, log?: LogFn): PipeCache | null;
-export function runAtEnergy(cc: PipeCache, E_eV: number, np: number, boxNm: number, ceEV: number,
-  dna: DNATarget | null, runChemistry?: ChemCallback | null, seed?: number): Promise<EnergyResult>;
+export function runAtEnergy(device: DnaDevice, buffers: { np: number }, pipelines: object, E_eV: number, np: number,
+  boxNm: number, ceEV: number, dna: DNATarget | null, runChemistry?: ChemCallback | null,
+  seed?: number): Promise<EnergyResult>;
 export function runChemistryWorker(radBuf: Float32Array, radN: number, nTherm: number, E_eV: number,
   log?: LogFn, maxN?: number, seed?: number, options?: ChemOptions | null): Promise<ChemResult>;
 export function makeSsbRng(seed?: number): SsbRng;
-export function scoreDamageAt10keV(cc: PipeCache, dna: DNATarget, chem: ChemResult | null,
+export function scoreDamageAt10keV(dna: DNATarget, chem: ChemResult | null,
   radBuf: Float32Array, radN: number, totalDepositedEV: number, boxNm: number, rng: SsbRng, log?: LogFn,
   kernelHits?: number): Promise<DamageResult>;
 export function runValidation(cfg: ValidationConfig): Promise<{ rows: object[]; results: object | null } | null>;

@@ -5,7 +5,7 @@
  *
  * Same names, argument meaning and result layouts as the reference's TypeScript host:
  *   ensurePipelines(np, log)                     src/app.ts:65-84
- *   runAtEnergy(cc, E_eV, np, boxNm, ceEV, dna, runChemistry?)
+ *   runAtEnergy(device, buffers, pipelines, E_eV, np, boxNm, ceEV, dna, runChemistry?)
  *                                                src/gpu/dispatch.ts:85-285 -> EnergyResult
  *   runChemistryWorker(radBuf, radN, nTherm, E_eV, log?, maxN?, seed?, options?)
  *                                                src/chemistry/worker.ts:32-87 -> ChemResult
@@ -108,19 +108,25 @@ function ensurePipelines(np, log = noop) {
       log(`WebGPU-equivalent device not available: ${addon.lastError()}`, 'err');
       return null;
     }
-    cache = { ctx, np };
+    // the reference's PipeCache {device, buffers, pipelines, np} (app.ts:65-84): the device
+    // is the addon context; buffers are sized by the library on demand and its kernels are
+    // compiled in, so buffers and pipelines carry nothing the calls need
+    cache = { device: { ctx }, buffers: { np }, pipelines: {}, np, ctx };
   } else {
     cache.np = np;
+    cache.buffers = { np };
   }
   log(`Kernels loaded, buffers sized on demand (np=${np})`, 'data');
   return cache;
 }
 
-/** runAtEnergy (dispatch.ts:85-285): Phase A + B on the GPU, the host aggregation of
- * dispatch.ts:178-226, then the chemistry callback at 10 keV. */
-async function runAtEnergy(cc, E_eV, np, boxNm, ceEV, dna, runChemistry, seed) {
+/** runAtEnergy (dispatch.ts:85-285), with the reference's arguments: Phase A + B on the
+ * GPU of `device` (the context of ensurePipelines), the host aggregation of
+ * dispatch.ts:178-226, then the chemistry callback at 10 keV.  `seed` (not in the
+ * reference) overrides the reference's floor(E_eV). */
+async function runAtEnergy(device, buffers, pipelines, E_eV, np, boxNm, ceEV, dna, runChemistry, seed) {
   const t0 = Date.now();
-  const t = await addon.runTransport(cc.ctx, primaryParams(E_eV, np, boxNm, ceEV, dna, seed), null, COPY_ALL);
+  const t = await addon.runTransport(device.ctx, primaryParams(E_eV, np, boxNm, ceEV, dna, seed), null, COPY_ALL);
   const f32 = new Float32Array(t.results), u32 = new Uint32Array(t.results);
   const total_deposited_eV = t.doseSum / 100;
   const per100 = total_deposited_eV / 100;
@@ -224,8 +230,10 @@ function makeSsbRng(seed = 0x12345678) {
  * the GPU.  The indirect hits are empty on the worker path (app.ts:178-180) unless the
  * chemistry kept its OH at 1 us (runChemistryWorker options.ohFinal): then scoreIndirectSSB
  * runs first on those, as on the reference's GPU-chemistry path. */
-async function scoreDamageAt10keV(cc, dna, chem, radBuf, radN, totalDepositedEV, boxNm, rng, log = noop,
+async function scoreDamageAt10keV(dna, chem, radBuf, radN, totalDepositedEV, boxNm, rng, log = noop,
   kernelHits = 0) {
+  const cc = ensurePipelines(cache ? cache.np : 0, log);
+  if (!cc) throw new Error('scoreDamageAt10keV: no device');
   const withOh = chem && chem.chem_oh_n != null;
   const s = withOh ? await addon.scoreOh(cc.ctx, dna, radBuf, radN, null, chem.chem_oh_n, rng.state, false)
     : await addon.score(cc.ctx, dna, radBuf, radN, rng.state, false);
@@ -277,6 +285,7 @@ async function runValidation(cfg) {
   const energies = cfg.energies || ESTAR.map((e) => e.E);
   const cc = ensurePipelines(np, log);
   if (!cc) return null;
+  const { device, buffers, pipelines } = cc;
   log(`Running ${np} primaries at ${energies.length} energies, box=${boxNm}nm, cutoff=${ceEV}eV`, 'data');
   const dna = buildDNATarget(DNA_LENGTH_NM, DNA_GRID_N, DNA_SPACING_NM);
   const ssbRng = makeSsbRng();
@@ -288,10 +297,10 @@ async function runValidation(cfg) {
   let results = null;
   for (const E of energies) {
     const ref = ESTAR.find((e) => e.E === E) || { E, csda: NaN };
-    const r = await runAtEnergy(cc, E, np, boxNm, ceEV, dna, chemCb);
+    const r = await runAtEnergy(device, buffers, pipelines, E, np, boxNm, ceEV, dna, chemCb);
     let damage = null;
     if (r.chem_result && r.rad_buf_final && E === 10000) {
-      damage = await scoreDamageAt10keV(cc, dna, r.chem_result, r.rad_buf_final, r.rad_n_stored,
+      damage = await scoreDamageAt10keV(dna, r.chem_result, r.rad_buf_final, r.rad_n_stored,
         r.total_deposited_eV, boxNm, ssbRng, log, r.kernel_dna_hits);
     }
     log(`  E=${E} eV: CSDA=${r.mean_total.toFixed(1)}nm (${(r.mean_total / ref.csda).toFixed(2)}x) ` +

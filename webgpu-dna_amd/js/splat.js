@@ -152,7 +152,7 @@ async function generateSnapshotAt10keV(cfg) {
   const dna = host.buildDNATarget();
   const chem = (rad, radN, nTherm, E) => host.runChemistryWorker(rad, radN, nTherm, E, undefined, undefined, 1,
     { dump_snapshots: true, snapshot_n: snapshotN });
-  const r = await host.runAtEnergy(cc, 10000, np, boxNm, ceEV, dna, chem);
+  const r = await host.runAtEnergy(cc.device, cc.buffers, cc.pipelines, 10000, np, boxNm, ceEV, dna, chem);
   if (!r.chem_result || !r.chem_result.snapshots || !r.rad_buf_final) {
     log('Snapshot capture: no chemistry result returned.', 'err');
     return null;

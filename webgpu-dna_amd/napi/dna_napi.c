@@ -7,7 +7,8 @@
  * WebGPU is asynchronous and IRT runs in a Worker, src/chemistry/worker.ts:32-87).
  * Outputs are fresh ArrayBuffers owned by JS, like the reference's `.slice(0)` copies of
  * mapped ranges (src/gpu/dispatch.ts:122,157,175,237).  Calls on one context are
- * serialised by a per-context mutex.  A missing device resolves ctxCreate to null, as
+ * serialised by a per-context mutex, and each pending call holds a reference on its
+ * context object, so garbage collection cannot destroy a context under queued work.  A missing device resolves ctxCreate to null, as
  * initGPU returns null (src/gpu/device.ts:12-21); any other failure rejects with the
  * library's error text.
  *
@@ -49,7 +50,12 @@ static void handle_finalize(napi_env env, void *data, void *hint)
 {
     (void)env; (void)hint;
     Handle *h = (Handle *)data;
+    /* every pending request holds a reference on the context object, so none is running;
+       the lock orders this after the last request's unlock all the same */
+    pthread_mutex_lock(&h->mu);
     if (h->ctx) dna_ctx_destroy(h->ctx);
+    h->ctx = NULL;
+    pthread_mutex_unlock(&h->mu);
     pthread_mutex_destroy(&h->mu);
     free(h);
 }
@@ -131,6 +137,7 @@ struct Req {
     napi_async_work work;
     napi_deferred deferred;
     Handle *h;
+    napi_ref hold;          /* the context object: not finalized while the request is pending */
     napi_ref keep[6];
     exec_fn exec;
     done_fn done;
@@ -191,6 +198,7 @@ static void exec_cb(napi_env env, void *data)
 static void free_req(napi_env env, Req *r)
 {
     for (int i = 0; i < 6; i++) if (r->keep[i]) napi_delete_reference(env, r->keep[i]);
+    if (r->hold) napi_delete_reference(env, r->hold);
     free(r->res_copy); free(r->dose_copy); free(r->rad_copy); free(r->hits_copy);
     free(r->key); free(r->start); free(r->order);
     free(r->oh_copy); free(r->parts_copy);
@@ -349,6 +357,7 @@ static napi_value js_run_transport(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     napi_value P = argv[1];
     r->p.n = (uint32_t)get_num(env, P, "n", 0);
     r->p.box = (float)get_num(env, P, "box", 15000);
@@ -427,6 +436,7 @@ static napi_value js_run_irt(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     size_t nr = 0, nck = 0;
     double d;
     if (get_typed(env, argv[1], napi_float32_array, (void **)&r->rad, &nr, &r->keep[0]) ||
@@ -518,6 +528,7 @@ static napi_value js_score(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     if (parse_target(env, argv[1], r)) {
         free_req(env, r);
         napi_throw_type_error(env, NULL, "target.fy/fz/rbb0/rbb1 must be Float32Arrays of the target's sizes");
@@ -571,6 +582,7 @@ static napi_value js_hash_cells(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     size_t n = 0;
     if (get_typed(env, argv[1], napi_float32_array, (void **)&r->pos, &n, &r->keep[0]) || !r->pos) {
         free_req(env, r);
@@ -665,6 +677,7 @@ static napi_value js_irt_oh_final(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     r->want_copy = 1;
     if (argc > 1) { bool b = true; if (napi_get_value_bool(env, argv[1], &b) == napi_ok) r->want_copy = b; }
     r->exec = exec_oh_final;
@@ -699,6 +712,7 @@ static napi_value js_irt_particles(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     r->exec = exec_particles;
     r->done = done_particles;
     return start(env, r, "dna_irt_particles");
@@ -743,6 +757,7 @@ static napi_value js_score_oh(napi_env env, napi_callback_info info)
     if (!h) return NULL;
     Req *r = (Req *)calloc(1, sizeof(Req));
     r->h = h;
+    napi_create_reference(env, argv[0], 1, &r->hold);   /* the context outlives the request */
     if (parse_target(env, argv[1], r)) {
         free_req(env, r);
         napi_throw_type_error(env, NULL, "target.fy/fz/rbb0/rbb1 must be Float32Arrays of the target's sizes");
