@@ -208,6 +208,25 @@ int dna_score_oh(struct dna_ctx *ctx, const dna_dna_target *target, const float 
                  const float *oh, uint64_t n_oh, uint32_t *lcg_state, int copy_hits,
                  dna_score_out *out, dna_indirect_out *ind);
 
+/* Allocation-free scoring.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  dna_score on device-resident input, with every scratch buffer in one
+ * fixed array per GPU (about 17 MB, held by each GPU the library runs on until the process
+ * ends); the call allocates nothing.  rad_dev: a device pointer to rad_n (x, y, z, w) float4
+ * entries, or NULL for the radicals resident from the last dna_run_transport.  The combined
+ * hit array is not returned (out->hits = NULL); counts, dsb_expected and the LCG state are
+ * dna_score's.  Limits: targets of at most DNA_SCORE_NOALLOC_MAX_FIBERS fibers,
+ * DNA_SCORE_NOALLOC_MAX_BP_PER bp per fiber and DNA_SCORE_NOALLOC_MAX_BP bp in all
+ * (DNA_ERR_CAPACITY beyond); any rad_n below 2^32 - 16, scored in chunks of
+ * DNA_SCORE_NOALLOC_CHUNK.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (two contexts or threads) race, and the caller must serialise them. */
+#define DNA_SCORE_NOALLOC_CHUNK      (1u << 20)
+#define DNA_SCORE_NOALLOC_MAX_FIBERS 1024u
+#define DNA_SCORE_NOALLOC_MAX_BP_PER 16384u
+#define DNA_SCORE_NOALLOC_MAX_BP     (1u << 22)
+int dna_score_noalloc(struct dna_ctx *ctx, const dna_dna_target *target, const float *rad_dev, uint64_t rad_n,
+                      uint32_t *lcg_state, dna_score_out *out);
+
 /* Spatial-hash binning: for n points (x,y,z,w float4, host pointer) compute
  * cell = floor(p / cell_nm) (f32 division) and key = Teschner hash mod 2^hash_bits;
  * then counting-sort the points by key: cell_start[2^hash_bits + 1] (CSR offsets) and

@@ -85,4 +85,5 @@ struct dna_ctx {
     uint32_t *d_ntot = nullptr;        size_t cap_ntot = 0;    // particles per track at the end
     float4 *d_oh = nullptr;            size_t cap_oh = 0;      // alive OH at 1 us
     uint64_t oh_n = 0;
+    void *na_scratch = nullptr;        // device address of the fixed scoring array (dna_score_noalloc)
 };

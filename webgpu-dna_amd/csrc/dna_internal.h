@@ -18,6 +18,10 @@ void set_error(const std::string &msg);
 void ctx_retain(dna_ctx *c);
 void ctx_release(dna_ctx *c);
 
+// Device address of dna_score_noalloc's fixed array on `device` (the current device),
+// looked up once per device.
+int score_scratch_init(int device, void **ptr);
+
 // Secondary-electron record of the device work queue (64 bytes).
 struct SecRec {
     float4 posE;      // x, y, z, E

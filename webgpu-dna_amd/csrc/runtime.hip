@@ -58,7 +58,7 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     *out = nullptr;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { set_error("no HIP device"); return DNA_ERR_NO_DEVICE; }
-    if (device < 0 || device >= ndev) { set_error("device index out of range"); return DNA_ERR_ARG; }
+    if (device < 0 || device >= ndev || device >= 64) { set_error("device index out of range"); return DNA_ERR_ARG; }
     std::string path = table_path ? std::string(table_path) : default_table_path();
     FILE *f = fopen(path.c_str(), "rb");
     if (!f) { set_error("cannot open table file " + path); return DNA_ERR_TABLES; }
@@ -85,6 +85,10 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     DNA_HIP(hipHostMalloc((void **)&c->h_ctr, C_NCTR * sizeof(uint32_t), hipHostMallocDefault));
     DNA_HIP(hipMalloc((void **)&c->d_u64, 64 * sizeof(unsigned long long)));
     for (int i = 0; i < 8; i++) DNA_HIP(hipEventCreate(&c->ev[i]));
+    {
+        int rc = score_scratch_init(device, &c->na_scratch);
+        if (rc) { ctx_release(c); return rc; }
+    }
     *out = c;
     return DNA_OK;
 }

@@ -69,11 +69,13 @@ __device__ int classify(const ScoreGeom &G, const float *fy, const float *fz, co
 
 // pass 1: visited/classify; flag[i] = 1 if an in-reach event; hitidx[i] = bit; counts.
 // dedupe: the direct scorer's adjacent-duplicate skipping (the indirect scorer visits all).
-__global__ void k_ssb_classify(const float4 *rad, uint64_t n, ScoreGeom G, const float *fy, const float *fz,
+// Entries [i0, n) of rad; flag and hitidx are indexed from i0 (a chunk of the input), while
+// the duplicate walk-back reads the whole input before i0.
+__global__ void k_ssb_classify(const float4 *rad, uint64_t i0, uint64_t n, ScoreGeom G, const float *fy, const float *fz,
                                const float *rbb0, const float *rbb1, uint32_t *flag, uint64_t *hitidx,
                                unsigned long long *cnt /* [0] candidates, [1] in reach */, int dedupe)
 {
-    uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    uint64_t i = i0 + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     unsigned long long cand = 0, reach = 0;
     if (i < n) {
         float4 e = rad[i];
@@ -84,9 +86,9 @@ __global__ void k_ssb_classify(const float4 *rad, uint64_t n, ScoreGeom G, const
             uint64_t h = 0;
             int c = classify(G, fy, fz, rbb0, rbb1, e, &h);
             if (c >= 1) cand = 1;
-            if (c == 2) { reach = 1; f = 1; hitidx[i] = h; }
+            if (c == 2) { reach = 1; f = 1; hitidx[i - i0] = h; }
         }
-        flag[i] = f;
+        flag[i - i0] = f;
     }
     for (int o = 32; o > 0; o >>= 1) { cand += __shfl_xor(cand, o, 64); reach += __shfl_xor(reach, o, 64); }
     if ((threadIdx.x & 63) == 0) {
@@ -103,17 +105,19 @@ __device__ __forceinline__ uint32_t lcg_jump(uint32_t s0, uint32_t m)
     return s;
 }
 
-// pass 2: draws for in-reach events (rank from the exclusive scan of flag); a hit counts
-// when its bit was clear, in cnt[2] (direct) or per strand in cnt[2], cnt[3] (indirect:
-// per_strand, bits at or above n_bp belong to strand 1)
+// pass 2: draws for in-reach events (rank from the exclusive scan of flag, plus *carry, the
+// in-reach events of earlier chunks, when carry is set); a hit counts when its bit was
+// clear, in cnt[2] (direct) or per strand in cnt[2], cnt[3] (indirect: per_strand, bits at
+// or above n_bp belong to strand 1)
 __global__ void k_ssb_hits(const uint32_t *flag, const uint32_t *rank, const uint64_t *hitidx, uint64_t n,
                            uint32_t s0, double p, uint64_t n_bp, int per_strand, uint32_t *bits,
-                           unsigned long long *cnt)
+                           unsigned long long *cnt, const uint32_t *carry)
 {
     uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     unsigned long long add0 = 0, add1 = 0;
     if (i < n && flag[i]) {
-        uint32_t s = lcg_jump(s0, rank[i] + 1u);
+        const uint32_t before = carry ? *carry : 0u;
+        uint32_t s = lcg_jump(s0, before + rank[i] + 1u);
         double u = (double)s / 4294967296.0;
         if (u < p) {
             uint64_t h = hitidx[i];
@@ -247,6 +251,53 @@ static std::mutex g_lcg_mu;                   // contexts on several host thread
 
 using namespace dna;
 
+// the affine maps of 2^k LCG steps in constant memory, once per device
+static int lcg_const_init(int device)
+{
+    std::lock_guard<std::mutex> lk(g_lcg_mu);
+    if (!g_lcg_ready[device]) {
+        uint32_t A[32], Cc[32];
+        uint32_t a = 1664525u, cc = 1013904223u;
+        for (int k = 0; k < 32; k++) { A[k] = a; Cc[k] = cc; cc = a * cc + cc; a = a * a; }
+        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_a), A, sizeof(A)));
+        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_c), Cc, sizeof(Cc)));
+        g_lcg_ready[device] = true;
+    }
+    return DNA_OK;
+}
+
+static ScoreGeom score_geom(const dna_dna_target *t)
+{
+    ScoreGeom G;
+    G.x_half = (t->n_bp_per - 1) * t->rise * 0.5;
+    G.rise_inv = 1 / t->rise;
+    G.grid_off = -((t->grid_N - 1) * t->spacing_nm) * 0.5;
+    G.inv_spacing = 1 / t->spacing_nm;
+    G.r_bb = t->r_bb; G.x0 = t->x0; G.rise = t->rise;
+    G.grid_N = t->grid_N; G.n_bp_per = t->n_bp_per; G.n_bp = t->n_bp; G.n_fibers = t->n_fibers;
+    return G;
+}
+
+static bool target_ok(const dna_dna_target *t)
+{
+    return t->fy && t->fz && t->rbb0 && t->rbb1 && t->n_bp_per != 0 && t->grid_N != 0 &&
+           (uint64_t)t->n_fibers * t->n_bp_per == t->n_bp && t->n_fibers == t->grid_N * t->grid_N;
+}
+
+// closed-form expected DSB (ssb-dsb.ts:398-403), summed over fibers in order, from the
+// per-fiber strand hit counts fk[2f], fk[2f+1]
+static double dsb_expected(const dna_dna_target *t, const uint32_t *fk)
+{
+    double p_pair = 21.0 / (double)t->n_bp_per;
+    if (p_pair > 1) p_pair = 1;
+    double ex = 0;
+    for (uint32_t f = 0; f < t->n_fibers; f++) {
+        uint64_t k0 = fk[2 * f], k1 = fk[2 * f + 1];
+        if (k0 > 0 && k1 > 0) ex += 1 - std::pow(1 - p_pair, (double)(k0 * k1));
+    }
+    return ex;
+}
+
 static uint32_t lcg_advance(uint32_t s, uint64_t m)
 {
     uint32_t a = 1664525u, cc = 1013904223u;
@@ -266,33 +317,15 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
                       uint64_t n_oh, uint32_t *lcg_state, int copy_hits, dna_score_out *out, dna_indirect_out *ind)
 {
     if (!c || !t || !lcg_state || !out) { set_error("dna_score: NULL argument"); return DNA_ERR_ARG; }
-    if (!t->fy || !t->fz || !t->rbb0 || !t->rbb1 || t->n_bp_per == 0 || t->grid_N == 0 ||
-        (uint64_t)t->n_fibers * t->n_bp_per != t->n_bp || t->n_fibers != t->grid_N * t->grid_N) {
-        set_error("dna_score: inconsistent DNA target"); return DNA_ERR_ARG;
-    }
+    if (!target_ok(t)) { set_error("dna_score: inconsistent DNA target"); return DNA_ERR_ARG; }
     if (rad_n >= 0xFFFFFFF0ull || n_oh >= 0xFFFFFFF0ull) { set_error("dna_score: too many entries"); return DNA_ERR_CAPACITY; }
     if (ind && !oh && n_oh > c->oh_n) { set_error("dna_score_oh: oh == NULL but fewer OH are resident"); return DNA_ERR_ARG; }
     DNA_HIP(hipSetDevice(c->device));
     hipStream_t st = c->stream;
     memset(out, 0, sizeof(*out));
     if (ind) memset(ind, 0, sizeof(*ind));
-    std::unique_lock<std::mutex> lk(g_lcg_mu);
-    if (!g_lcg_ready[c->device]) {
-        uint32_t A[32], Cc[32];
-        uint32_t a = 1664525u, cc = 1013904223u;
-        for (int k = 0; k < 32; k++) { A[k] = a; Cc[k] = cc; cc = a * cc + cc; a = a * a; }
-        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_a), A, sizeof(A)));
-        DNA_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_lcg_c), Cc, sizeof(Cc)));
-        g_lcg_ready[c->device] = true;
-    }
-    lk.unlock();
-    ScoreGeom G;
-    G.x_half = (t->n_bp_per - 1) * t->rise * 0.5;
-    G.rise_inv = 1 / t->rise;
-    G.grid_off = -((t->grid_N - 1) * t->spacing_nm) * 0.5;
-    G.inv_spacing = 1 / t->spacing_nm;
-    G.r_bb = t->r_bb; G.x0 = t->x0; G.rise = t->rise;
-    G.grid_N = t->grid_N; G.n_bp_per = t->n_bp_per; G.n_bp = t->n_bp; G.n_fibers = t->n_fibers;
+    int rc = lcg_const_init(c->device); if (rc) return rc;
+    const ScoreGeom G = score_geom(t);
     // target arrays
     size_t ntg = 2 * (size_t)t->n_fibers + 4 * (size_t)t->n_bp_per;
     DNA_HIP(sgrow<float>(c->sc.tgt, c->sc.cap_tgt, ntg));
@@ -341,12 +374,12 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
         }
         if (n_oh) {
             uint32_t gi = (uint32_t)((n_oh + B - 1) / B);
-            hipLaunchKernelGGL(k_ssb_classify, dim3(gi), dim3(B), 0, st, d_oh, n_oh, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx,
-                               icnt, 0);
+            hipLaunchKernelGGL(k_ssb_classify, dim3(gi), dim3(B), 0, st, d_oh, (uint64_t)0, n_oh, G, d_fy, d_fz, d_r0, d_r1,
+                               flag, hitidx, icnt, 0);
             DNA_HIP(hipMemcpyAsync(rank, flag, n_oh * 4, hipMemcpyDeviceToDevice, st));
             DNA_HIP(exclusive_scan_u32(rank, (uint32_t)n_oh, scan_tmp, st));
             hipLaunchKernelGGL(k_ssb_hits, dim3(gi), dim3(B), 0, st, flag, rank, hitidx, n_oh, *lcg_state, 0.4,
-                               (uint64_t)t->n_bp, 1, ibits, icnt);
+                               (uint64_t)t->n_bp, 1, ibits, icnt, (const uint32_t *)nullptr);
         }
         unsigned long long hi[4] = {0, 0, 0, 0};
         DNA_HIP(hipMemcpyAsync(hi, icnt, sizeof(hi), hipMemcpyDeviceToHost, st));
@@ -357,12 +390,12 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
     }
     uint32_t gb = (uint32_t)((rad_n + B - 1) / B);
     if (rad_n) {
-        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, rad_n, G, d_fy, d_fz, d_r0, d_r1, flag, hitidx,
-                           cnt, 1);
+        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, (uint64_t)0, rad_n, G, d_fy, d_fz, d_r0, d_r1,
+                           flag, hitidx, cnt, 1);
         DNA_HIP(hipMemcpyAsync(rank, flag, rad_n * 4, hipMemcpyDeviceToDevice, st));
         DNA_HIP(exclusive_scan_u32(rank, (uint32_t)rad_n, scan_tmp, st));
         hipLaunchKernelGGL(k_ssb_hits, dim3(gb), dim3(B), 0, st, flag, rank, hitidx, rad_n, s_dir, 0.15,
-                           (uint64_t)t->n_bp, 0, bits, cnt);
+                           (uint64_t)t->n_bp, 0, bits, cnt, (const uint32_t *)nullptr);
     }
     if (ind) {
         hipLaunchKernelGGL(k_or_bits, dim3((uint32_t)((nwords + B - 1) / B)), dim3(B), 0, st, bits, ibits, nwords);
@@ -394,15 +427,7 @@ static int score_impl(dna_ctx *c, const dna_dna_target *t, const float *rad, uin
     if (fk[2 * t->n_fibers]) { set_error("dna_score: more than 4096 hits on one strand of one fiber"); return DNA_ERR_CAPACITY; }
     out->candidates = h_cnt[0]; out->in_reach = h_cnt[1]; out->ssb_count = h_cnt[2];
     out->dsb = h_cnt[3]; out->ssb0 = h_cnt[4]; out->ssb1 = h_cnt[5];
-    // closed-form expected DSB (ssb-dsb.ts:398-403), summed over fibers in order
-    double p_pair = 21.0 / (double)t->n_bp_per;
-    if (p_pair > 1) p_pair = 1;
-    double ex = 0;
-    for (uint32_t f = 0; f < t->n_fibers; f++) {
-        uint64_t k0 = fk[2 * f], k1 = fk[2 * f + 1];
-        if (k0 > 0 && k1 > 0) ex += 1 - std::pow(1 - p_pair, (double)(k0 * k1));
-    }
-    out->dsb_expected = ex;
+    out->dsb_expected = dsb_expected(t, fk.data());
     // advance the caller's LCG by the draws consumed (indirect, then direct)
     *lcg_state = lcg_advance(s_dir, h_cnt[1]);
     float ms = 0; (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
@@ -450,5 +475,130 @@ extern "C" int dna_hash_cells(dna_ctx *c, const float *pos4, uint64_t n, float c
     DNA_HIP(hipMemcpyAsync(cell_start_out, d_start, ((size_t)nb + 1) * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipMemcpyAsync(order_out, d_ord, n * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipStreamSynchronize(st));
+    return DNA_OK;
+}
+
+// ------------------------------------------------------------ scoring without allocation
+// dna_score_noalloc keeps every scratch buffer of the direct scoring in one fixed array per
+// GPU: the target copy, the hit bitmap, counters, and per-radical flags, ranks and hit
+// indices for one chunk of DNA_SCORE_NOALLOC_CHUNK radicals.  The input is scored chunk by
+// chunk: the duplicate walk-back reads the input itself, and the in-reach events of earlier
+// chunks (a device-side carry) offset each chunk's LCG draws, so the result equals
+// dna_score's.  Host code uses the array only through the address looked up once per
+// device (score_scratch_init, from dna_ctx_create), never through its name.
+#define NA_TGT_BYTES (((2u * DNA_SCORE_NOALLOC_MAX_FIBERS + 4u * DNA_SCORE_NOALLOC_MAX_BP_PER) * 4u + 255u) & ~255u)
+#define NA_BITS_BYTES ((2ull * DNA_SCORE_NOALLOC_MAX_BP + 63) / 64 * 8)
+#define NA_MISC_BYTES 16384u            // counters (8 u64) at 0, carry at 64, fiber counts at 128
+#define NA_FLAG_BYTES ((((size_t)DNA_SCORE_NOALLOC_CHUNK + 1) * 4 + 255) & ~(size_t)255)
+#define NA_TMP_BYTES ((((size_t)DNA_SCORE_NOALLOC_CHUNK / 1024 + 16) * 4 + 255) & ~(size_t)255)
+#define NA_HIT_BYTES ((((size_t)DNA_SCORE_NOALLOC_CHUNK + 1) * 8 + 255) & ~(size_t)255)
+#define NA_SCRATCH_BYTES (NA_TGT_BYTES + NA_BITS_BYTES + NA_MISC_BYTES + 2 * NA_FLAG_BYTES + NA_TMP_BYTES + NA_HIT_BYTES)
+static_assert(128 + 4 * (2 * DNA_SCORE_NOALLOC_MAX_FIBERS + 1) <= NA_MISC_BYTES, "fiber counts fit");
+
+static __device__ __align__(256) unsigned char scratch[NA_SCRATCH_BYTES];
+
+namespace dna {
+static std::mutex g_na_mu;
+static void *g_na_ptr[64];                    // the array's address on each device
+
+// Look up the array's device address on `device` (current), once per device.
+int score_scratch_init(int device, void **ptr)
+{
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("scoring scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return DNA_ERR_HIP;
+        }
+        g_na_ptr[device] = p;
+    }
+    *ptr = g_na_ptr[device];
+    return DNA_OK;
+}
+}  // namespace dna
+
+__global__ void k_fill_u32(uint32_t *p, size_t n, uint32_t v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_carry_add(uint32_t *carry, const uint32_t *chunk_total) { *carry += *chunk_total; }
+
+extern "C" int dna_score_noalloc(dna_ctx *c, const dna_dna_target *t, const float *rad_dev, uint64_t rad_n,
+                                 uint32_t *lcg_state, dna_score_out *out)
+{
+    if (!c || !t || !lcg_state || !out) { set_error("dna_score_noalloc: NULL argument"); return DNA_ERR_ARG; }
+    if (!target_ok(t)) { set_error("dna_score_noalloc: inconsistent DNA target"); return DNA_ERR_ARG; }
+    if (t->n_fibers > DNA_SCORE_NOALLOC_MAX_FIBERS || t->n_bp_per > DNA_SCORE_NOALLOC_MAX_BP_PER ||
+        t->n_bp > DNA_SCORE_NOALLOC_MAX_BP) {
+        set_error("dna_score_noalloc: the target exceeds the fixed scratch (at most 1024 fibers, 16384 bp per "
+                  "fiber, 2^22 bp in all)");
+        return DNA_ERR_CAPACITY;
+    }
+    if (rad_n >= 0xFFFFFFF0ull) { set_error("dna_score_noalloc: too many entries"); return DNA_ERR_CAPACITY; }
+    if (!rad_dev && rad_n > c->rad_n) {
+        set_error("dna_score_noalloc: rad_dev == NULL but fewer radicals are resident"); return DNA_ERR_ARG;
+    }
+    if (!c->na_scratch) { set_error("dna_score_noalloc: no scoring scratch on this device"); return DNA_ERR_HIP; }
+    DNA_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    memset(out, 0, sizeof(*out));
+    int rc = lcg_const_init(c->device); if (rc) return rc;
+    const ScoreGeom G = score_geom(t);
+    // carve the array (at its device address)
+    unsigned char *b = (unsigned char *)c->na_scratch;
+    float *d_fy = (float *)b, *d_fz = d_fy + t->n_fibers, *d_r0 = d_fz + t->n_fibers, *d_r1 = d_r0 + 2 * (size_t)t->n_bp_per;
+    uint32_t *bits = (uint32_t *)(b + NA_TGT_BYTES);
+    unsigned char *misc = b + NA_TGT_BYTES + NA_BITS_BYTES;
+    unsigned long long *cnt = (unsigned long long *)misc;
+    uint32_t *carry = (uint32_t *)(misc + 64), *fiber_k = (uint32_t *)(misc + 128);
+    uint32_t *flag = (uint32_t *)(misc + NA_MISC_BYTES), *rank = (uint32_t *)((unsigned char *)flag + NA_FLAG_BYTES);
+    uint32_t *scan_tmp = (uint32_t *)((unsigned char *)rank + NA_FLAG_BYTES);
+    uint64_t *hitidx = (uint64_t *)((unsigned char *)scan_tmp + NA_TMP_BYTES);
+    const float4 *d_rad = rad_dev ? (const float4 *)rad_dev : c->d_rad;
+    DNA_HIP(hipEventRecord(c->ev[5], st));
+    DNA_HIP(hipMemcpyAsync(d_fy, t->fy, t->n_fibers * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_fz, t->fz, t->n_fibers * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_r0, t->rbb0, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
+    DNA_HIP(hipMemcpyAsync(d_r1, t->rbb1, 2 * (size_t)t->n_bp_per * 4, hipMemcpyHostToDevice, st));
+    // the array keeps the last call's contents: clear the bitmap, counters, carry and fiber
+    // counts with a fill kernel
+    const uint64_t nwords = (2 * (uint64_t)t->n_bp + 31) / 32;
+    hipLaunchKernelGGL(k_fill_u32, dim3(256), dim3(256), 0, st, bits, (size_t)nwords, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(8), dim3(256), 0, st, (uint32_t *)misc, (size_t)NA_MISC_BYTES / 4, 0u);
+    DNA_HIP(hipGetLastError());
+    const uint32_t B = 256, CH = DNA_SCORE_NOALLOC_CHUNK;
+    const uint32_t s0 = *lcg_state;
+    for (uint64_t off = 0; off < rad_n; off += CH) {
+        const uint32_t nc = (uint32_t)(rad_n - off < CH ? rad_n - off : CH);
+        const uint32_t gb = (nc + B - 1) / B;
+        hipLaunchKernelGGL(k_ssb_classify, dim3(gb), dim3(B), 0, st, d_rad, off, off + nc, G, d_fy, d_fz, d_r0, d_r1,
+                           flag, hitidx, cnt, 1);
+        DNA_HIP(hipMemcpyAsync(rank, flag, (size_t)nc * 4, hipMemcpyDeviceToDevice, st));
+        DNA_HIP(exclusive_scan_u32(rank, nc, scan_tmp, st));
+        hipLaunchKernelGGL(k_ssb_hits, dim3(gb), dim3(B), 0, st, flag, rank, hitidx, (uint64_t)nc, s0, 0.15,
+                           (uint64_t)t->n_bp, 0, bits, cnt, (const uint32_t *)carry);
+        hipLaunchKernelGGL(k_carry_add, dim3(1), dim3(1), 0, st, carry, rank + nc);
+        DNA_HIP(hipGetLastError());
+    }
+    uint32_t *ovf = fiber_k + 2 * t->n_fibers;
+    hipLaunchKernelGGL(k_cluster, dim3(t->n_fibers), dim3(CL_BLOCK), 0, st, bits, G, fiber_k, cnt, ovf);
+    DNA_HIP(hipGetLastError());
+    unsigned long long h_cnt[8];
+    uint32_t fk[2 * DNA_SCORE_NOALLOC_MAX_FIBERS + 1];
+    DNA_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof(h_cnt), hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipMemcpyAsync(fk, fiber_k, (2 * (size_t)t->n_fibers + 1) * 4, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipEventRecord(c->ev[6], st));
+    DNA_HIP(hipStreamSynchronize(st));
+    if (fk[2 * t->n_fibers]) { set_error("dna_score_noalloc: more than 4096 hits on one strand of one fiber"); return DNA_ERR_CAPACITY; }
+    out->candidates = h_cnt[0]; out->in_reach = h_cnt[1]; out->ssb_count = h_cnt[2];
+    out->dsb = h_cnt[3]; out->ssb0 = h_cnt[4]; out->ssb1 = h_cnt[5];
+    out->dsb_expected = dsb_expected(t, fk);
+    out->hits = nullptr;
+    *lcg_state = lcg_advance(s0, h_cnt[1]);
+    float ms = 0; (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+    out->t_ms = ms;
     return DNA_OK;
 }

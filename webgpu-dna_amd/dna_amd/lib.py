@@ -87,6 +87,7 @@ def lib():
     L.dna_irt_reactions.argtypes = [vp]
     L.dna_score_oh.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, vp, u64, C.POINTER(u32), C.c_int,
                                C.POINTER(DnaScoreOut), C.POINTER(DnaIndirectOut)]
+    L.dna_score_noalloc.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.POINTER(DnaScoreOut)]
     L.dna_run_irt_tracks.argtypes = [vp, vp, u64, u32, C.c_float, vp, u32, vp, C.c_int, u64, vp,
                                      C.POINTER(DnaChemOut)]
     ip = C.POINTER(C.c_int)
@@ -115,7 +116,7 @@ EXPORTED_SYMBOLS = ["dna_ctx_create", "dna_ctx_destroy", "dna_last_error", "dna_
                     "dna_surrogate_set_params", "dna_surrogate_set_norm", "dna_surrogate_forward",
                     "dna_surrogate_grad", "dna_surrogate_train", "dna_track_features",
                     "dna_surrogate_predict", "dna_xs_probe", "dna_irt_oh_final", "dna_score_oh", "dna_irt_particles",
-                    "dna_irt_reactions"]
+                    "dna_irt_reactions", "dna_score_noalloc"]
 
 
 def _check(fn, rc):
