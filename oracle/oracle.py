@@ -218,6 +218,11 @@ def irt_params():
     return o.reshape(9, 6)
 
 
+def meesungnoen_sigma(k):
+    """meesungnoen_sigma (primary.wgsl:90-98) of the transport restatement, FP32."""
+    return lib().oracle_meesungnoen_sigma(float(k))
+
+
 def irt_sample(r0, ri, seed, track, a, b):
     return lib().oracle_irt_sample(r0, ri, seed, track, a, b)
 

@@ -11,6 +11,8 @@ Reads (in the reference checkout, as data, nothing is executed):
   experiments/results/2026-05-07/level-2/E5 json       CSDA statistics
   experiments/results/2026-05-07/level-4/E10 json      IRT G(1 us) per energy
   tests/unit/cross-sections.test.ts:27-33               Geant4 sigma reference bands (numbers)
+  experiments/results/2026-05-08/level-0/B1-harness-liveness.json  the reference harness's
+      100 eV x 1024 results-table row (src/ui/table.ts:49-71 cell order) and its log line
 The output is committed; tests never read the reference checkout.
 """
 import csv
@@ -58,6 +60,18 @@ def main():
     for m in re.finditer(r"(\d+):\s*\{\s*ion:\s*([0-9.e-]+),\s*el:\s*([0-9.e-]+),\s*total:\s*([0-9.e-]+)", ts):
         g4ref[m.group(1)] = {"ion": float(m.group(2)), "el": float(m.group(3)), "total": float(m.group(4))}
     fx["xs_test_g4_ref"] = g4ref
+    b1 = j("experiments/results/2026-05-08/level-0/B1-harness-liveness.json")
+    row = b1["rows"][0]
+    cells = row["firstRowCells"]
+    names = ["E", "n_therm", "n_esc", "mean_total", "estar_csda", "csda_ratio", "mean_prod", "mean_sp",
+             "estar_sp", "sp_ratio", "mean_ions", "sec_per_pri", "cons_pct", "G_OH", "G_eaq", "G_H",
+             "ssb_dir", "ssb_ind", "dsb", "dsb_per_gy_gbp", "dt_ms"]
+    assert len(cells) == len(names)
+    log = row["logTail"] if isinstance(row["logTail"], str) else " ".join(row["logTail"])
+    m = re.search(r"E=100 eV: .*?rad_n=(\d+) E_cons=([0-9.]+)%", log)
+    fx["B1_100eV_row"] = {"np": 1024, "cells": dict(zip(names, cells)), "rad_n": int(m.group(1)),
+                          "log_E_cons_pct": m.group(2),
+                          "source": "experiments/results/2026-05-08/level-0/B1-harness-liveness.json"}
     with open(OUT, "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)
     print("wrote", OUT)

@@ -42,13 +42,17 @@ def test_10kev_against_reference_harness(run10k):
     # CSDA: per-event sd ~584 nm (E5 g4Stats), so the reference's 4096-run SEM is ~9 nm
     ref_sem = FX["E5_summary"]["g4Stats"]["semCsdaNm"]
     sig = np.hypot(a["csda_sem"], ref_sem)
-    assert abs(a["csda"] - ref["csdaNm"]) < max(0.01 * ref["csdaNm"], 2 * sig)
-    assert abs(a["ions"] / ref["ionsPerPriPrimaryOnly"] - 1) < 0.01
-    assert abs(a["sec"] / ref["secPerPri"] - 1) < 0.02
+    # within 1% and within 2 sigma of the combined MC error
+    d = abs(a["csda"] - ref["csdaNm"])
+    assert d < 0.01 * ref["csdaNm"] and d < 2 * sig, (a["csda"], ref["csdaNm"], sig)
+    ions_sig = np.hypot(a["ions_sem"], FX["g4_per_event_summary"]["ions"]["std"] / np.sqrt(4096))
+    d = abs(a["ions"] - ref["ionsPerPriPrimaryOnly"])
+    assert d < 0.01 * ref["ionsPerPriPrimaryOnly"] and d < 2 * ions_sig, (a["ions"], ions_sig)
+    assert abs(a["sec"] / ref["secPerPri"] - 1) < 0.01
     assert 0.999 < a["cons"] <= 1.0
     g = FX["webgpu_results"]["preChemistry"]
     for got, want in zip(a["G"], (g["G_OH_init"], g["G_eaq_init"], g["G_H_init"])):
-        assert abs(got / want - 1) < 0.02
+        assert abs(got / want - 1) < 0.01
 
 
 def test_kernel_dna_hits_rate(run10k):
@@ -108,3 +112,45 @@ def test_seed_independence_of_sharding():
     full = O.transport(p)
     part = O.transport(p, first=25, count=15)
     assert np.array_equal(full["results"][25:].view(np.uint32), part["results"].view(np.uint32))
+
+
+def test_100ev_reproduces_the_reference_harness_row():
+    """The reference harness's own 100 eV x 1024 run (experiments/results/2026-05-08/level-0/
+    B1-harness-liveness.json): the same seeds (floor(E) = 100, buffers.ts:91-104) give the
+    same radical count, 16749, and every results-table cell (src/ui/table.ts:49-65) at its
+    printed precision."""
+    from dna_amd import host
+    b1 = FX["B1_100eV_row"]
+    p = default_params(100.0, b1["np"])
+    o = O.transport(p, n_threads=4)
+    assert o["rad_n"] == b1["rad_n"] == 16749
+    r = host.aggregate_energy(100.0, p.n, o["results"], o["counters"], o["dose_sum"], o["sec_n"])
+    estar = [e for e in host.ESTAR if e["E"] == 100][0]
+    got = host.table_row_cells(r, estar)
+    for k, v in got.items():
+        assert v == b1["cells"][k], (k, v, b1["cells"][k])
+
+
+def test_meesungnoen_sigma_against_the_workers_power_sum():
+    """The transport's FP32 Horner form of the Meesungnoen 2002 fit (primary.wgsl:90-98)
+    against the worker's float64 power sum of the same fit (irt-worker.js:16-28), times
+    0.6267, within Higham's bound for Horner's rule in FP32 (2n u sum|g_i| k^i, n = 12, plus
+    the final product and clamp): a dropped term, a wrong sign or a shifted coefficient
+    exceeds it.  Zero at k <= 0.1; a negative fit value clamps to zero."""
+    g = [-4.06217193e-08, 3.06848412e-06, -9.93217814e-05, 1.80172797e-03, -2.01135480e-02,
+         1.42939448e-01, -6.48348714e-01, 1.85227848e+00, -3.36450378e+00, 4.37785068e+00,
+         -4.20557339e+00, 3.81679083e+00, -2.34069784e-01]
+    u = 2.0 ** -24
+    ks = np.concatenate([np.linspace(0.1001, 30.0, 4000), np.random.default_rng(2).uniform(0.1, 7.4, 2000)])
+    for k in ks.astype(np.float32):
+        kd = float(k)
+        r = sum(g[12 - i] * kd ** i for i in range(13))
+        want = max(r, 0.0) * 0.6267
+        absum = sum(abs(g[12 - i]) * kd ** i for i in range(13))
+        bound = (2 * 13 * u * absum + 3 * u * abs(r)) * 0.6267 + 1e-30
+        got = float(O.meesungnoen_sigma(k))
+        if r < 0:
+            assert got == 0.0 or abs(got) <= bound, (kd, got)
+        else:
+            assert abs(got - want) <= bound, (kd, got, want, bound)
+    assert O.meesungnoen_sigma(np.float32(0.1)) == 0.0 and O.meesungnoen_sigma(np.float32(0.05)) == 0.0

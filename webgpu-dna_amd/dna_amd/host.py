@@ -73,9 +73,36 @@ def run_at_energy(device, buffers, pipelines, E_eV, np_, box_nm, ce_eV, dna, run
     p = default_params(E_eV, np_, box_nm=box_nm, ce_eV=ce_eV, dna=dna, seed=seed)
     t0 = time.perf_counter()
     t = ctx.transport(p, copy=DNA_COPY_ALL)
-    res = t["results"]
-    total_deposited_eV = t["dose_sum"] / 100.0
+    agg = aggregate_energy(E_eV, np_, t["results"], t["counters"], t["dose_sum"], t["sec_n"])
     c = t["counters"]
+    rad_buf = t.get("rad")
+    chem_result = None
+    if E_eV == 10000 and rad_buf is not None and t["rad_n"] > 0 and run_chemistry is not None:
+        chem_result = run_chemistry(rad_buf, t["rad_n"], agg["n_therm"], E_eV)
+    dt = (time.perf_counter() - t0) * 1e3
+    out = {"E": E_eV}
+    out.update(agg)
+    out.update({
+        "dt": dt, "t_prim": t["t_prim_ms"], "t_sec": t["t_sec_ms"],
+        "sec_n": t["sec_n"], "sec_dropped": t["sec_dropped"],
+        "sec_terminated_cutoff": int(t["sec_stats"][1]), "sec_terminated_bounds": int(t["sec_stats"][2]),
+        "sec_steps": int(t["sec_stats"][3]), "sec_tertiary_ions": int(t["sec_stats"][4]),
+        "rad_OH": int(c[0]), "rad_eaq": int(c[1]), "rad_H": int(c[2]),
+        "rad_n_raw": t["rad_n_raw"], "rad_n_stored": t["rad_n"], "rad_dropped": t["rad_dropped"],
+        "kernel_dna_hits": int(c[4]), "chem_result": chem_result, "rad_buf_final": rad_buf,
+        "dose_arr": t.get("dose") if E_eV == 10000 else None,
+    })
+    return out
+
+
+def aggregate_energy(E_eV, np_, results, counters, dose_sum, sec_n):
+    """The host aggregation of runAtEnergy (dispatch.ts:178-226) from the per-primary
+    results R[np], the counters, the exact dose sum (100 units per eV) and the secondary
+    count: means over the primaries that did not escape, G_init per 100 eV deposited and
+    the energy conservation ratio."""
+    res = results
+    total_deposited_eV = dose_sum / 100.0
+    c = counters
     per100 = total_deposited_eV / 100.0
     esc = res["esc"] == 1
     therm = ~esc
@@ -87,27 +114,28 @@ def run_at_energy(device, buffers, pipelines, E_eV, np_, box_nm, ce_eV, dna, run
     sp = np.where(path > 0, (E_eV - fE) / np.where(path > 0, path, 1), 0.0)
     mean = (lambda a: float(a.sum() / n_therm) if n_therm > 0 else 0.0)
     expected = n_therm * E_eV
-    rad_buf = t.get("rad")
-    chem_result = None
-    if E_eV == 10000 and rad_buf is not None and t["rad_n"] > 0 and run_chemistry is not None:
-        chem_result = run_chemistry(rad_buf, t["rad_n"], n_therm, E_eV)
-    dt = (time.perf_counter() - t0) * 1e3
     return {
-        "E": E_eV, "n_therm": n_therm, "n_esc": n_esc,
+        "n_therm": n_therm, "n_esc": n_esc,
         "mean_total": mean(path), "mean_prod": mean(prod), "mean_sp": mean(sp), "mean_ions": mean(ni),
-        "dt": dt, "t_prim": t["t_prim_ms"], "t_sec": t["t_sec_ms"],
-        "sec_n": t["sec_n"], "sec_dropped": t["sec_dropped"], "sec_per_pri": t["sec_n"] / np_,
-        "sec_terminated_cutoff": int(t["sec_stats"][1]), "sec_terminated_bounds": int(t["sec_stats"][2]),
-        "sec_steps": int(t["sec_stats"][3]), "sec_tertiary_ions": int(t["sec_stats"][4]),
+        "sec_per_pri": sec_n / np_,
         "total_deposited_eV": total_deposited_eV, "expected_deposited_eV": expected,
         "cons_ratio": total_deposited_eV / expected if expected > 0 else 0.0,
-        "rad_OH": int(c[0]), "rad_eaq": int(c[1]), "rad_H": int(c[2]),
         "G_OH": c[0] / per100 if per100 > 0 else 0.0, "G_eaq": c[1] / per100 if per100 > 0 else 0.0,
         "G_H": c[2] / per100 if per100 > 0 else 0.0,
-        "rad_n_raw": t["rad_n_raw"], "rad_n_stored": t["rad_n"], "rad_dropped": t["rad_dropped"],
-        "kernel_dna_hits": int(c[4]), "chem_result": chem_result, "rad_buf_final": rad_buf,
-        "dose_arr": t.get("dose") if E_eV == 10000 else None,
     }
+
+
+def table_row_cells(r, estar):
+    """The results-table cells of one energy (src/ui/table.ts:49-65, toFixed formatting)
+    from an aggregate_energy / run_at_energy dict and its ESTAR entry."""
+    return {"n_therm": str(r["n_therm"]), "n_esc": str(r["n_esc"]),
+            "mean_total": to_fixed(r["mean_total"], 1),
+            "csda_ratio": to_fixed(r["mean_total"] / estar["csda"], 2) + "\u00d7",
+            "mean_prod": to_fixed(r["mean_prod"], 1), "mean_sp": to_fixed(r["mean_sp"], 2),
+            "sp_ratio": to_fixed(r["mean_sp"] / estar["sp"], 2) + "\u00d7",
+            "mean_ions": to_fixed(r["mean_ions"], 1), "sec_per_pri": to_fixed(r["sec_per_pri"], 1),
+            "cons_pct": to_fixed(r["cons_ratio"] * 100, 1) + "%",
+            "G_OH": to_fixed(r["G_OH"], 2), "G_eaq": to_fixed(r["G_eaq"], 2), "G_H": to_fixed(r["G_H"], 2)}
 
 
 def to_fixed(x, digits):
