@@ -58,12 +58,16 @@ struct IrtArgs {
     float4 *pos;                  // (x, y, z, species) per particle slot
     uint8_t *alive;               // per particle slot: species + 1 while alive, 0 once reacted
     uint32_t *lst;                // 6 species lists of capacity cap per track (may hold reacted particles)
+    float4 *lpos;                 // beside each list entry: the particle's x, y, z and its birth time
+                                  // rounded down to FP32 (contiguous reads for the re-pairing screen)
     double *tb;
     double *ev_t;
     uint2 *ev_ab;
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
-    uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy
+    uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy (pairs kernel)
+    uint32_t *tstate;             // TSTATE u32 per track (launch order): pairs kernel -> loop kernel
+    uint32_t al_lds;              // bytes of dynamic LDS for the alive flags (loop kernel)
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
     uint32_t *final_ntot;         // optional: particles per track at the end (track order); the
@@ -356,35 +360,88 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
     idx[bstart[k] + S[i] - firstj[k]] = (uint32_t)i;
 }
 
-// ------------------------------------------------------------ per-track kernel
-#define IRT_BLOCK 256                // largest workgroup; the launch picks 64, 128 or 256
+// ------------------------------------------------------------ per-track kernels
+// Two launches per batch, both in launch order (largest tracks first):
+//   k_irt_pairs  one workgroup per track (64, 128 or 256 threads, the size autotuned): the
+//                displacement of the initial species, the species-sorted copy (in LDS), the
+//                initial pairs of every reaction channel and their FP32 screens; the pairs
+//                that may react are queued in HBM with their FP64 distance
+//   k_irt_loop   one wavefront per track, 16 tracks per CU (128 VGPRs, under 10 KB of LDS):
+//                the FP64 sampling of the queued pairs (the drain), then the event loop.  The
+//                next event comes from per-block minima of the event list kept in LDS: the
+//                global minimum is popped, and when it is stale (a reactant has reacted) it is
+//                dropped and only its block is rescanned (lazy deletion).  The events that one
+//                reaction's re-pairing pushes are staged in LDS and merged into the minima.
+//                A wavefront needs no workgroup barrier between the serial steps of a
+//                reaction, and 16 independent event chains share each CU to hide their
+//                latency.
+#define IRT_BLOCK 256                // largest workgroup of k_irt_pairs
 #define IRT_NT (blockDim.x)
+#define BM_MAX 128u                  // event blocks per track (a block holds 64 k events)
+#define STAGE_MAX 64u                // events staged per reaction (more: blocks rescanned)
+#define AL_LDS_MAX 3584u             // alive flags in LDS up to this many particle slots
+#define TSTATE 16                    // u32 per track handed from k_irt_pairs to k_irt_loop:
+                                     // [0] queued pairs, [1..6] species list lengths,
+                                     // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n
 
 __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint64_t ab2)
 {
     return t1 < t2 || (t1 == t2 && ab1 < ab2);
 }
 
-struct TrackShared {
+// Wavefront minimum of (t, ab) in every lane; returns the lane that holds it.  Event times
+// are non-negative, so their bit patterns order like the values: one 64-bit butterfly
+// finds the least time, and only lanes sharing it (a tie, e.g. contact reactions at the
+// same instant) take a second butterfly on ab.
+__device__ __forceinline__ uint32_t wave_min_ev(double &t, uint64_t &ab)
+{
+    uint64_t tk = (uint64_t)__double_as_longlong(t);
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t ok = __shfl_xor(tk, o, 64);
+        tk = ok < tk ? ok : tk;
+    }
+    const bool at = (uint64_t)__double_as_longlong(t) == tk;
+    const uint64_t m = __ballot(at);
+    uint64_t k = at ? ab : ~0ull;
+    if (m & (m - 1ull)) {                       // a tie on t: least ab among the tied lanes
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t ok = __shfl_xor(k, o, 64);
+            k = ok < k ? ok : k;
+        }
+    } else {
+        k = __shfl(k, (int)__ffsll((long long)m) - 1, 64);
+    }
+    const uint32_t src = (uint32_t)__ffsll((long long)__ballot(at && ab == k)) - 1u;
+    t = __longlong_as_double((long long)tk);
+    ab = k;
+    return src;
+}
+
+// shared state of k_irt_pairs
+struct PairShared {
+    uint32_t npend;                       // queued pairs
+    uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
+    int      bri[9], nblk;                // channel blocks: reaction, units per row
+    uint32_t bcpr[9], bu0[9], units;
+    unsigned long long c[3];              // initial OH, e-aq, H
+};
+
+// shared state of k_irt_loop (one wavefront)
+struct LoopShared {
+    static constexpr uint32_t LONG_CAP = 128;   // >= 2 x 64, the small-queue drain lists every pair
     uint32_t n_ev;
-    uint32_t ev_lds;             // events [0, ev_lds) live in LDS (event loop), the rest in HBM
     uint32_t n_total;
     double   win_t;
     uint64_t win_ab;
     int      np;
     uint32_t prod_idx[2];
-    double   wt[IRT_BLOCK / 64];
-    uint64_t wab[IRT_BLOCK / 64];
     unsigned long long cnt[5];   // running OH, e-aq, H, H2O2, H2
     unsigned long long rxc[9];   // reaction counts
     unsigned long long n_re;
     uint32_t npend;              // queued stage-2 (SamplePDC) pairs
     uint32_t pend_next;          // drain cursor
     uint32_t long_n, long_next;  // pairs handed from the per-lane to the wavefront schedule
-    uint32_t long_item[512];
-    uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
-    int      bri[9], nblk;                // channel blocks: reaction, units per row
-    uint32_t bcpr[9], bu0[9], units;
+    uint32_t long_item[LONG_CAP];
     // reaction tables by species pair (row * 6 + col) and by species, copied from c_irt
     // so that lanes with different species read them from LDS
     RxB      bnd[36];                     // cannot_react bounds of the pair's reaction
@@ -402,33 +459,38 @@ struct TrackShared {
     // re-pairing segments (product p, partner species sj) over the lists
     int      nseg;
     uint32_t seg_u0[13], seg_ps[12];      // first flattened index; p << 4 | sj
-    uint32_t rq[IRT_BLOCK / 64][128];     // per wavefront: re-pairings that passed the bound
-    uint32_t rqn[IRT_BLOCK / 64];         // and how many are left after the scan
+    uint32_t rq[1][128];                  // re-pairings that passed the bound
+    uint32_t rqn[1];
+    // event selection: blocks of bs events, the minimum live (t, a << 32 | b) of each block
+    // (a minimum may have gone stale since), and the events pushed in the current reaction
+    uint32_t bs, nblk, e0;
+    double   bm_t[BM_MAX];
+    uint64_t bm_ab[BM_MAX];
+    uint32_t nt_n;
+    uint32_t nt_e[STAGE_MAX];
+    double   nt_t[STAGE_MAX];
+    uint64_t nt_ab[STAGE_MAX];
+    uint32_t snap[DNA_MAX_CKPT * 5];      // species counts at each checkpoint
 };
 
 extern __shared__ float4 dyn_lds[];
 
-// Event storage in LDS during the event loop, behind the alive flags (cap bytes): times
-// [L] then pairs packed a << 16 | b [L] (cap <= 65535).
-__device__ __forceinline__ double *ev_lds_t(const IrtTrack &T)
-{
-    return (double *)((char *)dyn_lds + (((size_t)T.cap + 15u) & ~(size_t)15u));
-}
-__device__ __forceinline__ uint32_t *ev_lds_ab(const IrtTrack &T, uint32_t L) { return (uint32_t *)(ev_lds_t(T) + L); }
-
-__device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
+// An event of the loop kernel: stored in HBM and staged for the block minima.
+__device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, LoopShared &sh,
                                            double t, uint32_t a, uint32_t b)
 {
-    uint32_t e = atomicAdd(&sh.n_ev, 1u);
-    const uint32_t L = sh.ev_lds;
-    if (e < L) { ev_lds_t(T)[e] = t; ev_lds_ab(T, L)[e] = (a << 16) | b; }
-    else if (e < T.ev_cap) { A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b); }
-    else atomicOr(A.overflow, 2u);
+    const uint32_t e = atomicAdd(&sh.n_ev, 1u);
+    if (e < T.ev_cap) {
+        A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b);
+        const uint32_t s = atomicAdd(&sh.nt_n, 1u);
+        if (s < STAGE_MAX) { sh.nt_e[s] = e; sh.nt_t[s] = t; sh.nt_ab[s] = ((uint64_t)a << 32) | b; }
+    } else atomicOr(A.overflow, 2u);
 }
 
 // A queued pair: indices a < b with their species in the top 4 bits (track capacities are
 // below 2^28 particles), and r0.
-__device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T, TrackShared &sh,
+template <class SH>
+__device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T, SH &sh,
                                              uint32_t a, uint32_t b, uint32_t sa, uint32_t sb, double r0)
 {
     uint32_t q = atomicAdd(&sh.npend, 1u);
@@ -451,7 +513,7 @@ __device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T
 struct PendPair { uint32_t a, b; double r0; int ri; double D; uint64_t ps; };
 
 // A queued pair with its r0 (r0 < 0: a re-pairing beyond R_CUT, no event).
-__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh,
+__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, const LoopShared &sh,
                                                  uint64_t tk, uint32_t item)
 {
     uint4 e = A.pend[T.ev_off + item];
@@ -526,7 +588,6 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
 }
 
 #define LANE_TRIALS 64u
-#define LONG_ITEMS 512u
 
 // per-lane schedule; a pair still running after LANE_TRIALS trials is handed to the
 // wavefront schedule (sh.long_item / sh.long_n) and resumed there at that trial.  Lanes
@@ -534,7 +595,7 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
 // refill, so the first stage and the SamplePDC setup (a global load and FP64 code) run once
 // per refill for all idle lanes instead of in almost every trial.
 #define REFILL 8u
-__device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+__device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
                                 uint32_t npend, double t_base)
 {
     double a = 1, b = 1, p = 0, q = 0, M = 0;
@@ -564,7 +625,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
             } else if (trial == LANE_TRIALS) {
                 // to the wavefront schedule; when its list is full the lane goes on itself
                 const uint32_t l = atomicAdd(&sh.long_n, 1u);
-                if (l < LONG_ITEMS) { sh.long_item[l] = item; have = false; }
+                if (l < LoopShared::LONG_CAP) { sh.long_item[l] = item; have = false; }
             } else if (trial == 10000u) have = false;
         }
     }
@@ -572,11 +633,11 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
 
 // wavefront schedule over items [from the queue, or the long list when use_long; those
 // resume at trial long_start]
-__device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+__device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
                                 uint32_t npend, double t_base, bool use_long, uint32_t long_start = LANE_TRIALS)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t nlong = sh.long_n < LONG_ITEMS ? sh.long_n : LONG_ITEMS;
+    const uint32_t nlong = sh.long_n < LoopShared::LONG_CAP ? sh.long_n : LoopShared::LONG_CAP;
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(use_long ? &sh.long_next : &sh.pend_next, 1u);
@@ -607,10 +668,10 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
-// small queues (fewer than 2 x workgroup size items, so at most LONG_ITEMS - 1): the first
+// small queues (fewer than 2 x 64 items, so at most LoopShared::LONG_CAP - 1): the first
 // stage of every pair on the first wavefront, a lane per pair; the pairs that need SamplePDC
 // trials are listed and run on the wavefront schedule from trial 0
-__device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh,
+__device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
                                 uint32_t npend, double t_base)
 {
     if (threadIdx.x < 64u) {
@@ -626,7 +687,7 @@ __device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk
 }
 
 // drain the whole queue; all threads call it after a barrier
-__device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, TrackShared &sh, double t_base)
+__device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh, double t_base)
 {
     const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
     if (npend >= 2u * IRT_NT) {
@@ -640,7 +701,7 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, Trac
 
 // The exact part for one staged re-pairing (product p, partner jj): the partner's
 // Brownian step, r0, the first sampling stage, and the queueing of pairs that can react.
-__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al,
+__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
                                              uint64_t tk, double t_new, uint32_t e)
 {
     const float4 *P = A.pos + T.part_off;
@@ -702,12 +763,11 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
 // the survivors are staged per wavefront and finished 64 at a time (the FP64 step,
 // r0 = |product - partner| (irt-worker.js:443-456), the first sampling stage), so that
 // part runs on full wavefronts; pairs that can react are queued for the drain.
-__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al,
+__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
                               uint64_t tk, double t_new, unsigned long long &wrep)
 {
-    const float4 *P = A.pos + T.part_off;
-    const double *tb = A.tb + T.part_off;
     const uint32_t *lst = A.lst + 6 * T.part_off;
+    const float4 *lps = A.lpos + 6 * T.part_off;
     const int ns = sh.nseg, np = sh.np;
     const uint32_t tot = sh.seg_u0[ns];
     const uint32_t pidx1 = sh.prod_idx[1];
@@ -723,29 +783,45 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
             int sg = 0;
             while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
             const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
-            const uint32_t jj = lst[(size_t)sj * T.cap + (u - sh.seg_u0[sg])];
+            const size_t slot = (size_t)sj * T.cap + (u - sh.seg_u0[sg]);
+            const uint32_t jj = lst[slot];
+            const float4 pj = lps[slot];          // x, y, z, birth time rounded down
             const uint32_t idx = sh.prod_idx[p];
             if (al[jj] && jj != idx && !(p == 0 && np == 2 && jj == pidx1)) {
                 wrep++;
                 const uint32_t pc = sh.prod_sp[p] * 6u + sj;
-                const float4 pi = sh.prod_pos[p], pj = P[jj];
+                const float4 pi = sh.prod_pos[p];
                 const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
                 const uint64_t ps = dna_pair_state(tk, a, b);
-                const double dt = t_new - tb[jj];
-                // each randn() of the step is at most sqrt(-2 ln(u + 1e-30)) in magnitude, so
-                // the step is at most sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4)
+                // t_new - (birth time rounded down) >= the exact step time, so the bound's
+                // step is at least the true one: the screen stays conservative
+                const double dt = t_new - (double)pj.w;
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-                // (u01_lo <= the draw, so -ln of it is an upper bound; the three logs are taken
-                // as one of the FP32 product, within 2.4e-7 of their sum; u01_lo = 0 gives an
-                // infinite step and keeps the pair)
-                float rmin = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
-                if (dt > 0) {
-                    float m2 = -2.f * logf(u01_lo(ps, 0) * u01_lo(ps, 2) * u01_lo(ps, 4));
-                    float sgm = sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt));
-                    rmin -= sgm * sqrt_bound(m2 * 1.0001f + 1e-4f) * 1.0001f;
-                }
-                keep = !(rmin > c_irt.R_CUT_F) && !cannot_react(sh.bnd[pc], rmin, u01_lo(ps, 6u));
+                const float r = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
+                const float sgm = dt > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt)) : 0.f;
+                // first without any draw of the step: each randn() is at most
+                // sqrt(-2 ln(1e-30)) = 11.754 in magnitude, so the step is at most 20.36 sig; a
+                // partner beyond R_CUT by that much cannot pair, and the draw of the first
+                // sampling stage alone may already rule the pair out
+                const float rloose = r - sgm * 20.4f;
                 ent = (p << 31) | jj;
+                if (!(rloose > c_irt.R_CUT_F)) {
+                    const float U = u01_lo(ps, 6u);
+                    if (!cannot_react(sh.bnd[pc], rloose, U)) {
+                        // then with the step's own draws: each randn() is at most
+                        // sqrt(-2 ln(u + 1e-30)), so the step is at most
+                        // sig * sqrt(-2 (ln u0 + ln u2 + ln u4)) (FP32, inflated by 1e-4;
+                        // u01_lo <= the draw, so -ln of it is an upper bound; the three logs
+                        // are taken as one of the FP32 product, within 2.4e-7 of their sum;
+                        // u01_lo = 0 gives an infinite step and keeps the pair)
+                        float rmin = r;
+                        if (dt > 0) {
+                            const float m2 = -2.f * logf(u01_lo(ps, 0) * u01_lo(ps, 2) * u01_lo(ps, 4));
+                            rmin -= sgm * sqrt_bound(m2 * 1.0001f + 1e-4f) * 1.0001f;
+                        }
+                        keep = !(rmin > c_irt.R_CUT_F) && !cannot_react(sh.bnd[pc], rmin, U);
+                    }
+                }
             }
         }
         const uint64_t m = __ballot(keep);
@@ -778,33 +854,30 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, TrackShared &
 // barrier; ends with one).  The order inside a list does not matter: every chunk of 256
 // entries is read before any of its slots is rewritten, and an entry is only ever moved
 // to a lower position.
-__device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, const uint8_t *al)
+__device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al)
 {
     uint32_t *lst = A.lst + 6 * T.part_off;
+    float4 *lps = A.lpos + 6 * T.part_off;
     const uint32_t mask = sh.lcomp;
     for (int s = 0; s < 6; s++) {
         if (!(mask & (1u << s))) continue;
         uint32_t *L = lst + (size_t)s * T.cap;
+        float4 *LP = lps + (size_t)s * T.cap;
         const uint32_t len = sh.llen[s];
         if (threadIdx.x == 0) sh.lcnt = 0;
         __syncthreads();
         for (uint32_t c0 = 0; c0 < len; c0 += IRT_NT) {
             const uint32_t i = c0 + threadIdx.x;
             const uint32_t k = i < len ? L[i] : 0u;
+            const float4 q = i < len ? LP[i] : make_float4(0.f, 0.f, 0.f, 0.f);
             const bool live = i < len && al[k];
             __syncthreads();
-            if (live) L[atomicAdd(&sh.lcnt, 1u)] = k;
+            if (live) { const uint32_t d = atomicAdd(&sh.lcnt, 1u); L[d] = k; LP[d] = q; }
         }
         __syncthreads();
         if (threadIdx.x == 0) { sh.llen[s] = sh.lcnt; sh.ldead[s] = 0; }
         __syncthreads();
     }
-}
-
-__device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared &sh, int c,
-                         unsigned long long *loc)
-{
-    for (int s = 0; s < 5; s++) loc[c * 5 + s] += sh.cnt[s];
 }
 
 // Initial pairs of the channel blocks (see k_irt step 2), with the species-sorted copy in
@@ -815,7 +888,7 @@ __device__ void snapshot(const IrtArgs &A, const IrtTrack &T, const TrackShared 
 // sqrt(r2) (1 - 1e-5) and the draw's 24-bit lower bound.  The exact FP64 distance is
 // computed near the cutoff and for the pairs that are queued.
 template <bool SG_LDS>
-__device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T, TrackShared &sh, uint64_t tk,
+__device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T, PairShared &sh, uint64_t tk,
                                             unsigned long long &wcand, unsigned long long &wsamp)
 {
     const float4 *Sg = SG_LDS ? (const float4 *)dyn_lds : A.sorted + T.part_off;
@@ -875,36 +948,23 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
 }
 
 template <bool DEATH>
-__global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
+__global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
 {
     const IrtConst &C = c_irt;
-    __shared__ TrackShared sh;
-    __shared__ unsigned long long s_loc[DNA_MAX_CKPT * 5];
+    __shared__ PairShared sh;
     const IrtTrack T = A.tracks[blockIdx.x];          // launch order: largest tracks first
     const uint32_t tid = threadIdx.x;
     const uint32_t n = T.n;
-    const uint32_t *idx = A.idx + T.start;
+    const uint32_t *ord = A.idx + T.start;            // the track's entries in input order
     float4 *P = A.pos + T.part_off;
     uint8_t *al = A.alive + T.part_off;
     double *tb = A.tb + T.part_off;
     const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
-
-    // the track's entries in input order (the grouping keeps input order within a primary)
-    const uint32_t *ord = idx;
-    if (tid == 0) { sh.n_ev = 0; sh.ev_lds = 0; sh.n_total = n; sh.n_re = 0; sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; for (int s = 0; s < 5; s++) sh.cnt[s] = 0; for (int r = 0; r < 9; r++) sh.rxc[r] = 0; }
-    for (int i = tid; i < DNA_MAX_CKPT * 5; i += IRT_NT) s_loc[i] = 0;
-    if (tid < 36) {
-        const int r = C.rxnMap[tid];
-        sh.rmap[tid] = (int8_t)r;
-        RxB bz; bz.S = 0.f; bz.sc = 0.f; bz.R = 0.f;
-        sh.bnd[tid] = r >= 0 ? rx_bound(rx_constants(r, C.D[tid / 6] + C.D[tid % 6])) : bz;
-    }
-    if (tid < 9) { sh.nprod[tid] = (int8_t)C.nprod[tid]; sh.prods[tid][0] = (int8_t)C.prods[tid][0]; sh.prods[tid][1] = (int8_t)C.prods[tid][1]; }
-    if (tid < 6) { sh.Dd[tid] = C.D[tid]; sh.sqD[tid] = C.sqrtD[tid]; sh.Df[tid] = (float)C.D[tid]; }
+    if (tid == 0) { sh.npend = 0; sh.c[0] = 0; sh.c[1] = 0; sh.c[2] = 0; }
+    if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
     __syncthreads();
-
-    // phase timers (100 MHz wall clock, lane 0): load, pairs, scan, react, repair, iterations
-    unsigned long long tp0 = wall_clock64(), tp1 = 0, tp2 = 0, tsc = 0, trc = 0, trp = 0, trd = 0, nit = 0;
+    // phase timers (100 MHz wall clock, thread 0): load, pairs
+    const unsigned long long tp0 = wall_clock64();
     // 1. load + displacement (irt-worker.js:475-537)
     unsigned long long c0 = 0, c1 = 0, c2 = 0;
     for (uint32_t k = tid; k < n; k += IRT_NT) {
@@ -935,13 +995,12 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         if (DEATH) A.tdeath[T.part_off + k] = __longlong_as_double(0x7FF0000000000000ll);   // +inf
         tb[k] = 0.0;
         if (alv) { if (spc == 0) c0++; else if (spc == 1) c1++; else if (spc == 2) c2++; }
+        if (alv && spc >= 0 && spc < 6) atomicAdd(&sh.gcnt[spc], 1u);
     }
     for (int o = 32; o > 0; o >>= 1) { c0 += __shfl_xor(c0, o, 64); c1 += __shfl_xor(c1, o, 64); c2 += __shfl_xor(c2, o, 64); }
-    if ((tid & 63) == 0) { atomicAdd(&sh.cnt[0], c0); atomicAdd(&sh.cnt[1], c1); atomicAdd(&sh.cnt[2], c2); }
-    if (tid == 0) sh.cnt[4] = T.h2;
+    if ((tid & 63) == 0) { atomicAdd(&sh.c[0], c0); atomicAdd(&sh.c[1], c1); atomicAdd(&sh.c[2], c2); }
     __syncthreads();
-
-    tp1 = wall_clock64();
+    const unsigned long long tp1 = wall_clock64();
     // 2. initial pairs (irt-worker.js:541-558).  The particles are copied in species
     // order (Sg, original index in .w) and only the species blocks that have a reaction
     // channel are enumerated, in units of one row particle x 64 partners, so every lane
@@ -950,15 +1009,8 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     // d = m/2 for i < m/2 when m is even: every unordered pair exactly once, equal rows.
     // Pairs are identified by their original indices, so the enumeration order does not
     // change any draw or the event order.
-    unsigned long long wcand = 0, wsamp = 0, wrep = 0;
+    unsigned long long wcand = 0, wsamp = 0;
     float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
-    if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
-    __syncthreads();
-    for (uint32_t k = tid; k < n; k += IRT_NT) {
-        int spc = (int)P[k].w;
-        if (al[k] && spc >= 0 && spc < 6) atomicAdd(&sh.gcnt[spc], 1u);
-    }
-    __syncthreads();
     if (tid == 0) {
         uint32_t o = 0;
         for (int g = 0; g < 6; g++) { sh.goff[g] = o; o += sh.gcnt[g]; }
@@ -991,148 +1043,207 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
     for (uint32_t g = tid; g < n; g += IRT_NT) {
         int spc = 0;
         while (spc < 5 && g >= sh.goff[spc + 1]) spc++;
-        if (g < sh.goff[spc] + sh.gcnt[spc])
-            A.lst[6 * T.part_off + (size_t)spc * T.cap + (g - sh.goff[spc])] = (uint32_t)__float_as_int(Sg[g].w);
-    }
-    if (tid < 6) { sh.llen[tid] = sh.gcnt[tid]; sh.ldead[tid] = 0; }
-    __syncthreads();
-    unsigned long long tpl = wall_clock64();
-    pdc_drain(A, T, tk, sh, 0.0);
-    __syncthreads();
-    if (tid == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
-    __syncthreads();
-    unsigned long long n_init_ev = sh.n_ev;
-    int cp = 0;
-    // the event loop reads alive flags at random: keep them in LDS, where the sorted
-    // copy of the pair phase lived
-    uint8_t *const al_global = al;
-    if ((size_t)T.cap <= (size_t)A.lds_slots * 16u) {
-        uint8_t *la = (uint8_t *)dyn_lds;
-        for (uint32_t k = tid; k < T.cap; k += IRT_NT) la[k] = k < n ? al[k] : 0;
-        // and the events behind them, when all current ones fit: the scan below reads every
-        // event each iteration; later events beyond the LDS capacity go to HBM
-        const size_t lds_bytes = (size_t)A.lds_slots * 16u, abytes = ((size_t)T.cap + 15u) & ~(size_t)15u;
-        const uint32_t L = (T.cap <= 65535u && lds_bytes > abytes) ? (uint32_t)((lds_bytes - abytes) / 12u) : 0u;
-        const uint32_t ne0 = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
-        if (L >= 64u && ne0 <= L) {
-            double *lt = ev_lds_t(T);
-            uint32_t *lab = ev_lds_ab(T, L);
-            for (uint32_t e = tid; e < ne0; e += IRT_NT) {
-                uint2 ab = A.ev_ab[T.ev_off + e];
-                lt[e] = A.ev_t[T.ev_off + e];
-                lab[e] = (ab.x << 16) | ab.y;
-            }
-            if (tid == 0) sh.ev_lds = L;
+        if (g < sh.goff[spc] + sh.gcnt[spc]) {
+            const float4 q = Sg[g];
+            const size_t slot = 6 * T.part_off + (size_t)spc * T.cap + (g - sh.goff[spc]);
+            A.lst[slot] = (uint32_t)__float_as_int(q.w);
+            A.lpos[slot] = make_float4(q.x, q.y, q.z, 0.0f);
         }
+    }
+    __syncthreads();
+    const unsigned long long tp2 = wall_clock64();
+    for (int o = 32; o > 0; o >>= 1) { wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); }
+    if ((tid & 63) == 0) {
+        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 11;
+        if (wcand) atomicAdd(&g[0], wcand);
+        if (wsamp) atomicAdd(&g[1], wsamp);
+    }
+    if (tid == 0) {
+        // hand-over to the loop kernel (same launch index)
+        uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
+        st[0] = sh.npend;
+        for (int s = 0; s < 6; s++) st[1 + s] = sh.gcnt[s];
+        st[7] = (uint32_t)sh.c[0]; st[8] = (uint32_t)sh.c[1]; st[9] = (uint32_t)sh.c[2];
+        st[10] = T.h2; st[11] = n;
+        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
+        atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tp2 - tp1);
+    }
+}
+
+// The minimum of the live events of block j into the block minima (all lanes of the
+// wavefront; lane 0 writes it).  Events of reacted particles are skipped.
+__device__ __forceinline__ void block_min(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
+                                          uint32_t j)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t bs = sh.bs, ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
+    const uint32_t end = (j + 1) * bs < ne ? (j + 1) * bs : ne;
+    double bt = 1e300;
+    uint64_t bab = ~0ull;
+    for (uint32_t e = j * bs + lane; e < end; e += 64u) {
+        const uint2 ab = A.ev_ab[T.ev_off + e];
+        const double t = A.ev_t[T.ev_off + e];
+        if (!al[ab.x] || !al[ab.y]) continue;
+        const uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
+        if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
+    }
+    wave_min_ev(bt, bab);
+    if (lane == 0) { sh.bm_t[j] = bt; sh.bm_ab[j] = bab; }
+}
+
+template <bool DEATH>
+__global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
+{
+    const IrtConst &C = c_irt;
+    __shared__ LoopShared sh;
+    const IrtTrack T = A.tracks[blockIdx.x];
+    const uint32_t lane = threadIdx.x;
+    float4 *P = A.pos + T.part_off;
+    uint8_t *al = A.alive + T.part_off;
+    double *tb = A.tb + T.part_off;
+    const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
+    const uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
+    const unsigned long long tp0 = wall_clock64();
+    if (lane == 0) {
+        sh.npend = st[0]; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
+        sh.n_ev = 0; sh.n_total = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
+        for (int s = 0; s < 6; s++) { sh.llen[s] = st[1 + s]; sh.ldead[s] = 0; }
+        sh.cnt[0] = st[7]; sh.cnt[1] = st[8]; sh.cnt[2] = st[9]; sh.cnt[3] = 0; sh.cnt[4] = st[10];
+        for (int r = 0; r < 9; r++) sh.rxc[r] = 0;
+        // blocks of 64 k events, k the smallest that covers the event capacity with BM_MAX blocks
+        const uint32_t kb = (T.ev_cap + 64u * BM_MAX - 1u) / (64u * BM_MAX);
+        sh.bs = 64u * (kb ? kb : 1u);
+    }
+    if (lane < 36) {
+        const int r = C.rxnMap[lane];
+        sh.rmap[lane] = (int8_t)r;
+        RxB bz; bz.S = 0.f; bz.sc = 0.f; bz.R = 0.f;
+        sh.bnd[lane] = r >= 0 ? rx_bound(rx_constants(r, C.D[lane / 6] + C.D[lane % 6])) : bz;
+    }
+    if (lane < 9) { sh.nprod[lane] = (int8_t)C.nprod[lane]; sh.prods[lane][0] = (int8_t)C.prods[lane][0]; sh.prods[lane][1] = (int8_t)C.prods[lane][1]; }
+    if (lane < 6) { sh.Dd[lane] = C.D[lane]; sh.sqD[lane] = C.sqrtD[lane]; sh.Df[lane] = (float)C.D[lane]; }
+    for (uint32_t i = lane; i < DNA_MAX_CKPT * 5; i += 64u) sh.snap[i] = 0u;
+    for (uint32_t j = lane; j < BM_MAX; j += 64u) { sh.bm_t[j] = 1e300; sh.bm_ab[j] = ~0ull; }
+    __syncthreads();
+    // alive bytes in LDS when the track's slots fit (the loop reads them at random)
+    uint8_t *const al_global = al;
+    if (T.cap <= A.al_lds) {
+        uint8_t *la = (uint8_t *)dyn_lds;
+        const uint32_t nt0 = sh.n_total;
+        for (uint32_t k = lane; k < T.cap; k += 64u) la[k] = k < nt0 ? al[k] : 0;
         __syncthreads();
         al = la;
     }
-    tp2 = wall_clock64();
-
-    // 3. event loop
+    // 1. the queued initial pairs: the rest of sampleIRT (irt-worker.js:116-184) in FP64
+    pdc_drain(A, T, tk, sh, 0.0);
+    __syncthreads();
+    const unsigned long long n_init_ev = sh.n_ev;
+    if (lane == 0) {
+        sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; sh.nt_n = 0;
+        const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
+        sh.nblk = (ne + sh.bs - 1) / sh.bs;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < sh.nblk; j++) block_min(A, T, sh, al, j);
+    __syncthreads();
+    const unsigned long long tp1 = wall_clock64();
+    unsigned long long tsc = 0, trc = 0, trp = 0, trd = 0, nit = 0, wrep = 0;
+    int cp = 0;
+    // 2. event loop (irt-worker.js:566-645)
     for (;;) {
-        unsigned long long q0 = wall_clock64();
+        const unsigned long long q0 = wall_clock64();
         nit++;
-        uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
-        double bt = 1e300; uint64_t bab = ~0ull;
-        const uint32_t L = sh.ev_lds, nl = ne < L ? ne : L;
-        if (nl) {
-            const double *lt = ev_lds_t(T);
-            const uint32_t *lab = ev_lds_ab(T, L);
-            for (uint32_t e = tid; e < nl; e += IRT_NT) {
-                const uint32_t ab = lab[e], a = ab >> 16, b = ab & 0xFFFFu;
-                if (!al[a] || !al[b]) continue;
-                const double t = lt[e];
-                uint64_t k = ((uint64_t)a << 32) | b;
-                if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
+        // the next event: the least block minimum whose reactants are both alive; a stale
+        // minimum is replaced by its block's rescan
+        double t0;
+        uint64_t ab0;
+        for (;;) {
+            double bt = 1e300;
+            uint64_t bab = ~0ull;
+            uint32_t bj = 0;
+            for (uint32_t j = lane; j < sh.nblk; j += 64u) {
+                const double t = sh.bm_t[j];
+                const uint64_t k = sh.bm_ab[j];
+                if (ev_less(t, k, bt, bab)) { bt = t; bab = k; bj = j; }
             }
+            bj = __shfl(bj, (int)wave_min_ev(bt, bab), 64);
+            t0 = bt; ab0 = bab;
+            if (!(bt < 1e300)) break;
+            if (al[(uint32_t)(bab >> 32)] && al[(uint32_t)bab]) break;
+            block_min(A, T, sh, al, bj);
+            __syncthreads();
         }
-        for (uint32_t e = nl + tid; e < ne; e += IRT_NT) {
-            uint2 ab = A.ev_ab[T.ev_off + e];
-            double t = A.ev_t[T.ev_off + e];
-            if (!al[ab.x] || !al[ab.y]) continue;
-            uint64_t k = ((uint64_t)ab.x << 32) | ab.y;
-            if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            double ot = __shfl_xor(bt, o, 64);
-            uint64_t oab = __shfl_xor(bab, o, 64);
-            if (ev_less(ot, oab, bt, bab)) { bt = ot; bab = oab; }
-        }
-        if ((tid & 63) == 0) { sh.wt[tid >> 6] = bt; sh.wab[tid >> 6] = bab; }
-        __syncthreads();
-        unsigned long long q1 = wall_clock64();
+        const unsigned long long q1 = wall_clock64();
         tsc += q1 - q0;
-        if (tid == 0) {
-            double t0 = sh.wt[0]; uint64_t ab0 = sh.wab[0];
-            for (int w = 1; w < (int)(IRT_NT / 64u); w++) if (ev_less(sh.wt[w], sh.wab[w], t0, ab0)) { t0 = sh.wt[w]; ab0 = sh.wab[w]; }
+        if (!(t0 < 1e300)) break;
+        if (lane == 0) {
             sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
             sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
-            if (t0 < 1e300) {
-                while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
-                uint32_t a = (uint32_t)(ab0 >> 32), b = (uint32_t)ab0;
-                int si = (int)al[a] - 1, sj = (int)al[b] - 1;
-                int ri = sh.rmap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
-                al[a] = 0; al[b] = 0;
-                if (DEATH) { A.tdeath[T.part_off + a] = t0; A.tdeath[T.part_off + b] = t0; }
-                sh.ldead[si]++; sh.ldead[sj]++;
-                sh.n_re++; sh.rxc[ri]++;
-                if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
-                if (sj == 0) sh.cnt[0]--; else if (sj == 1) sh.cnt[1]--; else if (sj == 2) sh.cnt[2]--; else if (sj == 4) sh.cnt[3]--;
-                if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
-                int np = sh.nprod[ri];
-                if (np > 0) {
-                    float4 pa = P[a], pb = P[b];
-                    double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
-                    double wsum = sqDi + sqDj;
-                    double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
-                    double my = (sqDj * pa.y + sqDi * pb.y) / wsum;
-                    double mz = (sqDj * pa.z + sqDi * pb.z) / wsum;
-                    uint32_t nt = sh.n_total;
-                    int made = 0;
-                    for (int pp = 0; pp < np; pp++) {
-                        uint32_t id = nt;
-                        if (id >= T.cap_global) break;
-                        if (id >= T.cap) { atomicOr(A.overflow, 1u); break; }
-                        int ps = sh.prods[ri][pp];
-                        float4 q;
-                        if (np == 1) q = make_float4((float)mx, (float)my, (float)mz, (float)ps);
-                        else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
-                        else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
-                        P[id] = q; al[id] = (uint8_t)(ps + 1); tb[id] = t0;
-                        if (DEATH) A.tdeath[T.part_off + id] = __longlong_as_double(0x7FF0000000000000ll);
-                        A.lst[6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++] = id;
-                        if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
-                        sh.prod_idx[made] = id; sh.prod_sp[made] = (uint32_t)ps; sh.prod_pos[made] = q;
-                        made++;
-                        nt++;
-                    }
-                    sh.np = made;
-                    sh.n_total = nt;
+            sh.nt_n = 0; sh.e0 = sh.n_ev;
+            while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) sh.snap[cp * 5 + s] = (uint32_t)sh.cnt[s]; cp++; }
+            uint32_t a = (uint32_t)(ab0 >> 32), b = (uint32_t)ab0;
+            int si = (int)al[a] - 1, sj = (int)al[b] - 1;
+            int ri = sh.rmap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
+            al[a] = 0; al[b] = 0;
+            if (DEATH) { A.tdeath[T.part_off + a] = t0; A.tdeath[T.part_off + b] = t0; }
+            sh.ldead[si]++; sh.ldead[sj]++;
+            sh.n_re++; sh.rxc[ri]++;
+            if (si == 0) sh.cnt[0]--; else if (si == 1) sh.cnt[1]--; else if (si == 2) sh.cnt[2]--; else if (si == 4) sh.cnt[3]--;
+            if (sj == 0) sh.cnt[0]--; else if (sj == 1) sh.cnt[1]--; else if (sj == 2) sh.cnt[2]--; else if (sj == 4) sh.cnt[3]--;
+            if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
+            int np = sh.nprod[ri];
+            if (np > 0) {
+                float4 pa = P[a], pb = P[b];
+                double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
+                double wsum = sqDi + sqDj;
+                double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
+                double my = (sqDj * pa.y + sqDi * pb.y) / wsum;
+                double mz = (sqDj * pa.z + sqDi * pb.z) / wsum;
+                uint32_t nt = sh.n_total;
+                int made = 0;
+                for (int pp = 0; pp < np; pp++) {
+                    uint32_t id = nt;
+                    if (id >= T.cap_global) break;
+                    if (id >= T.cap) { atomicOr(A.overflow, 1u); break; }
+                    int ps = sh.prods[ri][pp];
+                    float4 q;
+                    if (np == 1) q = make_float4((float)mx, (float)my, (float)mz, (float)ps);
+                    else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
+                    else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
+                    P[id] = q; al[id] = (uint8_t)(ps + 1); tb[id] = t0;
+                    if (DEATH) A.tdeath[T.part_off + id] = __longlong_as_double(0x7FF0000000000000ll);
+                    const size_t slot = 6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++;
+                    A.lst[slot] = id;
+                    A.lpos[slot] = make_float4(q.x, q.y, q.z, __double2float_rd(t0));
+                    if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
+                    sh.prod_idx[made] = id; sh.prod_sp[made] = (uint32_t)ps; sh.prod_pos[made] = q;
+                    made++;
+                    nt++;
                 }
-                // segments of the re-pairing: (product, partner species with a channel)
-                uint32_t u = 0, lc = 0;
-                int ns = 0;
-                for (int pp = 0; pp < sh.np; pp++) {
-                    const uint32_t sp = sh.prod_sp[pp];
-                    for (int sj2 = 0; sj2 < 6; sj2++) {
-                        if (sh.rmap[sp * 6 + sj2] < 0 || sh.llen[sj2] == 0) continue;
-                        sh.seg_u0[ns] = u; sh.seg_ps[ns] = ((uint32_t)pp << 4) | (uint32_t)sj2;
-                        u += sh.llen[sj2];
-                        ns++;
-                        if (2u * sh.ldead[sj2] > sh.llen[sj2] && sh.llen[sj2] >= 64u) lc |= 1u << sj2;
-                    }
-                }
-                sh.seg_u0[ns] = u; sh.nseg = ns; sh.lcomp = lc;
+                sh.np = made;
+                sh.n_total = nt;
             }
+            // segments of the re-pairing: (product, partner species with a channel)
+            uint32_t u = 0, lc = 0;
+            int ns = 0;
+            for (int pp = 0; pp < sh.np; pp++) {
+                const uint32_t sp = sh.prod_sp[pp];
+                for (int sj2 = 0; sj2 < 6; sj2++) {
+                    if (sh.rmap[sp * 6 + sj2] < 0 || sh.llen[sj2] == 0) continue;
+                    sh.seg_u0[ns] = u; sh.seg_ps[ns] = ((uint32_t)pp << 4) | (uint32_t)sj2;
+                    u += sh.llen[sj2];
+                    ns++;
+                    if (2u * sh.ldead[sj2] > sh.llen[sj2] && sh.llen[sj2] >= 64u) lc |= 1u << sj2;
+                }
+            }
+            sh.seg_u0[ns] = u; sh.nseg = ns; sh.lcomp = lc;
         }
         __syncthreads();
-        unsigned long long q2 = wall_clock64();
+        const unsigned long long q2 = wall_clock64();
         trc += q2 - q1;
-        if (!(sh.win_t < 1e300)) break;
         if (sh.lcomp) {
             compact_lists(A, T, sh, al);
-            if (tid == 0) {                   // segment starts after the compaction
+            if (lane == 0) {                   // segment starts after the compaction
                 uint32_t u = 0;
                 for (int g = 0; g < sh.nseg; g++) { sh.seg_u0[g] = u; u += sh.llen[sh.seg_ps[g] & 15u]; }
                 sh.seg_u0[sh.nseg] = u; sh.lcomp = 0;
@@ -1141,43 +1252,62 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt(IrtArgs A)
         }
         if (sh.np) pair_products(A, T, sh, al, tk, sh.win_t, wrep);
         __syncthreads();
-        unsigned long long q3 = wall_clock64();
+        const unsigned long long q3 = wall_clock64();
         if (sh.npend) {
             pdc_drain(A, T, tk, sh, sh.win_t);
             __syncthreads();
         }
-        unsigned long long q4 = wall_clock64();
+        // merge this reaction's events into the block minima: staged (at most STAGE_MAX
+        // events, so at most two blocks), else by rescanning the blocks they went to
+        {
+            const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap, e0 = sh.e0, bs = sh.bs;
+            if (ne > e0) {
+                const uint32_t j0 = e0 / bs, j1 = (ne - 1) / bs;
+                if (sh.nt_n > STAGE_MAX) {
+                    for (uint32_t j = j0; j <= j1; j++) block_min(A, T, sh, al, j);
+                } else {
+                    const uint32_t ns = sh.nt_n;
+                    for (uint32_t j = j0; j <= j1; j++) {
+                        double t = 1e300;
+                        uint64_t k = ~0ull;
+                        if (lane < ns && sh.nt_e[lane] / bs == j) { t = sh.nt_t[lane]; k = sh.nt_ab[lane]; }
+                        wave_min_ev(t, k);
+                        if (lane == 0 && ev_less(t, k, sh.bm_t[j], sh.bm_ab[j])) { sh.bm_t[j] = t; sh.bm_ab[j] = k; }
+                    }
+                }
+                if (lane == 0 && j1 + 1 > sh.nblk) sh.nblk = j1 + 1;
+                __syncthreads();
+            }
+        }
+        const unsigned long long q4 = wall_clock64();
         trp += q3 - q2;
         trd += q4 - q3;
     }
+    const unsigned long long tp2 = wall_clock64();
     if (A.final_ntot) {                       // the final state for dna_irt_oh_final
         const uint32_t nt = sh.n_total;
-        if (al != al_global) for (uint32_t k = tid; k < nt; k += IRT_NT) al_global[k] = al[k];
-        if (tid == 0) A.final_ntot[T.idx] = nt;
+        if (al != al_global) for (uint32_t k = lane; k < nt; k += 64u) al_global[k] = al[k];
+        if (lane == 0) A.final_ntot[T.idx] = nt;
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); wrep += __shfl_xor(wrep, o, 64);
+    for (int o = 32; o > 0; o >>= 1) wrep += __shfl_xor(wrep, o, 64);
+    if (lane == 0) {
+        while (cp < A.nc) { for (int s = 0; s < 5; s++) sh.snap[cp * 5 + s] = (uint32_t)sh.cnt[s]; cp++; }
     }
-    if ((tid & 63) == 0) {
-        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 11;
-        if (wcand) atomicAdd(&g[0], wcand);
-        if (wsamp) atomicAdd(&g[1], wsamp);
-        if (wrep) atomicAdd(&g[2], wrep);
+    __syncthreads();
+    unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE;
+    for (int i = (int)lane; i < A.nc * 5; i += 64) {
+        if (sh.snap[i]) atomicAdd(&g[i], (unsigned long long)sh.snap[i]);
+        if (A.track_tally) A.track_tally[(size_t)T.idx * A.nc * 5 + i] = sh.snap[i];
     }
-    if (tid == 0) {
-        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
-        atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tpl - tp1); atomicAdd(&g[6], tp2 - tpl); atomicAdd(&g[2], tsc);
-        atomicAdd(&g[3], trc); atomicAdd(&g[4], trp); atomicAdd(&g[5], nit); atomicAdd(&g[7], trd);
-    }
-    if (tid == 0) {
-        while (cp < A.nc) { for (int s = 0; s < 5; s++) s_loc[cp * 5 + s] += sh.cnt[s]; cp++; }
-        unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE;
-        for (int i = 0; i < A.nc * 5; i++) if (s_loc[i]) atomicAdd(&g[i], s_loc[i]);
-        if (A.track_tally)
-            for (int i = 0; i < A.nc * 5; i++) A.track_tally[(size_t)T.idx * A.nc * 5 + i] = (uint32_t)s_loc[i];
+    if (lane == 0) {
         for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
+        if (wrep) atomicAdd(&g[DNA_MAX_CKPT * 5 + 13], wrep);
+        unsigned long long *pf = g + DNA_MAX_CKPT * 5 + 14;
+        atomicAdd(&pf[6], tp1 - tp0); atomicAdd(&pf[2], tsc); atomicAdd(&pf[3], trc); atomicAdd(&pf[4], trp);
+        atomicAdd(&pf[5], nit); atomicAdd(&pf[7], trd);
+        (void)tp2;
     }
 }
 
@@ -1479,7 +1609,13 @@ struct IrtBufs {
     size_t tally_bytes = 0;
 };
 
-static size_t bytes_scratch(uint64_t part, uint64_t rad_n) { return std::max<size_t>(part * (16 + 16 + 8 + 4 * 6 + 1 + 8) + 256, rad_n / 1024 * 4 + 64); }
+// per particle slot: position 16, sorted copy 16, birth time 8, species lists 24, alive 1,
+// death time 8, then (256-aligned) the list positions 96
+static size_t lpos_offset(uint64_t part) { return ((size_t)part * (16 + 16 + 8 + 4 * 6 + 1 + 8) + 8 + 255) & ~(size_t)255; }
+static size_t bytes_scratch(uint64_t part, uint64_t rad_n)
+{
+    return std::max<size_t>(lpos_offset(part) + (size_t)part * 6 * 16 + 256, rad_n / 1024 * 4 + 64);
+}
 static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
 {
     return tally_bytes + 256 + G.tr.size() * sizeof(IrtTrack) + (G.K + 1) * 4 + (size_t)G.run * 4 + tt_bytes + 1536;
@@ -1544,6 +1680,7 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     A.rad = d_rad; A.idx = b.idx; A.tracks = b.tr;
     A.pos = (float4 *)sp; A.sorted = (float4 *)(sp + part * 16); A.tb = (double *)(sp + part * 32);
     A.lst = (uint32_t *)(sp + part * 40); A.alive = (uint8_t *)(sp + part * 64);
+    A.lpos = (float4 *)(sp + lpos_offset(part));
     A.tdeath = nullptr;                         // set with final_ntot, in the space behind alive[]
     A.ev_t = (double *)b.events; A.ev_ab = (uint2 *)(b.events + evs * 8);
     A.pend = (uint4 *)(b.events + evs * 16);
@@ -1553,15 +1690,26 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in global
     // memory above; the event loop reuses that LDS for alive flags and events
     A.lds_slots = std::min<uint32_t>(G.maxn_all, 3072u);
+    // the loop kernel keeps a track's alive flags in LDS when its slots fit in AL_LDS_MAX
+    // bytes (16 one-wavefront workgroups per CU with LoopShared)
+    uint32_t capmax = 0;
+    for (auto &t : G.tr) capmax = t.cap > capmax ? t.cap : capmax;
+    A.al_lds = (std::min<uint32_t>(capmax, AL_LDS_MAX) + 15u) & ~15u;
+    A.tstate = (uint32_t *)(b.events + evs * 32);
     return A;
 }
 
 static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
 {
     const size_t shmem = (size_t)A.lds_slots * 16;
-    // the death-time stores are compiled only into the variant that keeps the final state
-    if (A.tdeath) hipLaunchKernelGGL((k_irt<true>), dim3(grid), dim3(block), shmem, st, A);
-    else hipLaunchKernelGGL((k_irt<false>), dim3(grid), dim3(block), shmem, st, A);
+    // the death-time stores are compiled only into the variants that keep the final state
+    if (A.tdeath) {
+        hipLaunchKernelGGL((k_irt_pairs<true>), dim3(grid), dim3(block), shmem, st, A);
+        hipLaunchKernelGGL((k_irt_loop<true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+    } else {
+        hipLaunchKernelGGL((k_irt_pairs<false>), dim3(grid), dim3(block), shmem, st, A);
+        hipLaunchKernelGGL((k_irt_loop<false>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+    }
 }
 
 // Workgroup size: the option, or autotuned per track-size class (log2 of the mean
@@ -1670,7 +1818,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         uint64_t part = 0, evs = 0;
         irt_size(G, attempt, ev_mult, part, evs);
         DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch, bytes_scratch(part, rad_n)));
-        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + 256));
+        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + (size_t)ntr * TSTATE * 4 + 256));
         const size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
         const size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
         DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, bytes_block(G, tally_bytes, tt_bytes)));
@@ -1836,7 +1984,7 @@ static int plan_build(dna_irt_plan *P)
     irt_size(G, P->attempt, P->ev_mult, part, evs);
     P->tally_bytes = (size_t)G.n_inst * TALLY_STRIDE * 8;
     const size_t s_bytes = (bytes_scratch(part, G.rad_n) + 255) & ~(size_t)255;
-    const size_t e_bytes = (evs * 32 + 256 + 255) & ~(size_t)255;
+    const size_t e_bytes = (evs * 32 + (size_t)ntr * TSTATE * 4 + 256 + 255) & ~(size_t)255;
     const size_t b_bytes = bytes_block(G, P->tally_bytes, 0);
     DNA_HIP(hipMalloc((void **)&P->buf, s_bytes + e_bytes + b_bytes + 256));
     IrtBufs b;
