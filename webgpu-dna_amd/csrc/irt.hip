@@ -546,33 +546,44 @@ __device__ __forceinline__ void pdc_setup(const PendPair &q, double &a, double &
 //  * for z >= 0, 1/(z + sqrt(z^2 + 2)) < (sqrt(pi)/2) erfcx(z) <= 1/(z + sqrt(z^2 + 4/pi))
 //    (Abramowitz & Stegun 7.1.13) bracket f; z and a sqrt(pi X) are taken with 1e-5
 //    relative margins (the bounds decrease in z) and f with 1e-5 absolute ones.
+// (The bracket below takes its FP32 divisions, square roots and the reciprocal of X with
+// the hardware instructions, each within a few ulp (< 1e-6 relative): far inside its
+// 1e-5 and 1e-4 margins, and the exact path decides every trial they do not.)
+__device__ __forceinline__ float fdiv_bound(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
 __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q, double M, uint64_t ps,
                                           uint32_t trial, double &X)
 {
-    double U = dna_u01(ps, 7u + 2u * trial);
+    // draws 7 + 2 trial and 8 + 2 trial of the pair stream: dna_u01(ps, n) is the finaliser
+    // of ps + G (n + 1), so the second argument is the first plus G
+    const uint64_t x0 = ps + DNA_GOLDEN * (uint64_t)(8u + 2u * trial);
+    double U = (double)(dna_mix64(x0) >> 11) * 1.1102230246251565404e-16;
     double v;
     if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
     else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
-    U = dna_u01(ps, 8u + 2u * trial);
+    U = (double)(dna_mix64(x0 + DNA_GOLDEN) >> 11) * 1.1102230246251565404e-16;
     {
         const bool c1 = X <= 2.0 * b / a, c2 = X >= 2.0 * b / a;
-        const float E = expf(-(float)(b * b / X));
+        const float Xf = (float)X, rX = __builtin_amdgcn_rcpf(Xf);
+        // b^2 / X to a few ulp: the exponent's error (< 3e-7 |b^2/X|) moves E by under 3e-5
+        // of itself while E is above the 1e-37 absolute margin (|b^2/X| < 87)
+        const float E = expf(-((float)(b * b) * rX));
         const float Ehi = E * 1.0001f + 1e-37f;
         const float fU = (float)U;
         // the value compared with lam: U (X below 2b/a) or U M / X (above); X = 2b/a exactly
         // (both tests) and NaN go to the exact path
-        const float w = c1 ? fU : fU * (float)M / (float)X;
+        const float w = c1 ? fU : fU * (float)M * rX;
         const float whi = w * 1.0001f, wlo = w * 0.9999f;
         if (c1 != c2) {
             if (wlo > Ehi) return false;
-            const float Xf = (float)X, bf = (float)b;
+            const float bf = (float)b;
             if (bf >= 0.f) {
-                const float sq = sqrtf(Xf);
-                const float z = bf / sq + (float)a * sq;
+                const float sq = __builtin_amdgcn_sqrtf(Xf);
+                const float z = fdiv_bound(bf, sq) + (float)a * sq;
                 const float K = (float)a * sq * 1.7724539f;                 // a sqrt(pi X)
                 const float zl = z * 0.99999f, zh = z * 1.00001f;
-                const float Lh = 1.1283792f / (zh + sqrtf(zh * zh + 2.f));   // erfcx(z) >  Lh
-                const float Ul = 1.1283792f / (zl + sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
+                const float Lh = fdiv_bound(1.1283792f, zh + __builtin_amdgcn_sqrtf(zh * zh + 2.f));        // erfcx(z) >  Lh
+                const float Ul = fdiv_bound(1.1283792f, zl + __builtin_amdgcn_sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
                 const float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
                 const float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
                 const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
