@@ -125,14 +125,21 @@ typedef struct dna_chem_out {
     uint64_t n_events;                 /* candidate reaction events sampled  */
     double   t_ms;                     /* device time of the whole call      */
     double   t_kernel_ms;              /* device time of the per-track kernel */
-    /* algorithmic work: pairs with a reaction channel (distance evaluated), pairs within
-     * the cut-off (reaction time sampled), product re-pairings with a Brownian step */
+    /* work: pairs with a reaction channel whose distance was evaluated, pairs within the
+     * cut-off (reaction time sampled), product re-pairings with a Brownian step */
     uint64_t n_pair_cand, n_pair_sampled, n_repair;
     double   algo_flops;               /* FP64 operations by the static cost model */
-    uint64_t prof[8];                  /* per-phase 100 MHz clock sums over tracks: [0] load,
-                                          [1] initial pairs, [2] event scan, [3] reaction,
-                                          [4] re-pairing scan, [5] event-loop iterations,
-                                          [6] initial sampling drain, [7] re-pairing drain */
+    uint64_t prof[8];                  /* per-phase 100 MHz clock sums over tracks: [0] load
+                                          and the initial-pair cells, [1] initial pairs,
+                                          [2] event scan, [3] reaction, [4] re-pairing scan,
+                                          [5] event-loop iterations, [6] initial sampling
+                                          drain, [7] re-pairing drain */
+    /* the worker's all-pairs count (pairs of alive initial particles with a reaction
+     * channel; n_pair_cand is the part of it that the cells left to visit), and the species
+     * list slots the re-pairing scanned (n_repair counts the worker's re-pairings: every
+     * alive partner with a channel).  algo_flops counts the worker's work:
+     * 9 n_pair_all + 6 n_pair_sampled + 167 n_repair. */
+    uint64_t n_pair_all, n_repair_scanned;
 } dna_chem_out;
 
 /* DNA fiber target, DNATarget of src/physics/types.ts:23-37 / buildDNATarget
@@ -260,6 +267,17 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * 4096 primaries on 256 CUs, 64 from 16384 primaries), or 1, 2, 4, ..., 64.  Results do
  * not depend on it. */
 #define DNA_OPT_PA_LANES        4
+/* DNA_OPT_IRT_CELL: cells of the initial-pair search of the IRT (the spatial hash that
+ * stands in for the worker's all-pairs scan, public/irt-worker.js:541-558; the cell
+ * semantics of src/shaders/chemistry.wgsl:66-80 with one axis): 0 = autotune together with
+ * DNA_OPT_IRT_BLOCK (every pair of candidates timed twice, fastest best-of-two kept per
+ * track-size class), -1 = none (all pairs of each reaction channel), or m = 1, 2, 4:
+ * slabs of width R_CUT / m (R_CUT = 1.45 + 2 sqrt(8 x 9.46 x 1000) nm, plus 0.1% + 1 nm)
+ * across the track's principal axis, each particle paired with the particles of the
+ * m slabs on either side.  DNA_OPT_IRT_CELL_LAST (get only): the cells the last IRT launch
+ * used.  The cells leave out only pairs beyond R_CUT: results do not depend on them. */
+#define DNA_OPT_IRT_CELL        5
+#define DNA_OPT_IRT_CELL_LAST   6
 int dna_set_option(struct dna_ctx *ctx, int option, int value);
 int dna_get_option(struct dna_ctx *ctx, int option, int *value);
 

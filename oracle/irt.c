@@ -203,6 +203,7 @@ static ev hpop(heap *h)
 typedef struct {
     float *x, *y, *z; int8_t *sp; uint8_t *alive; double *tb, *td; int64_t cap;
     heap h;
+    uint64_t n_all, n_samp, n_rep;   /* pairs with a channel, within R_CUT, re-pairings */
 } work;
 
 static void work_reserve(work *w, int64_t cap)
@@ -232,6 +233,7 @@ static void pair_with_alive(work *w, uint64_t tk, uint32_t idx, uint32_t n_total
         int sj = w->sp[jj];
         int ri = rxnMap[si * 6 + sj];
         if (ri < 0) continue;
+        w->n_rep++;
         double Dj = IRT_D[sj];
         uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
         uint64_t ps = dna_pair_state(tk, a, b);
@@ -306,11 +308,13 @@ static uint32_t run_track(work *w, const float *rad, const uint64_t *idx, uint32
             int sj = w->sp[j];
             int ri = rxnMap[si * 6 + sj];
             if (ri < 0) continue;
+            w->n_all++;
             double ddx = (double)w->x[i] - (double)w->x[j];
             double ddy = (double)w->y[i] - (double)w->y[j];
             double ddz = (double)w->z[i] - (double)w->z[j];
             double r2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (r2 > R_CUT2) continue;
+            w->n_samp++;
             double t = sample_irt(sqrt(r2), ri, Di + IRT_D[sj], dna_pair_state(tk, i, j));
             if (t >= 0 && t < DNA_T_END_NS) hpush(&w->h, t, i, j);
         }
@@ -460,6 +464,7 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
     uint64_t (*tally)[DNA_MAX_CKPT][5] = calloc((size_t)n_threads, sizeof(*tally));
     uint64_t (*rxc)[DNA_N_RXN] = calloc((size_t)n_threads, sizeof(*rxc));
     uint64_t *nre = calloc((size_t)n_threads, 8), *nev = calloc((size_t)n_threads, 8);
+    uint64_t (*work3)[3] = calloc((size_t)n_threads, sizeof(*work3));
     float **oh_tr = oh_out ? (float **)calloc(ntr ? ntr : 1, sizeof(float *)) : NULL;
     uint64_t *oh_tn = oh_out ? (uint64_t *)calloc(ntr ? ntr : 1, 8) : NULL;
     double **pt_tr = pt_out ? (double **)calloc(ntr ? ntr : 1, sizeof(double *)) : NULL;
@@ -483,13 +488,17 @@ static int irt_instance(const float *rad, uint64_t rad_n, uint32_t n_therm, floa
             if (oh_tr) collect_oh(&w, nt, tk, &oh_tr[t], &oh_tn[t]);
             if (pt_tr) collect_particles(&w, nt, (uint64_t)tr[t].pid, &pt_tr[t], &pt_tn[t]);
         }
+        work3[tid][0] = w.n_all; work3[tid][1] = w.n_samp; work3[tid][2] = w.n_rep;
         free(w.x); free(w.y); free(w.z); free(w.sp); free(w.alive); free(w.tb); free(w.td); free(w.h.v);
     }
     for (int th = 0; th < n_threads; th++) {
         for (int c = 0; c < nc; c++) for (int s = 0; s < 5; s++) out->tally[c][s] += tally[th][c][s];
         for (int r = 0; r < 9; r++) out->rxn_counts[r] += rxc[th][r];
         out->n_reacted += nre[th]; out->n_events += nev[th];
+        out->n_pair_all += work3[th][0]; out->n_pair_sampled += work3[th][1]; out->n_repair += work3[th][2];
     }
+    out->n_pair_cand = out->n_pair_all;   /* the worker visits every pair */
+    free(work3);
     double per100 = (double)n_therm * (double)E_eV / 100.0;
     for (int c = 0; c < nc; c++) for (int s = 0; s < 5; s++)
         out->G[c][s] = per100 > 0 ? (double)out->tally[c][s] / per100 : 0.0;

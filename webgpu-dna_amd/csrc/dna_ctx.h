@@ -67,10 +67,13 @@ struct dna_ctx {
     IrtBuffers irt;
     ScoreBuffers sc;
     HashBuffers hs;
-    // launch tuning: IRT workgroup size (0 = autotune per track-size class)
+    // launch tuning: IRT workgroup size and initial-pair cells (0 = autotune per track-size
+    // class; cells: -1 = none, m = cells of R_CUT / m)
     int irt_block = 0;
+    int irt_cell = 0;
     int irt_last_block = 0;            // workgroup size of the last IRT launch
-    std::map<int, int> irt_tuned;      // log2(mean particles per track) -> workgroup size
+    int irt_last_cell = 0;             // cells of the last IRT launch (-1 = none)
+    std::map<int, std::pair<int, int>> irt_tuned;   // log2(mean particles per track) -> (size, cells)
     // final IRT state (DNA_OPT_IRT_FINAL) and the alive-OH list made from it
     int irt_final = 0;
     int pa_lanes = 0;                 // DNA_OPT_PA_LANES (0: chosen from the batch size)

@@ -36,6 +36,7 @@ struct IrtConst {
     double D[6], sqrtD[6];
     double R_CUT2, sig_oh, sig_h, sig_eaq;
     float R_CUT_F;                    // R_CUT rounded up to FP32 (a bound, not the test)
+    float CELL_R;                     // reach covered by m cells of the pair hash (> R_CUT)
     int rxnMap[36];
     int type[9], nprod[9], prods[9][2];
 };
@@ -66,6 +67,8 @@ struct IrtArgs {
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy (pairs kernel)
+    int cell_m;                   // initial-pair hash: 0 = one cell per species, m = cells of
+                                  // CELL_R / m along the track's principal axis
     uint32_t *tstate;             // TSTATE u32 per track (launch order): pairs kernel -> loop kernel
     uint32_t al_lds;              // bytes of dynamic LDS for the alive flags (loop kernel)
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
@@ -79,7 +82,8 @@ struct IrtArgs {
     uint64_t seed;
     const uint64_t *seed_ptr;     // when set, the seed is read here (graph replays)
 };
-#define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 5 + 8)   // tallies, rxn, n_re, n_ev, 3 work counters, 8 phase timers
+#define TALLY_STRIDE (DNA_MAX_CKPT * 5 + 9 + 5 + 8 + 2)   // tallies, rxn, n_re, n_ev, 3 work counters,
+                                                       // 8 phase timers, all pairs, re-pair slots scanned
 
 // ------------------------------------------------------------ samplers (FP64)
 // sampleIRT (irt-worker.js:116-184) is split in two.  Stage 1 is everything except the
@@ -380,9 +384,13 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 #define BM_MAX 128u                  // event blocks per track (a block holds 64 k events)
 #define STAGE_MAX 64u                // events staged per reaction (more: blocks rescanned)
 #define AL_LDS_MAX 3584u             // alive flags in LDS up to this many particle slots
-#define TSTATE 16                    // u32 per track handed from k_irt_pairs to k_irt_loop:
+#define TSTATE 128                   // u32 per track handed from k_irt_pairs to k_irt_loop:
                                      // [0] queued pairs, [1..6] species list lengths,
-                                     // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n
+                                     // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n,
+                                     // [12] windows on, [13..15] axis, [16] umin, [17] cinv,
+                                     // [18] cells, [19] cells per bucket, [20] buckets,
+                                     // [32..127] bucket ends, 2 x u16 per word, 16 words per species
+#define WB_MAX 32u                   // buckets of the re-pairing windows
 
 __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint64_t ab2)
 {
@@ -418,12 +426,21 @@ __device__ __forceinline__ uint32_t wave_min_ev(double &t, uint64_t &ab)
 }
 
 // shared state of k_irt_pairs
+#define NX_MAX 128u                       // cells per track of the initial-pair hash
 struct PairShared {
     uint32_t npend;                       // queued pairs
-    uint32_t gcnt[8], gcur[8], goff[8];   // species groups of the sorted copy
-    int      bri[9], nblk;                // channel blocks: reaction, units per row
-    uint32_t bcpr[9], bu0[9], units;
+    uint32_t gcnt[8], goff[8];            // species groups of the sorted copy
+    int      bri[9], nblk;                // channel blocks (reactions with particles)
     unsigned long long c[3];              // initial OH, e-aq, H
+    // the hash: nx cells of width >= CELL_R / m along the unit axis ax, starting at umin.
+    // The sorted copy is ordered by (species, cell); once it is filled, cs[s * nx + k] is
+    // the end of the run of species s in cell k (its start is the previous entry's end).
+    uint32_t nx, m;
+    float    ax[3], umin, cinv;
+    double   red[4][9];                   // per-wavefront partial sums
+    float    rlo[4], rhi[4];
+    uint32_t cs[6 * NX_MAX + 1];
+    uint32_t cu[NX_MAX + 1];              // units of the current channel block before each cell
 };
 
 // shared state of k_irt_loop (one wavefront)
@@ -455,10 +472,16 @@ struct LoopShared {
     // species lists: length and reacted entries; products are appended, reacted particles
     // stay until a list is compacted (when they outnumber the alive ones)
     uint32_t llen[6], ldead[6], lcomp;
-    uint32_t lcnt;
-    // re-pairing segments (product p, partner species sj) over the lists
+    // re-pairing windows (win): the first n0[s] entries of list s are initial particles in
+    // hash-cell order, in buckets of wg cells; wend[s][b] is the end of bucket b
+    uint32_t win, wnx, wg, wnb;
+    float    wax[3], wumin, wcinv;
+    uint32_t n0[6];
+    uint16_t wend[6][WB_MAX];
+    // re-pairing segments (product p, partner species sj, a list range) flattened
     int      nseg;
-    uint32_t seg_u0[13], seg_ps[12];      // first flattened index; p << 4 | sj
+    uint32_t seg_u0[25], seg_lo[24];      // first flattened index, first list entry
+    uint8_t  seg_ps[24];                  // p << 4 | sj
     uint32_t rq[1][128];                  // re-pairings that passed the bound
     uint32_t rqn[1];
     // event selection: blocks of bs events, the minimum live (t, a << 32 | b) of each block
@@ -471,6 +494,10 @@ struct LoopShared {
     double   nt_t[STAGE_MAX];
     uint64_t nt_ab[STAGE_MAX];
     uint32_t snap[DNA_MAX_CKPT * 5];      // species counts at each checkpoint
+    // per-track sums kept here, not in registers: clock ticks of the event scan, the
+    // reaction, the re-pairing and its drain, loop iterations, the worker's re-pairings,
+    // re-pairing list slots scanned
+    unsigned long long acc[7];
 };
 
 extern __shared__ float4 dyn_lds[];
@@ -765,9 +792,9 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
 }
 
 // Re-pair the products of the reaction at t_new with every alive particle that has a
-// reaction channel with them (irt-worker.js:430-465).  The (product, partner species)
-// segments of one reaction are flattened into one index range over the species lists
-// (table built by the reacting thread); list entries that have reacted are skipped.
+// reaction channel with them (irt-worker.js:430-465).  The (product, partner species, list
+// range) segments of one reaction (build_segments) are flattened into one index range over
+// the species lists; list entries that have reacted are skipped.
 // Product 0 is not paired with product 1: it does not exist yet at that point of the
 // worker's loop (irt-worker.js:618-644).  Each pair first meets the necessary condition
 // of cannot_react on a lower bound of its distance after the partner's Brownian step;
@@ -775,7 +802,7 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
 // r0 = |product - partner| (irt-worker.js:443-456), the first sampling stage), so that
 // part runs on full wavefronts; pairs that can react are queued for the drain.
 __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
-                              uint64_t tk, double t_new, unsigned long long &wrep)
+                              uint64_t tk, double t_new)
 {
     const uint32_t *lst = A.lst + 6 * T.part_off;
     const float4 *lps = A.lpos + 6 * T.part_off;
@@ -794,12 +821,11 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
             int sg = 0;
             while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
             const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
-            const size_t slot = (size_t)sj * T.cap + (u - sh.seg_u0[sg]);
+            const size_t slot = (size_t)sj * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
             const uint32_t jj = lst[slot];
             const float4 pj = lps[slot];          // x, y, z, birth time rounded down
             const uint32_t idx = sh.prod_idx[p];
             if (al[jj] && jj != idx && !(p == 0 && np == 2 && jj == pidx1)) {
-                wrep++;
                 const uint32_t pc = sh.prod_sp[p] * 6u + sj;
                 const float4 pi = sh.prod_pos[p];
                 const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
@@ -861,43 +887,183 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
     }
 }
 
-// Drop reacted entries from the species lists flagged in sh.lcomp (all threads, after a
-// barrier; ends with one).  The order inside a list does not matter: every chunk of 256
-// entries is read before any of its slots is rewritten, and an entry is only ever moved
-// to a lower position.
+// Drop reacted entries from the species lists flagged in sh.lcomp (the loop kernel's one
+// wavefront).  Stable: the entries keep their order, so the initial part of a list stays in
+// cell order, and the ends of its buckets are moved with it.  In place: a chunk of 64
+// entries is read (the loads complete before the ballot that the stores depend on) before
+// any of its slots is rewritten, and an entry only ever moves to a lower position.
 __device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al)
 {
     uint32_t *lst = A.lst + 6 * T.part_off;
     float4 *lps = A.lpos + 6 * T.part_off;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t mask = sh.lcomp;
     for (int s = 0; s < 6; s++) {
         if (!(mask & (1u << s))) continue;
         uint32_t *L = lst + (size_t)s * T.cap;
         float4 *LP = lps + (size_t)s * T.cap;
-        const uint32_t len = sh.llen[s];
-        if (threadIdx.x == 0) sh.lcnt = 0;
-        __syncthreads();
-        for (uint32_t c0 = 0; c0 < len; c0 += IRT_NT) {
-            const uint32_t i = c0 + threadIdx.x;
-            const uint32_t k = i < len ? L[i] : 0u;
-            const float4 q = i < len ? LP[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            const bool live = i < len && al[k];
-            __syncthreads();
-            if (live) { const uint32_t d = atomicAdd(&sh.lcnt, 1u); L[d] = k; LP[d] = q; }
+        // the ranges: the buckets of the initial part in order (they end at n0), then the
+        // appended products up to the list's length
+        const uint32_t len = sh.llen[s], nb = sh.wnb;
+        uint32_t dst = 0, i0 = 0, nkeep = 0;
+        for (uint32_t r = 0; r <= nb; r++) {
+            const uint32_t i1 = r < nb ? (uint32_t)sh.wend[s][r] : len;
+            for (uint32_t c0 = i0; c0 < i1; c0 += 64u) {
+                const uint32_t i = c0 + lane;
+                const bool in = i < i1;
+                const uint32_t k = in ? L[i] : 0u;
+                const float4 q = in ? LP[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                const bool live = in && al[k];
+                const uint64_t m = __ballot(live);
+                if (live) { const uint32_t d = dst + (uint32_t)__popcll(m & below); L[d] = k; LP[d] = q; }
+                dst += (uint32_t)__popcll(m);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (r < nb) { if (lane == 0) sh.wend[s][r] = (uint16_t)dst; nkeep = dst; }
+            i0 = i1;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) { sh.llen[s] = sh.lcnt; sh.ldead[s] = 0; }
-        __syncthreads();
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) { sh.llen[s] = dst; sh.ldead[s] = 0; sh.n0[s] = nkeep; }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
-// Initial pairs of the channel blocks (see k_irt step 2), with the species-sorted copy in
+// projection onto a hash axis (explicit FMAs: the same value wherever it is computed)
+__device__ __forceinline__ float axis_u(float a0, float a1, float a2, const float4 &p)
+{
+    return fmaf(a2, p.z, fmaf(a1, p.y, a0 * p.x));
+}
+
+// the cell of axis coordinate u: the same expression as the pairs kernel's keys, monotone
+// in u
+__device__ __forceinline__ uint32_t hash_cell(float u, float umin, float cinv, uint32_t nx)
+{
+    const float c = (u - umin) * cinv;
+    return c > 0.f ? (c < (float)(nx - 1) ? (uint32_t)c : nx - 1) : 0u;   // NaN: cell 0
+}
+
+// The re-pairing segments of the current reaction (the loop kernel's one wavefront): lane
+// (p * 6 + sj) * 2 + part covers product p and partner species sj, part 0 the initial part
+// of list sj, part 1 its appended products.  With windows, part 0 is cut to the buckets
+// within reach of the product: an initial particle (born at t = 0) whose axis coordinate
+// differs from the product's by more than reach = (R_CUT + 20.4 sig) (1.001) + 1 nm, with
+// sig = sqrt(2 D_sj t_new) the bound of its Brownian step, is farther than R_CUT + 20.4 sig
+// (the axis is a unit vector; the margins cover the FP32 coordinates), which is exactly the
+// distance at which pair_products' first screen drops it without a draw: leaving it out
+// changes nothing.  Also counts the re-pairings of the worker (every alive partner with a
+// channel, public/irt-worker.js:430-465) and the list slots to scan.
+__device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t pc = lane >> 1, part = lane & 1u, p = pc / 6u, sj = pc - 6u * (pc / 6u);
+    const int np = sh.np;
+    uint32_t lo = 0, len = 0, nalive = 0;
+    if (lane < 24u && (int)p < np) {
+        const uint32_t sp = sh.prod_sp[p];
+        const uint32_t ll = sh.llen[sj], n0 = sh.n0[sj];
+        if (sh.rmap[sp * 6 + sj] >= 0 && ll > 0) {
+            if (part) {
+                lo = n0; len = ll - n0;
+                nalive = ll - sh.ldead[sj] - (sp == sj ? 1u : 0u) - ((p == 0 && np == 2 && sh.prod_sp[1] == sj) ? 1u : 0u);
+            } else if (n0) {
+                const float u = axis_u(sh.wax[0], sh.wax[1], sh.wax[2], sh.prod_pos[p]);
+                const float sg = t_new > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * t_new)) * 1.00001f : 0.f;
+                const float reach = (c_irt.R_CUT_F + 20.4f * sg) * 1.001f + 1.0f;
+                const uint32_t blo = hash_cell(u - reach, sh.wumin, sh.wcinv, sh.wnx) / sh.wg;
+                const uint32_t bhi = hash_cell(u + reach, sh.wumin, sh.wcinv, sh.wnx) / sh.wg;
+                lo = blo ? sh.wend[sj][blo - 1] : 0u;
+                len = sh.wend[sj][bhi] - lo;
+            }
+        }
+    }
+    uint32_t x = len;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    const uint64_t m = __ballot(len > 0);
+    if (len > 0) {
+        const uint32_t k = (uint32_t)__popcll(m & below);
+        sh.seg_u0[k] = x - len; sh.seg_lo[k] = lo; sh.seg_ps[k] = (uint8_t)((p << 4) | sj);
+    }
+    uint32_t na = nalive;
+    for (int o = 32; o > 0; o >>= 1) na += __shfl_xor(na, o, 64);
+    const uint32_t total = __shfl(x, 63, 64);
+    if (lane == 0) {
+        const int ns = __popcll(m);
+        sh.nseg = ns; sh.seg_u0[ns] = total;
+        sh.acc[5] += na; sh.acc[6] += total;
+    }
+}
+
+// exclusive scan of a[0..len) in LDS by wavefront 0, the total into a[len]; the caller
+// synchronizes the workgroup before and after
+__device__ __forceinline__ void wave0_scan(uint32_t *a, uint32_t len)
+{
+    if (threadIdx.x >= 64) return;
+    const uint32_t lane = threadIdx.x;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < len; base += 64u) {
+        const uint32_t i = base + lane;
+        const uint32_t v = i < len ? a[i] : 0u;
+        uint32_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= (uint32_t)o) x += y;
+        }
+        if (i < len) a[i] = carry + x - v;
+        carry += __shfl(x, 63, 64);
+    }
+    if (lane == 0) a[len] = carry;
+}
+
+// the (species s, cell k) run of the sorted copy
+__device__ __forceinline__ uint32_t run_end(const PairShared &sh, uint32_t s, uint32_t k) { return sh.cs[s * sh.nx + k]; }
+__device__ __forceinline__ uint32_t run_start(const PairShared &sh, uint32_t s, uint32_t k)
+{
+    const uint32_t q = s * sh.nx + k;
+    return q ? sh.cs[q - 1] : 0u;
+}
+
+// The pairs of one cell k of a channel block (species sa x sb): the rows are the sa
+// particles of cell k, each row in units of 64 partners.
+//  * sa != sb: the partners are the sb particles of cells k-m .. k+m;
+//  * sa == sb: first the pairs inside cell k, enumerated circularly (row i with
+//    i + d mod n for d = 1..(n-1)/2, and d = n/2 for i < n/2 when n is even: every
+//    unordered pair once, with equal rows), then the particles of cells k+1 .. k+m.
+// Every pair of particles whose cells are at most m apart is visited exactly once.
+struct CellWin { uint32_t r0, nrow, cprA, cpr, wlo, wlen; };
+__device__ __forceinline__ CellWin cell_window(const PairShared &sh, uint32_t sa, uint32_t sb, uint32_t k)
+{
+    CellWin w;
+    const uint32_t nx = sh.nx, m = sh.m;
+    const uint32_t khi = k + m < nx ? k + m : nx - 1;
+    w.r0 = run_start(sh, sa, k);
+    w.nrow = run_end(sh, sa, k) - w.r0;
+    if (sa == sb) {
+        w.cprA = w.nrow >= 2 ? ((w.nrow - 1) / 2 + 1 + 63) / 64 : 0u;
+        w.wlo = run_end(sh, sa, k);
+        w.wlen = run_end(sh, sa, khi) - w.wlo;
+    } else {
+        w.cprA = 0;
+        w.wlo = run_start(sh, sb, k > m ? k - m : 0u);
+        w.wlen = run_end(sh, sb, khi) - w.wlo;
+    }
+    w.cpr = w.cprA + (w.wlen + 63) / 64;
+    return w;
+}
+
+// Initial pairs of the channel blocks (see k_irt_pairs step 2), with the sorted copy in
 // LDS (SG_LDS: DS loads) or in global memory.  A pair is dropped without FP64 work when an
 // FP32 bound decides it: the FP32 squared distance is within 3.1e-7 (relative) of the FP64
 // one (five roundings of non-negative terms), so r2 above R_CUT^2 (1 + 1e-5) or below
 // R_CUT^2 (1 - 1e-5) in FP32 settles the cutoff test, and the bound of cannot_react takes
 // sqrt(r2) (1 - 1e-5) and the draw's 24-bit lower bound.  The exact FP64 distance is
-// computed near the cutoff and for the pairs that are queued.
+// computed near the cutoff and for the pairs that are queued.  The cells leave out only
+// pairs farther apart than R_CUT (see the hash in k_irt_pairs), so the queued pairs are
+// those of the full enumeration.
 template <bool SG_LDS>
 __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T, PairShared &sh, uint64_t tk,
                                             unsigned long long &wcand, unsigned long long &wsamp)
@@ -907,35 +1073,46 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
     // (readfirstlane -> scalar registers), so the per-pair path has no table loads
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
+    const uint32_t nx = __builtin_amdgcn_readfirstlane(sh.nx);
     const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
     const double RC2 = c_irt.R_CUT2;
     const float RC2_hi = (float)(RC2 * 1.00001), RC2_lo = (float)(RC2 * 0.99999);
     for (int q = 0; q < nb; q++) {
         const int ri = __builtin_amdgcn_readfirstlane(sh.bri[q]);
-        const int sa = BA[ri], sb = BB[ri];
-        const uint32_t cpr = __builtin_amdgcn_readfirstlane(sh.bcpr[q]);
-        const uint32_t nA = __builtin_amdgcn_readfirstlane(sh.gcnt[sa]);
-        const uint32_t nB = __builtin_amdgcn_readfirstlane(sh.gcnt[sb]);
-        const uint32_t oA = __builtin_amdgcn_readfirstlane(sh.goff[sa]);
-        const uint32_t oB = __builtin_amdgcn_readfirstlane(sh.goff[sb]);
-        const uint32_t units = nA * cpr;
+        const uint32_t sa = BA[ri], sb = BB[ri];
+        // units of each cell, then their running sum
+        for (uint32_t k = tid; k < nx; k += IRT_NT) {
+            const CellWin w = cell_window(sh, sa, sb, k);
+            sh.cu[k] = w.nrow * w.cpr;
+        }
+        __syncthreads();
+        wave0_scan(sh.cu, nx);
+        __syncthreads();
+        const uint32_t units = __builtin_amdgcn_readfirstlane(sh.cu[nx]);
         const RxB kb = rx_bound(rx_constants(ri, c_irt.D[sa] + c_irt.D[sb]));
-        const uint32_t h = (nA - 1) / 2;
+        uint32_t k = 0, ku0 = 0, ku1 = sh.cu[1];
+        CellWin w = cell_window(sh, sa, sb, 0);
         for (uint32_t ul = wave; ul < units; ul += IRT_NT / 64u) {
-            const uint32_t i = ul / cpr, c = ul - i * cpr;
-            const float4 pi = Sg[oA + i];
+            if (ul >= ku1) {                          // the next cell with units
+                do { k++; ku0 = ku1; ku1 = sh.cu[k + 1]; } while (ul >= ku1);
+                w = cell_window(sh, sa, sb, k);
+            }
+            const uint32_t loc = ul - ku0;
+            const uint32_t i = loc / w.cpr, c = loc - i * w.cpr;
+            const float4 pi = Sg[w.r0 + i];
             uint32_t j;
             bool valid;
-            if (sa == sb) {
-                uint32_t dmax = h + (((nA & 1u) == 0 && i < nA / 2) ? 1u : 0u);
-                uint32_t d = c * 64u + lane + 1u;
+            if (c < w.cprA) {
+                const uint32_t n = w.nrow, h = (n - 1) / 2;
+                const uint32_t dmax = h + (((n & 1u) == 0 && i < n / 2) ? 1u : 0u);
+                const uint32_t d = c * 64u + lane + 1u;
                 valid = d <= dmax;
-                j = i + d; if (j >= nA) j -= nA;
-                j += oA;
+                uint32_t jl = i + d; if (jl >= n) jl -= n;
+                j = w.r0 + jl;
             } else {
-                uint32_t jj = c * 64u + lane;
-                valid = jj < nB;
-                j = oB + jj;
+                const uint32_t jj = (c - w.cprA) * 64u + lane;
+                valid = jj < w.wlen;
+                j = w.wlo + jj;
             }
             if (!valid) continue;
             wcand++;
@@ -955,6 +1132,7 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
             double ddx = (double)pi.x - (double)pj.x, ddy = (double)pi.y - (double)pj.y, ddz = (double)pi.z - (double)pj.z;
             push_pending(A, T, sh, a, b, oi < oj ? sa : sb, oi < oj ? sb : sa, sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
         }
+        __syncthreads();                              // cu is rewritten by the next block
     }
 }
 
@@ -972,7 +1150,7 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
     double *tb = A.tb + T.part_off;
     const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
     if (tid == 0) { sh.npend = 0; sh.c[0] = 0; sh.c[1] = 0; sh.c[2] = 0; }
-    if (tid < 8) { sh.gcnt[tid] = 0; sh.gcur[tid] = 0; }
+    if (tid < 8) sh.gcnt[tid] = 0;
     __syncthreads();
     // phase timers (100 MHz wall clock, thread 0): load, pairs
     const unsigned long long tp0 = wall_clock64();
@@ -1010,44 +1188,129 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
     }
     for (int o = 32; o > 0; o >>= 1) { c0 += __shfl_xor(c0, o, 64); c1 += __shfl_xor(c1, o, 64); c2 += __shfl_xor(c2, o, 64); }
     if ((tid & 63) == 0) { atomicAdd(&sh.c[0], c0); atomicAdd(&sh.c[1], c1); atomicAdd(&sh.c[2], c2); }
+    // 2. the sorted copy: the alive initial particles ordered by (species, cell), original
+    // index in .w.  Without the hash (cell_m = 0) a species is one cell.  With it, the cells
+    // are slabs of width 1 / cinv >= CELL_R / m across the track's principal axis (the top
+    // eigenvector of the positions' covariance, by power iteration): two particles within
+    // R_CUT have axis coordinates within R_CUT (the axis is a unit vector), so their cells
+    // are at most m apart, and the cell windows of pair_blocks keep every pair that the
+    // full enumeration would sample (CELL_R covers the FP32 error of the coordinate and of
+    // the cell index, and clamping the index to the nx cells only brings cells closer).
+    // The coordinate, then the key, is kept in tb[] between the passes, so that every pass
+    // sees the same key.
+    const uint32_t wave = tid >> 6, lane = tid & 63u, nw = IRT_NT / 64u;
+    const bool hashed = A.cell_m > 0;
     __syncthreads();
-    const unsigned long long tp1 = wall_clock64();
-    // 2. initial pairs (irt-worker.js:541-558).  The particles are copied in species
-    // order (Sg, original index in .w) and only the species blocks that have a reaction
-    // channel are enumerated, in units of one row particle x 64 partners, so every lane
-    // of a wavefront evaluates the same reaction with the same constants.  Within a
-    // species the pairs are enumerated circularly, (i, i+d mod m) for d = 1..(m-1)/2 and
-    // d = m/2 for i < m/2 when m is even: every unordered pair exactly once, equal rows.
-    // Pairs are identified by their original indices, so the enumeration order does not
-    // change any draw or the event order.
-    unsigned long long wcand = 0, wsamp = 0;
-    float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
-    if (tid == 0) {
-        uint32_t o = 0;
-        for (int g = 0; g < 6; g++) { sh.goff[g] = o; o += sh.gcnt[g]; }
-        // channel blocks (irt-worker.js:216-226): (a, b, reaction)
-        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
-        uint32_t u = 0;
-        int nb = 0;
-        for (int r = 0; r < 9; r++) {
-            uint32_t nA = sh.gcnt[BA[r]], nB = sh.gcnt[BB[r]];
-            uint32_t units = 0, cpr = 0;
-            if (BA[r] == BB[r]) { if (nA >= 2) { cpr = ((nA - 1) / 2 + 1 + 63) / 64; units = nA * cpr; } }
-            else if (nA && nB) { cpr = (nB + 63) / 64; units = nA * cpr; }
-            if (units) { sh.bri[nb] = r; sh.bcpr[nb] = cpr; sh.bu0[nb] = u; u += units; nb++; }
+    if (hashed) {
+        double m1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t k = tid; k < n; k += IRT_NT) {
+            if (!al[k]) continue;
+            const float4 q = P[k];
+            const double x = q.x, y = q.y, z = q.z;
+            m1[0] += x; m1[1] += y; m1[2] += z;
+            m1[3] += x * x; m1[4] += x * y; m1[5] += x * z; m1[6] += y * y; m1[7] += y * z; m1[8] += z * z;
         }
-        sh.nblk = nb; sh.units = u;
-    }
+        for (int v = 0; v < 9; v++) {
+            double r = m1[v];
+            for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+            if (lane == 0) sh.red[wave][v] = r;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double S[9];
+#pragma unroll
+            for (int v = 0; v < 9; v++) { S[v] = 0; for (uint32_t w = 0; w < nw; w++) S[v] += sh.red[w][v]; }
+            double N = 0;
+            for (int g = 0; g < 6; g++) N += sh.gcnt[g];
+            double a0 = 1.0, a1 = 0.0, a2 = 0.0;
+            if (N >= 2) {
+                const double mx = S[0] / N, my = S[1] / N, mz = S[2] / N;
+                const double cxx = S[3] / N - mx * mx, cxy = S[4] / N - mx * my, cxz = S[5] / N - mx * mz;
+                const double cyy = S[6] / N - my * my, cyz = S[7] / N - my * mz, czz = S[8] / N - mz * mz;
+                // start on the axis of the largest variance
+                if (cyy > cxx && cyy >= czz) { a0 = 0.0; a1 = 1.0; }
+                else if (czz > cxx && czz > cyy) { a0 = 0.0; a2 = 1.0; }
+                for (int it = 0; it < 32; it++) {
+                    const double w0 = cxx * a0 + cxy * a1 + cxz * a2;
+                    const double w1 = cxy * a0 + cyy * a1 + cyz * a2;
+                    const double w2 = cxz * a0 + cyz * a1 + czz * a2;
+                    const double nrm = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+                    if (!(nrm > 0) || !(nrm < 1e300)) break;
+                    a0 = w0 / nrm; a1 = w1 / nrm; a2 = w2 / nrm;
+                }
+            }
+            sh.ax[0] = (float)a0; sh.ax[1] = (float)a1; sh.ax[2] = (float)a2;
+        }
+        __syncthreads();
+        float lo = 3e38f, hi = -3e38f;
+        for (uint32_t k = tid; k < n; k += IRT_NT) {
+            if (!al[k]) continue;
+            const float u = axis_u(sh.ax[0], sh.ax[1], sh.ax[2], P[k]);
+            tb[k] = (double)u;
+            lo = fminf(lo, u); hi = fmaxf(hi, u);
+        }
+        for (int o = 32; o > 0; o >>= 1) { lo = fminf(lo, __shfl_xor(lo, o, 64)); hi = fmaxf(hi, __shfl_xor(hi, o, 64)); }
+        if (lane == 0) { sh.rlo[wave] = lo; sh.rhi[wave] = hi; }
+        __syncthreads();
+        if (tid == 0) {
+            for (uint32_t w = 1; w < nw; w++) { lo = fminf(lo, sh.rlo[w]); hi = fmaxf(hi, sh.rhi[w]); }
+            const uint32_t m = (uint32_t)A.cell_m;
+            float cinv = (float)m / C.CELL_R;
+            const float ext = (hi > lo && hi - lo < 1e30f) ? hi - lo : 0.f;   // non-finite: one cell
+            // at most NX_MAX cells: wider ones for long tracks (m cells still span CELL_R)
+            if (ext * cinv > (float)(NX_MAX - 1)) cinv = (float)(NX_MAX - 1) / ext;
+            sh.nx = (uint32_t)(ext * cinv) + 1u;
+            if (sh.nx > NX_MAX) sh.nx = NX_MAX;
+            sh.m = m; sh.umin = lo; sh.cinv = cinv;
+        }
+    } else if (tid == 0) { sh.nx = 1; sh.m = 0; }
+    __syncthreads();
+    const uint32_t nx = sh.nx;
+    for (uint32_t k = tid; k <= 6 * nx; k += IRT_NT) sh.cs[k] = 0;
     __syncthreads();
     for (uint32_t k = tid; k < n; k += IRT_NT) {
-        float4 q = P[k];
-        int spc = (int)q.w;
-        if (al[k] && spc >= 0 && spc < 6) {
-            uint32_t slot = sh.goff[spc] + atomicAdd(&sh.gcur[spc], 1u);
-            Sg[slot] = make_float4(q.x, q.y, q.z, __int_as_float((int)k));
+        const uint8_t av = al[k];
+        if (!av) continue;
+        uint32_t key = (uint32_t)(av - 1) * nx;
+        if (hashed) {
+            key += hash_cell((float)tb[k], sh.umin, sh.cinv, nx);
+            tb[k] = (double)key;
         }
+        atomicAdd(&sh.cs[key], 1u);
     }
     __syncthreads();
+    wave0_scan(sh.cs, 6 * nx);
+    __syncthreads();
+    if (tid == 0) {
+        for (int g = 0; g < 6; g++) sh.goff[g] = sh.cs[g * nx];
+        // channel blocks (irt-worker.js:216-226): the reactions whose species are present
+        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+        int nb = 0;
+        for (int r = 0; r < 9; r++) {
+            const uint32_t nA = sh.gcnt[BA[r]], nB = sh.gcnt[BB[r]];
+            if (BA[r] == BB[r] ? nA >= 2 : (nA && nB)) sh.bri[nb++] = r;
+        }
+        sh.nblk = nb;
+    }
+    __syncthreads();
+    unsigned long long wcand = 0, wsamp = 0;
+    float4 *Sg = (n <= A.lds_slots) ? dyn_lds : A.sorted + T.part_off;
+    for (uint32_t k = tid; k < n; k += IRT_NT) {
+        const uint8_t av = al[k];
+        if (!av) continue;
+        uint32_t key = (uint32_t)(av - 1);
+        if (hashed) { key = (uint32_t)tb[k]; tb[k] = 0.0; }
+        const float4 q = P[k];
+        const uint32_t slot = atomicAdd(&sh.cs[key], 1u);
+        Sg[slot] = make_float4(q.x, q.y, q.z, __int_as_float((int)k));
+    }
+    __syncthreads();
+    const unsigned long long tp1 = wall_clock64();
+    // 3. initial pairs (irt-worker.js:541-558) of the species blocks that have a reaction
+    // channel, in units of one row particle x 64 partners, so every lane of a wavefront
+    // evaluates the same reaction with the same constants.  Pairs are identified by their
+    // original indices, so neither the enumeration order nor the cells change any draw or
+    // the event order.
     if (n <= A.lds_slots) pair_blocks<true>(A, T, sh, tk, wcand, wsamp);
     else pair_blocks<false>(A, T, sh, tk, wcand, wsamp);
     // species lists of the alive initial particles = the species groups of Sg
@@ -1069,15 +1332,42 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
         if (wcand) atomicAdd(&g[0], wcand);
         if (wsamp) atomicAdd(&g[1], wsamp);
     }
+    // hand-over to the loop kernel (same launch index).  With the hash, each species list
+    // starts with its initial particles in cell order: the loop kernel's re-pairing reads
+    // only the buckets (runs of consecutive cells) within reach of a product.
+    uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
+    bool win = hashed && nx > 1;
+    for (int g = 0; g < 6; g++) win = win && sh.gcnt[g] < 65536u;
+    const uint32_t wg = (nx + WB_MAX - 1) / WB_MAX, wnb = (nx + wg - 1) / wg;
+    if (win)
+        for (uint32_t k = tid; k < 6 * WB_MAX / 2; k += IRT_NT) {
+            const uint32_t s = k / (WB_MAX / 2), b = 2 * (k % (WB_MAX / 2));
+            uint32_t e0 = 0, e1 = 0;
+            if (b < wnb) e0 = run_end(sh, s, (b + 1) * wg < nx ? (b + 1) * wg - 1 : nx - 1) - sh.goff[s];
+            if (b + 1 < wnb) e1 = run_end(sh, s, (b + 2) * wg < nx ? (b + 2) * wg - 1 : nx - 1) - sh.goff[s];
+            st[32 + k] = e0 | (e1 << 16);
+        }
     if (tid == 0) {
-        // hand-over to the loop kernel (same launch index)
-        uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
         st[0] = sh.npend;
         for (int s = 0; s < 6; s++) st[1 + s] = sh.gcnt[s];
         st[7] = (uint32_t)sh.c[0]; st[8] = (uint32_t)sh.c[1]; st[9] = (uint32_t)sh.c[2];
         st[10] = T.h2; st[11] = n;
+        st[12] = win ? 1u : 0u;
+        if (win) {
+            st[13] = __float_as_uint(sh.ax[0]); st[14] = __float_as_uint(sh.ax[1]); st[15] = __float_as_uint(sh.ax[2]);
+            st[16] = __float_as_uint(sh.umin); st[17] = __float_as_uint(sh.cinv);
+            st[18] = nx; st[19] = wg; st[20] = wnb;
+        }
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
         atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tp2 - tp1);
+        // the worker's own enumeration: every pair of alive initial particles with a channel
+        const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
+        unsigned long long all = 0;
+        for (int r = 0; r < 9; r++) {
+            const unsigned long long nA = sh.gcnt[BA[r]], nB = sh.gcnt[BB[r]];
+            all += BA[r] == BB[r] ? nA * (nA - (nA ? 1ull : 0ull)) / 2 : nA * nB;
+        }
+        if (all) atomicAdd(&g[8], all);
     }
 }
 
@@ -1119,6 +1409,15 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         sh.npend = st[0]; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
         sh.n_ev = 0; sh.n_total = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
         for (int s = 0; s < 6; s++) { sh.llen[s] = st[1 + s]; sh.ldead[s] = 0; }
+        sh.win = st[12];
+        for (int s = 0; s < 6; s++) sh.n0[s] = sh.win ? st[1 + s] : 0u;
+        sh.wnb = 0; sh.wnx = 1; sh.wg = 1; sh.wumin = 0.f; sh.wcinv = 0.f;
+        sh.wax[0] = 1.f; sh.wax[1] = 0.f; sh.wax[2] = 0.f;
+        if (sh.win) {
+            sh.wax[0] = __uint_as_float(st[13]); sh.wax[1] = __uint_as_float(st[14]); sh.wax[2] = __uint_as_float(st[15]);
+            sh.wumin = __uint_as_float(st[16]); sh.wcinv = __uint_as_float(st[17]);
+            sh.wnx = st[18]; sh.wg = st[19]; sh.wnb = st[20];
+        }
         sh.cnt[0] = st[7]; sh.cnt[1] = st[8]; sh.cnt[2] = st[9]; sh.cnt[3] = 0; sh.cnt[4] = st[10];
         for (int r = 0; r < 9; r++) sh.rxc[r] = 0;
         // blocks of 64 k events, k the smallest that covers the event capacity with BM_MAX blocks
@@ -1133,6 +1432,11 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     }
     if (lane < 9) { sh.nprod[lane] = (int8_t)C.nprod[lane]; sh.prods[lane][0] = (int8_t)C.prods[lane][0]; sh.prods[lane][1] = (int8_t)C.prods[lane][1]; }
     if (lane < 6) { sh.Dd[lane] = C.D[lane]; sh.sqD[lane] = C.sqrtD[lane]; sh.Df[lane] = (float)C.D[lane]; }
+    if (st[12])
+        for (uint32_t k = lane; k < 6 * WB_MAX / 2; k += 64u) {
+            const uint32_t wv = st[32 + k], s6 = k / (WB_MAX / 2), b = 2 * (k % (WB_MAX / 2));
+            sh.wend[s6][b] = (uint16_t)(wv & 0xFFFFu); sh.wend[s6][b + 1] = (uint16_t)(wv >> 16);
+        }
     for (uint32_t i = lane; i < DNA_MAX_CKPT * 5; i += 64u) sh.snap[i] = 0u;
     for (uint32_t j = lane; j < BM_MAX; j += 64u) { sh.bm_t[j] = 1e300; sh.bm_ab[j] = ~0ull; }
     __syncthreads();
@@ -1158,12 +1462,12 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     for (uint32_t j = 0; j < sh.nblk; j++) block_min(A, T, sh, al, j);
     __syncthreads();
     const unsigned long long tp1 = wall_clock64();
-    unsigned long long tsc = 0, trc = 0, trp = 0, trd = 0, nit = 0, wrep = 0;
+    if (lane == 0) for (int k = 0; k < 7; k++) sh.acc[k] = 0;
     int cp = 0;
     // 2. event loop (irt-worker.js:566-645)
     for (;;) {
         const unsigned long long q0 = wall_clock64();
-        nit++;
+        if (lane == 0) sh.acc[4]++;
         // the next event: the least block minimum whose reactants are both alive; a stale
         // minimum is replaced by its block's rescan
         double t0;
@@ -1185,7 +1489,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             __syncthreads();
         }
         const unsigned long long q1 = wall_clock64();
-        tsc += q1 - q0;
+        if (lane == 0) sh.acc[0] += q1 - q0;
         if (!(t0 < 1e300)) break;
         if (lane == 0) {
             sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
@@ -1234,34 +1538,28 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                 sh.np = made;
                 sh.n_total = nt;
             }
-            // segments of the re-pairing: (product, partner species with a channel)
-            uint32_t u = 0, lc = 0;
-            int ns = 0;
+            // the partner lists of the re-pairing that are mostly reacted are compacted first
+            uint32_t lc = 0;
             for (int pp = 0; pp < sh.np; pp++) {
                 const uint32_t sp = sh.prod_sp[pp];
-                for (int sj2 = 0; sj2 < 6; sj2++) {
-                    if (sh.rmap[sp * 6 + sj2] < 0 || sh.llen[sj2] == 0) continue;
-                    sh.seg_u0[ns] = u; sh.seg_ps[ns] = ((uint32_t)pp << 4) | (uint32_t)sj2;
-                    u += sh.llen[sj2];
-                    ns++;
-                    if (2u * sh.ldead[sj2] > sh.llen[sj2] && sh.llen[sj2] >= 64u) lc |= 1u << sj2;
-                }
+                for (int sj2 = 0; sj2 < 6; sj2++)
+                    if (sh.rmap[sp * 6 + sj2] >= 0 && 2u * sh.ldead[sj2] > sh.llen[sj2] && sh.llen[sj2] >= 64u)
+                        lc |= 1u << sj2;
             }
-            sh.seg_u0[ns] = u; sh.nseg = ns; sh.lcomp = lc;
+            sh.lcomp = lc;
         }
         __syncthreads();
         const unsigned long long q2 = wall_clock64();
-        trc += q2 - q1;
-        if (sh.lcomp) {
-            compact_lists(A, T, sh, al);
-            if (lane == 0) {                   // segment starts after the compaction
-                uint32_t u = 0;
-                for (int g = 0; g < sh.nseg; g++) { sh.seg_u0[g] = u; u += sh.llen[sh.seg_ps[g] & 15u]; }
-                sh.seg_u0[sh.nseg] = u; sh.lcomp = 0;
+        if (lane == 0) sh.acc[1] += q2 - q1;
+        if (sh.np) {
+            if (sh.lcomp) {
+                compact_lists(A, T, sh, al);
+                if (lane == 0) sh.lcomp = 0;
             }
+            build_segments(sh, sh.win_t);
             __syncthreads();
+            pair_products(A, T, sh, al, tk, sh.win_t);
         }
-        if (sh.np) pair_products(A, T, sh, al, tk, sh.win_t, wrep);
         __syncthreads();
         const unsigned long long q3 = wall_clock64();
         if (sh.npend) {
@@ -1291,8 +1589,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             }
         }
         const unsigned long long q4 = wall_clock64();
-        trp += q3 - q2;
-        trd += q4 - q3;
+        if (lane == 0) { sh.acc[2] += q3 - q2; sh.acc[3] += q4 - q3; }
     }
     const unsigned long long tp2 = wall_clock64();
     if (A.final_ntot) {                       // the final state for dna_irt_oh_final
@@ -1300,7 +1597,6 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         if (al != al_global) for (uint32_t k = lane; k < nt; k += 64u) al_global[k] = al[k];
         if (lane == 0) A.final_ntot[T.idx] = nt;
     }
-    for (int o = 32; o > 0; o >>= 1) wrep += __shfl_xor(wrep, o, 64);
     if (lane == 0) {
         while (cp < A.nc) { for (int s = 0; s < 5; s++) sh.snap[cp * 5 + s] = (uint32_t)sh.cnt[s]; cp++; }
     }
@@ -1314,10 +1610,11 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         for (int r = 0; r < 9; r++) if (sh.rxc[r]) atomicAdd(&g[DNA_MAX_CKPT * 5 + r], sh.rxc[r]);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 9], sh.n_re);
         atomicAdd(&g[DNA_MAX_CKPT * 5 + 10], n_init_ev);
-        if (wrep) atomicAdd(&g[DNA_MAX_CKPT * 5 + 13], wrep);
+        if (sh.acc[5]) atomicAdd(&g[DNA_MAX_CKPT * 5 + 13], sh.acc[5]);
+        if (sh.acc[6]) atomicAdd(&g[DNA_MAX_CKPT * 5 + 23], sh.acc[6]);
         unsigned long long *pf = g + DNA_MAX_CKPT * 5 + 14;
-        atomicAdd(&pf[6], tp1 - tp0); atomicAdd(&pf[2], tsc); atomicAdd(&pf[3], trc); atomicAdd(&pf[4], trp);
-        atomicAdd(&pf[5], nit); atomicAdd(&pf[7], trd);
+        atomicAdd(&pf[6], tp1 - tp0); atomicAdd(&pf[2], sh.acc[0]); atomicAdd(&pf[3], sh.acc[1]);
+        atomicAdd(&pf[4], sh.acc[2]); atomicAdd(&pf[5], sh.acc[4]); atomicAdd(&pf[7], sh.acc[3]);
         (void)tp2;
     }
 }
@@ -1445,6 +1742,9 @@ static void build_const(IrtConst &C)
     double R_CUT = 1.45 + 2 * sqrt(8 * 9.46 * 1000);
     C.R_CUT2 = R_CUT * R_CUT;
     C.R_CUT_F = (float)R_CUT * 1.00001f;
+    // 0.1% and 1 nm above R_CUT: far above the FP32 error of a projection onto the axis
+    // (a few ulp of coordinates below 10^5 nm) and of the cell index
+    C.CELL_R = (float)(R_CUT * 1.001 + 1.0);
     static const double g[13] = {-4.06217193e-08, 3.06848412e-06, -9.93217814e-05, 1.80172797e-03,
         -2.01135480e-02, 1.42939448e-01, -6.48348714e-01, 1.85227848e+00, -3.36450378e+00,
         4.37785068e+00, -4.20557339e+00, 3.81679083e+00, -2.34069784e-01};
@@ -1723,39 +2023,67 @@ static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int b
     }
 }
 
-// Workgroup size: the option, or autotuned per track-size class (log2 of the mean
-// particles per track): the first launch of a class times each size on its first tracks
-// (results do not depend on the size; the trial tallies are cleared afterwards).
-static int irt_pick_block(dna_ctx *c, const IrtGroup &G, const IrtArgs &A, const IrtBufs &b, int &block)
+static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
+{
+    const size_t shmem = (size_t)A.lds_slots * 16;
+    if (A.tdeath) hipLaunchKernelGGL((k_irt_pairs<true>), dim3(grid), dim3(block), shmem, st, A);
+    else hipLaunchKernelGGL((k_irt_pairs<false>), dim3(grid), dim3(block), shmem, st, A);
+}
+
+// Workgroup size and initial-pair cells (both only shape k_irt_pairs): the options, or
+// autotuned per track-size class (log2 of the mean particles per track): the first launch
+// of a class times k_irt_pairs for every candidate pair on up to 3072 of its tracks, twice,
+// and keeps the fastest best-of-two, so one disturbed trial (another context's work on
+// the GPU, a cold cache) does not settle the class for the process.  Results do not depend
+// on either; the trial tallies are cleared afterwards.  Small batches (under 1024 tracks)
+// take a rule instead: 128 threads below 512 particles per track, else 256, and the cells
+// of R_CUT / 2 from 4096 particles per track (tracks longer than a few R_CUT).
+static int irt_pick_launch(dna_ctx *c, const IrtGroup &G, IrtArgs &A, const IrtBufs &b, int &block)
 {
     hipStream_t st = c->stream;
-    block = c->irt_block;
-    if (block) return DNA_OK;
     const uint32_t ntr = (uint32_t)G.tr.size();
+    const uint64_t mean = G.run / ntr;
+    auto cell_arg = [](int v) { return v < 0 ? 0 : v; };
+    int cell = c->irt_cell;
+    block = c->irt_block;
+    if (block && cell) { A.cell_m = cell_arg(cell); return DNA_OK; }
     int cls = 0;
-    for (uint64_t m = G.run / ntr; m > 1; m >>= 1) cls++;
+    for (uint64_t m = mean; m > 1; m >>= 1) cls++;
     auto it = c->irt_tuned.find(cls);
-    if (it != c->irt_tuned.end()) { block = it->second; return DNA_OK; }
-    if (ntr < 1024) { block = (G.run / ntr < 512) ? 128 : 256; return DNA_OK; }
+    if (it != c->irt_tuned.end()) {
+        block = it->second.first; A.cell_m = cell_arg(it->second.second); return DNA_OK;
+    }
+    if (ntr < 1024) {
+        if (!block) block = mean < 512 ? 128 : 256;
+        if (!cell) cell = mean >= 4096 ? 2 : -1;
+        A.cell_m = cell_arg(cell);
+        return DNA_OK;
+    }
     const uint32_t Kt = std::min<uint32_t>(ntr, 3072u);
-    // each size is timed twice and keeps its faster time, so one disturbed trial (another
-    // context's work on the GPU, a cold cache) does not settle the class for the process
-    const int cands[3] = {256, 128, 64};
-    float t[3] = {1e30f, 1e30f, 1e30f};
+    std::vector<int> bs, cs;
+    if (block) bs.push_back(block); else bs = {256, 128, 64};
+    if (cell) cs.push_back(cell); else cs = {-1, 1, 2, 4};
+    std::vector<float> t(bs.size() * cs.size(), 1e30f);
     for (int round = 0; round < 2; round++)
-        for (int k = 0; k < 3; k++) {
-            DNA_HIP(hipEventRecord(c->ev[0], st));
-            irt_launch(G, A, Kt, cands[k], st);
-            DNA_HIP(hipGetLastError());
-            DNA_HIP(hipEventRecord(c->ev[1], st));
-            DNA_HIP(hipEventSynchronize(c->ev[1]));
-            float ms = 0;
-            DNA_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-            t[k] = ms < t[k] ? ms : t[k];
-        }
+        for (size_t i = 0; i < bs.size(); i++)
+            for (size_t j = 0; j < cs.size(); j++) {
+                A.cell_m = cell_arg(cs[j]);
+                DNA_HIP(hipEventRecord(c->ev[0], st));
+                irt_launch_pairs(A, Kt, bs[i], st);
+                DNA_HIP(hipGetLastError());
+                DNA_HIP(hipEventRecord(c->ev[1], st));
+                DNA_HIP(hipEventSynchronize(c->ev[1]));
+                float ms = 0;
+                DNA_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                float &tk = t[i * cs.size() + j];
+                tk = ms < tk ? ms : tk;
+            }
     float best = 1e30f;
-    for (int k = 0; k < 3; k++) if (t[k] < best) { best = t[k]; block = cands[k]; }
-    c->irt_tuned[cls] = block;
+    for (size_t i = 0; i < bs.size(); i++)
+        for (size_t j = 0; j < cs.size(); j++)
+            if (t[i * cs.size() + j] < best) { best = t[i * cs.size() + j]; block = bs[i]; cell = cs[j]; }
+    c->irt_tuned[cls] = std::make_pair(block, cell);
+    A.cell_m = cell_arg(cell);
     DNA_HIP(hipMemsetAsync(b.tally, 0, b.tally_bytes + 8, st));
     return DNA_OK;
 }
@@ -1779,10 +2107,13 @@ static void irt_outputs(const IrtGroup &G, const unsigned long long *ht, const u
         o.n_pair_sampled = g2[DNA_MAX_CKPT * 5 + 12];
         o.n_repair = g2[DNA_MAX_CKPT * 5 + 13];
         for (int q = 0; q < 8; q++) o.prof[q] = g2[DNA_MAX_CKPT * 5 + 14 + q];
-        // static FP64 cost model (counted on the restated worker source): distance and
-        // cut-off test 9, sqrt + first sampling stage 6, Brownian re-pair step
+        o.n_pair_all = g2[DNA_MAX_CKPT * 5 + 22];
+        o.n_repair_scanned = g2[DNA_MAX_CKPT * 5 + 23];
+        // static FP64 cost model of the worker's algorithm (counted on the restated worker
+        // source), whatever the cells leave out: distance and cut-off test 9 per pair of the
+        // all-pairs scan, sqrt + first sampling stage 6, Brownian re-pair step
         // (3 x [log 20 + cos 25 + sqrt + 4] + sqrt(2 D dt) 3 + distance 9 + sampling 6) 167
-        o.algo_flops = 9.0 * (double)o.n_pair_cand + 6.0 * (double)o.n_pair_sampled + 167.0 * (double)o.n_repair;
+        o.algo_flops = 9.0 * (double)o.n_pair_all + 6.0 * (double)o.n_pair_sampled + 167.0 * (double)o.n_repair;
         o.t_ms = ms;
         o.t_kernel_ms = kms;
     }
@@ -1843,8 +2174,8 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
             A.tdeath = (double *)((char *)c->irt.scratch + ((part * 65 + 7) & ~(uint64_t)7));
         }
         int block = 256;
-        rc = irt_pick_block(c, G, A, b, block); if (rc) return rc;
-        c->irt_last_block = block;
+        rc = irt_pick_launch(c, G, A, b, block); if (rc) return rc;
+        c->irt_last_block = block; c->irt_last_cell = A.cell_m ? A.cell_m : -1;
         DNA_HIP(hipEventRecord(c->ev[7], st));
         irt_launch(G, A, ntr, block, st);
         DNA_HIP(hipEventRecord(c->ev[4], st));
@@ -1971,7 +2302,7 @@ struct dna_irt_plan {
     uint64_t *d_seed = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int block = 256, ev_mult = 8, attempt = 0;
+    int block = 256, cell = -1, ev_mult = 8, attempt = 0;
     size_t tally_bytes = 0;
 };
 
@@ -2004,7 +2335,8 @@ static int plan_build(dna_irt_plan *P)
     P->d_seed = (uint64_t *)(P->buf + s_bytes + e_bytes + b_bytes);
     rc = irt_prepare(c, G, b); if (rc) return rc;
     IrtArgs A = irt_args(G, P->rad, b, part, evs, P->tck.data(), P->nc, 0);
-    if (ntr) { rc = irt_pick_block(c, G, A, b, P->block); if (rc) return rc; }
+    if (ntr) { rc = irt_pick_launch(c, G, A, b, P->block); if (rc) return rc; }
+    P->cell = A.cell_m ? A.cell_m : -1;
     A.seed_ptr = P->d_seed;
     DNA_HIP(hipStreamSynchronize(st));
     // capture
@@ -2108,7 +2440,7 @@ extern "C" int dna_irt_plan_run(dna_irt_plan *P, uint64_t seed, dna_chem_out *ou
         for (uint32_t i = 0; i < P->G.n_inst; i++) out[i].n_tracks = 0;
         for (auto &t : P->G.tr) out[t.inst].n_tracks++;
         irt_outputs(P->G, P->h_tally, P->n_therm.data(), P->E.data(), P->nc, ms, ms, out);
-        c->irt_last_block = P->block;
+        c->irt_last_block = P->block; c->irt_last_cell = P->cell;
         return DNA_OK;
     }
     set_error("dna_irt_plan_run: event/particle capacity kept overflowing");

@@ -146,7 +146,15 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
             return DNA_ERR_ARG;
         }
         c->irt_block = value;
-        if (value == 0) c->irt_tuned.clear();
+        c->irt_tuned.clear();
+        return DNA_OK;
+    case DNA_OPT_IRT_CELL:
+        if (value != 0 && value != -1 && value != 1 && value != 2 && value != 4) {
+            set_error("dna_set_option: DNA_OPT_IRT_CELL must be 0 (autotune), -1 (no cells), 1, 2 or 4");
+            return DNA_ERR_ARG;
+        }
+        c->irt_cell = value;
+        c->irt_tuned.clear();
         return DNA_OK;
     case DNA_OPT_IRT_FINAL:
         if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_IRT_FINAL must be 0 or 1"); return DNA_ERR_ARG; }
@@ -171,6 +179,8 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     switch (option) {
     case DNA_OPT_IRT_BLOCK: *value = c->irt_block; return DNA_OK;
     case DNA_OPT_IRT_BLOCK_LAST: *value = c->irt_last_block; return DNA_OK;
+    case DNA_OPT_IRT_CELL: *value = c->irt_cell; return DNA_OK;
+    case DNA_OPT_IRT_CELL_LAST: *value = c->irt_last_cell; return DNA_OK;
     case DNA_OPT_IRT_FINAL: *value = c->irt_final; return DNA_OK;
     case DNA_OPT_PA_LANES: *value = c->pa_lanes; return DNA_OK;
     default:

@@ -6,6 +6,7 @@ DNA_MAX_CKPT = 32
 DNA_N_RXN = 9
 DNA_COPY_RESULTS, DNA_COPY_DOSE, DNA_COPY_RAD, DNA_COPY_ALL = 1, 2, 4, 7
 DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST, DNA_OPT_IRT_FINAL, DNA_OPT_PA_LANES = 1, 2, 3, 4
+DNA_OPT_IRT_CELL, DNA_OPT_IRT_CELL_LAST = 5, 6
 
 
 class DnaParams(C.Structure):
@@ -46,6 +47,7 @@ class DnaChemOut(C.Structure):
         ("t_ms", C.c_double), ("t_kernel_ms", C.c_double),
         ("n_pair_cand", C.c_uint64), ("n_pair_sampled", C.c_uint64), ("n_repair", C.c_uint64),
         ("algo_flops", C.c_double), ("prof", C.c_uint64 * 8),
+        ("n_pair_all", C.c_uint64), ("n_repair_scanned", C.c_uint64),
     ]
 
 
