@@ -262,11 +262,25 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * dna_irt_oh_final and dna_irt_particles, and records each particle's death time
  * (default 0). */
 #define DNA_OPT_IRT_FINAL       3
-/* DNA_OPT_PA_LANES: active lanes per 64-lane wavefront of the primary-transport kernel,
- * 0 = automatic (the widest power of two that still gives every CU one wavefront: 16 for
- * 4096 primaries on 256 CUs, 64 from 16384 primaries), or 1, 2, 4, ..., 64.  Results do
- * not depend on it. */
+/* Transport launch knobs, 0 = autotune (default): batches under 1024 primaries take the
+ * rule below; the first batch of each (log2 primaries, log2 mean energy) class times the
+ * candidates twice each and keeps the fastest (Phase A lanes on Phase A's time, then the
+ * radical chunk on both phases', then the work chunk on Phase B's).  Results do not depend
+ * on them.
+ * DNA_OPT_PA_LANES: active lanes per 64-lane wavefront of the primary-transport kernel, or
+ * 1, 2, 4, ..., 64 (rule and middle candidate: the widest power of two that still gives
+ * every CU one wavefront, 16 for 4096 primaries on 256 CUs; candidates half and double).
+ * DNA_OPT_RAD_CHUNK: radical slots a wavefront reserves with one atomic, or a power of two
+ * from 256 to 4096 (rule 256; candidates 256, 512, 1024).
+ * DNA_OPT_WORK_CHUNK: secondaries a Phase B wavefront claims with one atomic, or a power
+ * of two from 64 to 1024 (rule 64; candidates 64, 128, 256).
+ * DNA_OPT_*_LAST (get only): the values the last dna_run_transport used. */
 #define DNA_OPT_PA_LANES        4
+#define DNA_OPT_RAD_CHUNK       7
+#define DNA_OPT_WORK_CHUNK      8
+#define DNA_OPT_PA_LANES_LAST   9
+#define DNA_OPT_RAD_CHUNK_LAST  10
+#define DNA_OPT_WORK_CHUNK_LAST 11
 /* DNA_OPT_IRT_CELL: cells of the initial-pair search of the IRT (the spatial hash that
  * stands in for the worker's all-pairs scan, public/irt-worker.js:541-558; the cell
  * semantics of src/shaders/chemistry.wgsl:66-80 with one axis): 0 = autotune together with

@@ -62,6 +62,46 @@ def test_transport_lanes_per_wavefront(ctx, lanes):
         ctx.set_option(DNA_OPT_PA_LANES, 3)
 
 
+@pytest.mark.parametrize("rad_chunk,work_chunk", [(256, 64), (512, 128), (1024, 256), (4096, 1024)])
+def test_transport_chunks(ctx, rad_chunk, work_chunk):
+    """The radical-slot and secondary-claim chunk sizes (DNA_OPT_RAD_CHUNK, DNA_OPT_WORK_CHUNK)
+    change only which wavefront writes which raw slot: the canonical outputs are the
+    oracle's."""
+    from dna_amd.abi import DNA_OPT_RAD_CHUNK, DNA_OPT_WORK_CHUNK
+    p = default_params(3000.0, 200)
+    print(f"transport 3 keV x 200 with radical chunk {rad_chunk}, work chunk {work_chunk}", flush=True)
+    ctx.set_option(DNA_OPT_RAD_CHUNK, rad_chunk)
+    ctx.set_option(DNA_OPT_WORK_CHUNK, work_chunk)
+    try:
+        g = ctx.transport(p)
+    finally:
+        ctx.set_option(DNA_OPT_RAD_CHUNK, 0)
+        ctx.set_option(DNA_OPT_WORK_CHUNK, 0)
+    _cmp(g, O.transport(p), 200)
+    for opt, bad in ((DNA_OPT_RAD_CHUNK, 128), (DNA_OPT_RAD_CHUNK, 300), (DNA_OPT_WORK_CHUNK, 32)):
+        with pytest.raises(Exception, match="CHUNK"):
+            ctx.set_option(opt, bad)
+
+
+def test_transport_autotune(ctx):
+    """The first batch of a class (here 1024 x 2 keV) times the knob candidates and keeps
+    the fastest; the run's outputs are the oracle's, and a second run of the class reuses
+    the choice."""
+    from dna_amd.abi import (DNA_OPT_PA_LANES, DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK,
+                             DNA_OPT_RAD_CHUNK_LAST, DNA_OPT_WORK_CHUNK, DNA_OPT_WORK_CHUNK_LAST)
+    for o in (DNA_OPT_PA_LANES, DNA_OPT_RAD_CHUNK, DNA_OPT_WORK_CHUNK):
+        ctx.set_option(o, 0)
+    p = default_params(2000.0, 1024)
+    print("transport autotune on 1024 x 2 keV", flush=True)
+    g = ctx.transport(p)
+    knobs = tuple(ctx.get_option(o) for o in (DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK_LAST, DNA_OPT_WORK_CHUNK_LAST))
+    assert knobs[0] in (1, 2, 4, 8, 16, 32, 64) and knobs[1] in (256, 512, 1024) and knobs[2] in (64, 128, 256), knobs
+    _cmp(g, O.transport(p, n_threads=8), 1024)
+    ctx.transport(p)
+    assert tuple(ctx.get_option(o) for o in (DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK_LAST,
+                                              DNA_OPT_WORK_CHUNK_LAST)) == knobs
+
+
 def test_transport_caps(ctx):
     """max_sec / max_rad as caps with drop counts (the reference's MAX_SEC / MAX_RAD,
     dispatch.ts:124-126,190-192): the kept secondaries and radicals are the first ones in

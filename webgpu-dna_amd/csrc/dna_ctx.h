@@ -2,6 +2,7 @@
 #pragma once
 #include <atomic>
 #include <map>
+#include <array>
 #include "dna_internal.h"
 
 struct IrtBuffers {
@@ -76,7 +77,11 @@ struct dna_ctx {
     std::map<int, std::pair<int, int>> irt_tuned;   // log2(mean particles per track) -> (size, cells)
     // final IRT state (DNA_OPT_IRT_FINAL) and the alive-OH list made from it
     int irt_final = 0;
-    int pa_lanes = 0;                 // DNA_OPT_PA_LANES (0: chosen from the batch size)
+    // transport knobs (0 = autotune per (log2 primaries, log2 mean energy) class) and the
+    // values of the last run
+    int pa_lanes = 0, rad_chunk = 0, work_chunk = 0;
+    int last_pa_lanes = 0, last_rad_chunk = 0, last_work_chunk = 0;
+    std::map<std::pair<int, int>, std::array<uint32_t, 3>> tr_tuned;
     bool final_ready = false;
     uint32_t final_ntr = 0;
     uint64_t final_seed = 0;

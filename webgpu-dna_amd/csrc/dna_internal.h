@@ -62,6 +62,8 @@ struct TransportDev {
     // the secondaries whose canonical rank sec_base[pid] + k - 1 is below sec_keep
     const uint32_t *sec_base;
     uint32_t sec_keep;
+    uint32_t rad_chunk;         // radical slots a wavefront reserves at once (>= 192)
+    uint32_t work_chunk;        // secondaries a wavefront claims at once (>= 64)
 };
 // ctr layout
 enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, C_RAD = 7,
@@ -69,7 +71,7 @@ enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, 
        C_WRAP = 26 /* set when the radical slot counter would pass 2^32 */, C_NCTR = 32 };
 
 uint32_t phase_a_lanes(uint32_t n, int opt);
-size_t transport_slot_headroom(uint32_t n, uint32_t lanes);
+size_t transport_slot_headroom(uint32_t n, uint32_t lanes, uint32_t rad_chunk);
 hipError_t launch_phase_a(const TransportDev &d, uint32_t lanes, hipStream_t st);
 hipError_t launch_phase_b(const TransportDev &d, uint32_t sec_n, hipStream_t st);
 hipError_t launch_xs_probe(const dna_tables *T, const uint8_t *guide, const float *E, const float *r, uint32_t n,

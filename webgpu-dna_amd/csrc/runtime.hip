@@ -162,10 +162,27 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
         return DNA_OK;
     case DNA_OPT_PA_LANES:
         if (value < 0 || value > 64 || (value & (value - 1))) {
-            set_error("dna_set_option: DNA_OPT_PA_LANES must be 0 (automatic) or a power of two up to 64");
+            set_error("dna_set_option: DNA_OPT_PA_LANES must be 0 (autotune) or a power of two up to 64");
             return DNA_ERR_ARG;
         }
         c->pa_lanes = value;
+        c->tr_tuned.clear();
+        return DNA_OK;
+    case DNA_OPT_RAD_CHUNK:
+        if (value != 0 && (value < 256 || value > 4096 || (value & (value - 1)))) {
+            set_error("dna_set_option: DNA_OPT_RAD_CHUNK must be 0 (autotune) or a power of two from 256 to 4096");
+            return DNA_ERR_ARG;
+        }
+        c->rad_chunk = value;
+        c->tr_tuned.clear();
+        return DNA_OK;
+    case DNA_OPT_WORK_CHUNK:
+        if (value != 0 && (value < 64 || value > 1024 || (value & (value - 1)))) {
+            set_error("dna_set_option: DNA_OPT_WORK_CHUNK must be 0 (autotune) or a power of two from 64 to 1024");
+            return DNA_ERR_ARG;
+        }
+        c->work_chunk = value;
+        c->tr_tuned.clear();
         return DNA_OK;
     default:
         set_error("dna_set_option: unknown option");
@@ -183,6 +200,11 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     case DNA_OPT_IRT_CELL_LAST: *value = c->irt_last_cell; return DNA_OK;
     case DNA_OPT_IRT_FINAL: *value = c->irt_final; return DNA_OK;
     case DNA_OPT_PA_LANES: *value = c->pa_lanes; return DNA_OK;
+    case DNA_OPT_RAD_CHUNK: *value = c->rad_chunk; return DNA_OK;
+    case DNA_OPT_WORK_CHUNK: *value = c->work_chunk; return DNA_OK;
+    case DNA_OPT_PA_LANES_LAST: *value = c->last_pa_lanes; return DNA_OK;
+    case DNA_OPT_RAD_CHUNK_LAST: *value = c->last_rad_chunk; return DNA_OK;
+    case DNA_OPT_WORK_CHUNK_LAST: *value = c->last_work_chunk; return DNA_OK;
     default:
         set_error("dna_get_option: unknown option");
         return DNA_ERR_ARG;
@@ -250,6 +272,103 @@ static int ensure_transport(dna_ctx *c, uint32_t n, size_t rad_cap, size_t sec_c
     return DNA_OK;
 }
 
+// One pass of both transport phases with the given launch knobs, for timing only (the
+// outputs are not canonicalized; the real run repeats the work): phase times in ms.
+static int transport_trial(dna_ctx *c, TransportDev d, uint32_t lanes, float &ta, float &tb)
+{
+    hipStream_t st = c->stream;
+    const size_t v3 = (size_t)d.p.vc * d.p.vc * d.p.vc;
+    DNA_HIP(hipMemsetAsync(c->d_ctr, 0, C_NCTR * sizeof(uint32_t), st));
+    DNA_HIP(hipMemsetAsync(c->d_dose, 0, v3 * sizeof(uint32_t), st));
+    DNA_HIP(hipMemsetAsync(c->d_rad_key, 0xFF, (size_t)d.rad_cap * sizeof(uint64_t), st));
+    DNA_HIP(hipEventRecord(c->ev[0], st));
+    DNA_HIP(launch_phase_a(d, lanes, st));
+    DNA_HIP(hipEventRecord(c->ev[1], st));
+    DNA_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, C_NCTR * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipStreamSynchronize(st));
+    const uint32_t sec_n = c->h_ctr[C_SEC] < d.sec_cap ? c->h_ctr[C_SEC] : d.sec_cap;
+    DNA_HIP(hipEventRecord(c->ev[2], st));
+    DNA_HIP(launch_phase_b(d, sec_n, st));
+    DNA_HIP(hipEventRecord(c->ev[3], st));
+    DNA_HIP(hipStreamSynchronize(st));
+    DNA_HIP(hipEventElapsedTime(&ta, c->ev[0], c->ev[1]));
+    DNA_HIP(hipEventElapsedTime(&tb, c->ev[2], c->ev[3]));
+    return DNA_OK;
+}
+
+// Transport launch knobs: the options, or autotuned per batch class (log2 of the primaries,
+// log2 of the mean energy in eV).  Batches under 1024 primaries take the rule (Phase A lanes
+// from phase_a_lanes, 256-slot radical chunks, 64-item work chunks).  Otherwise the first
+// batch of a class is run with candidates, each timed twice and kept at its best: the
+// Phase A lanes (half, the rule's and double) on Phase A's time, then the radical chunk
+// (256, 512, 1024) on both phases', then the work chunk (64, 128, 256) on Phase B's.
+// Results do not depend on any of them.
+struct TrKnobs { uint32_t lanes, rad_chunk, work_chunk; };
+static int transport_knobs(dna_ctx *c, const dna_params &p, double sumE, const float *energies, TrKnobs &k)
+{
+    const uint32_t rule = phase_a_lanes(p.n, 0);
+    k.lanes = c->pa_lanes ? (uint32_t)c->pa_lanes : rule;
+    k.rad_chunk = c->rad_chunk ? (uint32_t)c->rad_chunk : 256u;
+    k.work_chunk = c->work_chunk ? (uint32_t)c->work_chunk : 64u;
+    if (c->pa_lanes && c->rad_chunk && c->work_chunk) return DNA_OK;
+    if (p.n < 1024) return DNA_OK;
+    int cn = 0, ce = 0;
+    for (uint32_t m = p.n; m > 1; m >>= 1) cn++;
+    for (double e = sumE / p.n; e >= 2.0; e *= 0.5) ce++;
+    const std::pair<int, int> key(cn, ce);
+    auto it = c->tr_tuned.find(key);
+    if (it != c->tr_tuned.end()) {
+        if (!c->pa_lanes) k.lanes = it->second[0];
+        if (!c->rad_chunk) k.rad_chunk = it->second[1];
+        if (!c->work_chunk) k.work_chunk = it->second[2];
+        return DNA_OK;
+    }
+    std::vector<uint32_t> L, R, W;
+    if (c->pa_lanes) L = {k.lanes};
+    else { if (rule > 1) L.push_back(rule / 2); L.push_back(rule); if (rule < 64) L.push_back(rule * 2); }
+    if (c->rad_chunk) R = {k.rad_chunk}; else R = {256u, 512u, 1024u};
+    if (c->work_chunk) W = {k.work_chunk}; else W = {64u, 128u, 256u};
+    // buffers for the most headroom any candidate needs (fewest lanes, largest chunk)
+    const size_t headroom = transport_slot_headroom(p.n, L.front(), R.back());
+    const size_t rad_cap = (size_t)(0.2 * sumE) + 65536 + headroom;
+    const size_t sec_cap = (size_t)(0.025 * sumE) + 4096;
+    if (rad_cap > (size_t)0xFFFFFFF0u || sec_cap > (size_t)0xFFFFFFF0u) return DNA_OK;   // the run reports it
+    int rc = ensure_transport(c, p.n, rad_cap, sec_cap, p.vc);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    if (energies) DNA_HIP(hipMemcpyAsync(c->d_energies, energies, p.n * sizeof(float), hipMemcpyHostToDevice, st));
+    TransportDev d;
+    d.tab = c->d_tab; d.guide = c->d_guide; d.p = p; d.energies = energies ? c->d_energies : nullptr;
+    d.results = c->d_results; d.dose = c->d_dose; d.rad_raw = c->d_rad_raw; d.rad_key = c->d_rad_key;
+    d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
+    d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
+    d.sec_base = nullptr; d.sec_keep = 0xFFFFFFFFu;
+    d.rad_chunk = k.rad_chunk; d.work_chunk = k.work_chunk;
+    auto best_of = [&](const std::vector<uint32_t> &cand, int which, uint32_t &out) -> int {
+        float best = 1e30f;
+        std::vector<float> t(cand.size(), 1e30f);
+        for (int round = 0; round < 2; round++)
+            for (size_t i = 0; i < cand.size(); i++) {
+                uint32_t lanes = k.lanes;
+                if (which == 0) lanes = cand[i];
+                else if (which == 1) d.rad_chunk = cand[i];
+                else d.work_chunk = cand[i];
+                float ta = 0, tb = 0;
+                int r = transport_trial(c, d, lanes, ta, tb); if (r) return r;
+                const float v = which == 0 ? ta : which == 1 ? ta + tb : tb;
+                t[i] = v < t[i] ? v : t[i];
+            }
+        for (size_t i = 0; i < cand.size(); i++) if (t[i] < best) { best = t[i]; out = cand[i]; }
+        return DNA_OK;
+    };
+    if (L.size() > 1) { rc = best_of(L, 0, k.lanes); if (rc) return rc; }
+    if (R.size() > 1) { rc = best_of(R, 1, k.rad_chunk); if (rc) return rc; }
+    d.rad_chunk = k.rad_chunk;
+    if (W.size() > 1) { rc = best_of(W, 2, k.work_chunk); if (rc) return rc; }
+    c->tr_tuned[key] = {k.lanes, k.rad_chunk, k.work_chunk};
+    return DNA_OK;
+}
+
 extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float *energies,
                                  uint32_t copy_flags, dna_transport_out *out)
 {
@@ -268,8 +387,12 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
     // caller's max_rad / max_sec are caps on what is kept, applied in canonical order after
     // the run (the reference's MAX_RAD / MAX_SEC with *_dropped counts, dispatch.ts:124-126,
     // 190-192), not buffer sizes.
-    const uint32_t pa_lanes = phase_a_lanes(p.n, c->pa_lanes);
-    const size_t headroom = transport_slot_headroom(p.n, pa_lanes);   // chunk tails left unused
+    TrKnobs kn;
+    int krc = transport_knobs(c, p, sumE, energies, kn);
+    if (krc) return krc;
+    c->last_pa_lanes = (int)kn.lanes; c->last_rad_chunk = (int)kn.rad_chunk; c->last_work_chunk = (int)kn.work_chunk;
+    const uint32_t pa_lanes = kn.lanes;
+    const size_t headroom = transport_slot_headroom(p.n, pa_lanes, kn.rad_chunk);   // chunk tails left unused
     size_t rad_cap = (size_t)(0.2 * sumE) + 65536 + headroom;
     size_t sec_cap = (size_t)(0.025 * sumE) + 4096;
     // slot counters are 32-bit: refuse a batch that may need 2^32 or more slots
@@ -297,6 +420,7 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         d.rad_cap = (uint32_t)rad_cap; d.sec = c->d_sec; d.sec_cap = (uint32_t)sec_cap;
         d.prim_nsec = c->d_prim_nsec; d.prim_nrad = c->d_prim_nrad; d.sec_nrad = c->d_sec_nrad; d.ctr = c->d_ctr;
         d.sec_base = nullptr; d.sec_keep = 0xFFFFFFFFu;
+        d.rad_chunk = kn.rad_chunk; d.work_chunk = kn.work_chunk;
         DNA_HIP(hipEventRecord(c->ev[0], st));
         DNA_HIP(launch_phase_a(d, pa_lanes, st));
         DNA_HIP(hipEventRecord(c->ev[1], st));

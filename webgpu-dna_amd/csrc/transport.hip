@@ -242,24 +242,24 @@ struct Sink {
 };
 
 // Per-wavefront allocation state in LDS (4 wavefronts per workgroup at most): radical
-// slots {base, fill} and work items {next, end}.  A wavefront reserves radical slots in
-// chunks of RAD_CHUNK with one atomic and hands them out to its lanes, so emitting lanes
-// almost never wait on a device-wide atomic; slots left unused in a chunk keep the
-// invalid key written before the launch and are skipped by the canonical scatter.
-#define RAD_CHUNK 256u
-#define WORK_CHUNK 64u
+// slots {base, fill, chunk} and work items {next, end, chunk}.  A wavefront reserves radical
+// slots in chunks of d.rad_chunk (>= 192, the most one call emits) with one atomic and hands
+// them out to its lanes, so emitting lanes almost never wait on a device-wide atomic; slots
+// left unused in a chunk keep the invalid key written before the launch and are skipped by
+// the canonical scatter.  Work items come in chunks of d.work_chunk (>= 64).  Both sizes
+// are launch options (autotuned by the host); results do not depend on them.
 // Different lanes lead successive calls, so every access goes through a volatile view:
 // no value may be kept in a register from one call to the next.
-__shared__ uint32_t s_wrad_[8], s_wwork_[8];
+__shared__ uint32_t s_wrad_[12], s_wwork_[12];
 #define s_wrad ((volatile uint32_t *)s_wrad_)
 #define s_wwork ((volatile uint32_t *)s_wwork_)
 
-__device__ __forceinline__ void wave_state_init()
+__device__ __forceinline__ void wave_state_init(uint32_t rad_chunk, uint32_t work_chunk)
 {
     if ((threadIdx.x & 63u) == 0) {
         const uint32_t w = threadIdx.x >> 6;
-        s_wrad[2 * w] = 0u; s_wrad[2 * w + 1] = RAD_CHUNK;      // empty: the first use reserves
-        s_wwork[2 * w] = 0u; s_wwork[2 * w + 1] = 0u;
+        s_wrad[3 * w] = 0u; s_wrad[3 * w + 1] = rad_chunk; s_wrad[3 * w + 2] = rad_chunk;   // empty: the first use reserves
+        s_wwork[3 * w] = 0u; s_wwork[3 * w + 1] = 0u; s_wwork[3 * w + 2] = work_chunk;
     }
 }
 
@@ -278,16 +278,17 @@ __device__ __forceinline__ void emit_n(Sink &k, int n, const float4 *e)
     if ((int)lane == leader) {
         const uint32_t w = threadIdx.x >> 6;
         const uint32_t tot = (uint32_t)(__popcll(m1) + 2 * __popcll(m2) + 3 * __popcll(m3));   // <= 192
-        uint32_t f = s_wrad[2 * w + 1];
-        if (f + tot > RAD_CHUNK) {
-            const uint32_t b = atomicAdd(k.rad_ctr, RAD_CHUNK);
+        const uint32_t ch = s_wrad[3 * w + 2];
+        uint32_t f = s_wrad[3 * w + 1];
+        if (f + tot > ch) {
+            const uint32_t b = atomicAdd(k.rad_ctr, ch);
             // the next claim would wrap the 32-bit counter: flag it for the host (which reports
             // DNA_ERR_CAPACITY); slots at or beyond rad_cap are never written
-            if (b >= 0xFFFFFFFFu - RAD_CHUNK) atomicOr(k.rad_ctr + (C_WRAP - C_RAD), 1u);
-            s_wrad[2 * w] = b; f = 0u;
+            if (b >= 0xFFFFFFFFu - ch) atomicOr(k.rad_ctr + (C_WRAP - C_RAD), 1u);
+            s_wrad[3 * w] = b; f = 0u;
         }
-        base = s_wrad[2 * w] + f;
-        s_wrad[2 * w + 1] = f + tot;
+        base = s_wrad[3 * w] + f;
+        s_wrad[3 * w + 1] = f + tot;
     }
     base = __shfl(base, leader, 64) + rank;
     for (int i = 0; i < n; i++) {
@@ -297,8 +298,8 @@ __device__ __forceinline__ void emit_n(Sink &k, int n, const float4 *e)
     k.seq += (uint32_t)n;
 }
 
-// Next work item for each calling lane from the wavefront's chunk of WORK_CHUNK items
-// (one atomic per chunk).  Items are handed out in increasing order, so a lane that
+// Next work item for each calling lane from the wavefront's chunk of work items (one
+// atomic per chunk).  Items are handed out in increasing order, so a lane that
 // receives an item past the end knows that no earlier item of its wavefront is left.
 __device__ __forceinline__ uint32_t wave_claim_chunked(uint32_t *ctr)
 {
@@ -309,13 +310,13 @@ __device__ __forceinline__ uint32_t wave_claim_chunked(uint32_t *ctr)
     uint32_t old = 0, avail = 0, fresh = 0;
     if ((int)lane == leader) {
         const uint32_t w = threadIdx.x >> 6, cnt = (uint32_t)__popcll(m);
-        const uint32_t nx = s_wwork[2 * w], end = s_wwork[2 * w + 1];
+        const uint32_t nx = s_wwork[3 * w], end = s_wwork[3 * w + 1], ch = s_wwork[3 * w + 2];
         old = nx; avail = end - nx;
-        if (avail >= cnt) { s_wwork[2 * w] = nx + cnt; }
+        if (avail >= cnt) { s_wwork[3 * w] = nx + cnt; }
         else {
-            fresh = atomicAdd(ctr, WORK_CHUNK);
-            s_wwork[2 * w] = fresh + (cnt - avail);
-            s_wwork[2 * w + 1] = fresh + WORK_CHUNK;
+            fresh = atomicAdd(ctr, ch);
+            s_wwork[3 * w] = fresh + (cnt - avail);
+            s_wwork[3 * w + 1] = fresh + ch;
         }
     }
     old = __shfl(old, leader, 64); avail = __shfl(avail, leader, 64); fresh = __shfl(fresh, leader, 64);
@@ -696,7 +697,7 @@ __global__ void __launch_bounds__(256) k_phase_a(TransportDev d, uint32_t lanes)
 {
     __shared__ SmallTab S;
     stage_tables(S, d.tab, d.guide);
-    wave_state_init();
+    wave_state_init(d.rad_chunk, d.work_chunk);
     const dna_params p = d.p;
     Sink k;
     k.rad = d.rad_raw; k.key = d.rad_key; k.rad_ctr = &d.ctr[C_RAD]; k.rad_cap = d.rad_cap;
@@ -841,7 +842,7 @@ __global__ void __launch_bounds__(256) k_phase_b(TransportDev d, uint32_t sec_n)
 {
     __shared__ SmallTab S;
     stage_tables(S, d.tab, d.guide);
-    wave_state_init();
+    wave_state_init(d.rad_chunk, d.work_chunk);
     const dna_params p = d.p;
     Sink k;
     k.rad = d.rad_raw; k.key = d.rad_key; k.rad_ctr = &d.ctr[C_RAD]; k.rad_cap = d.rad_cap;
@@ -902,12 +903,12 @@ static uint32_t phase_a_blocks(uint32_t n, uint32_t lanes)
     return (m + lanes - 1u) / lanes;
 }
 
-size_t transport_slot_headroom(uint32_t n, uint32_t lanes)
+size_t transport_slot_headroom(uint32_t n, uint32_t lanes, uint32_t rad_chunk)
 {
-    // every wavefront of either phase may leave up to RAD_CHUNK reserved slots unused
+    // every wavefront of either phase may leave up to rad_chunk reserved slots unused
     const size_t waves_a = phase_a_blocks(n, lanes);
     const size_t waves_b = (size_t)num_cu() * 4u * 4u;
-    return (waves_a + waves_b) * RAD_CHUNK;
+    return (waves_a + waves_b) * rad_chunk;
 }
 
 hipError_t launch_phase_a(const TransportDev &d, uint32_t lanes, hipStream_t st)

@@ -7,6 +7,8 @@ DNA_N_RXN = 9
 DNA_COPY_RESULTS, DNA_COPY_DOSE, DNA_COPY_RAD, DNA_COPY_ALL = 1, 2, 4, 7
 DNA_OPT_IRT_BLOCK, DNA_OPT_IRT_BLOCK_LAST, DNA_OPT_IRT_FINAL, DNA_OPT_PA_LANES = 1, 2, 3, 4
 DNA_OPT_IRT_CELL, DNA_OPT_IRT_CELL_LAST = 5, 6
+DNA_OPT_RAD_CHUNK, DNA_OPT_WORK_CHUNK = 7, 8
+DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK_LAST, DNA_OPT_WORK_CHUNK_LAST = 9, 10, 11
 
 
 class DnaParams(C.Structure):
