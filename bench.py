@@ -97,22 +97,28 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
     # held-out 5%: 10,000 tracks from another seed stream
     E, t = batch(10_000, 0x5EED0005)
     o, tt = ctx.irt_tracks(None, t["rad_n"], E.size, E.size, 1.0, tc, seed=7)
-    G_ref = S.aggregate_G(S.per_track_G(tt, E), E)
+    G_tr = S.per_track_G(tt, E)
+    G_ref = S.aggregate_G(G_tr, E)
     r = S.surrogate_chemistry(ctx, s, t["rad_n"], E, sigma_max, seed=7)
     err = S.g_error(r["G"], G_ref)
+    # per track: error of the network's mean against the track's own IRT G and the share of
+    # it within 1 and 2 predicted sigma (every track, fallback or not)
     out["held_out"] = {"tracks": int(E.size), "g_err_max": round(err["max_rel"], 4),
                        "g_err_mean": round(err["mean_rel"], 4), "outputs_compared": err["n_outputs"],
-                       "fallback": r["n_fallback"]}
+                       "fallback": r["n_fallback"],
+                       "per_track": S.sigma_calibration(r["mu"], r["sigma"], G_tr)}
     log(f"surrogate: held-out error {err}, fallback {r['n_fallback']}")
     # inference batch of 65,536 tracks: device time of plain IRT vs surrogate + fallback IRT
     E, t = batch(65_536, 0x5EED0065)
-    irt_ms, sur_ms, pred_ms, fb_ms, nfb = [], [], [], [], 0
+    irt_ms, sur_ms, pred_ms, fb_ms, net_ms, feat_ms, nfb = [], [], [], [], [], [], 0
     for k in range(max(2, min(steps, 5))):
         o, tt = ctx.irt_tracks(None, t["rad_n"], E.size, E.size, 1.0, tc, seed=11)
         r = S.surrogate_chemistry(ctx, s, t["rad_n"], E, sigma_max, seed=11)
         irt_ms.append(o.t_ms)
         sur_ms.append(r["t_predict_ms"] + r["t_irt_ms"])
         pred_ms.append(r["t_predict_ms"])
+        net_ms.append(r["t_network_ms"])
+        feat_ms.append(r["t_features_ms"])
         fb_ms.append(r["t_irt_ms"])
         nfb = r["n_fallback"]
     G_ref = S.aggregate_G(S.per_track_G(tt, E), E)
@@ -120,9 +126,13 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
     irt_m, sur_m = float(np.median(irt_ms)), float(np.median(sur_ms))
     no_pad = (70 + 31) // 32 * 32
     flops = 2.0 * E.size * (no_pad * 256 + 2 * 256 * 256 + 256 * 160)
+    nm = float(np.median(net_ms))
     out["inference"] = {"tracks": int(E.size), "irt_ms": round(irt_m, 3),
                         "surrogate_ms": round(sur_m, 3),
                         "predict_ms": round(float(np.median(pred_ms)), 3),
+                        "features_ms": round(float(np.median(feat_ms)), 3),
+                        "network_ms": round(nm, 4),
+                        "network_tflops": round(flops / (nm * 1e-3) / 1e12, 1) if nm > 0 else None,
                         "fallback_irt_ms": round(float(np.median(fb_ms)), 3), "fallback": int(nfb),
                         "speedup": round(irt_m / sur_m, 2), "g_err_max": round(err["max_rel"], 4),
                         "g_err_mean": round(err["mean_rel"], 4),

@@ -281,6 +281,11 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
 #define DNA_OPT_PA_LANES_LAST   9
 #define DNA_OPT_RAD_CHUNK_LAST  10
 #define DNA_OPT_WORK_CHUNK_LAST 11
+/* DNA_OPT_MLP_FUSED: 1 (default) = the surrogate's forward pass (dna_surrogate_forward,
+ * _grad, _train, _predict) in one fused kernel launch when its shapes allow it (padded
+ * input, hidden width and 2 n_out multiples of 16 and at most 256); 0 = one GEMM launch per
+ * layer.  Both give the same values. */
+#define DNA_OPT_MLP_FUSED       12
 /* DNA_OPT_IRT_CELL: cells of the initial-pair search of the IRT (the spatial hash that
  * stands in for the worker's all-pairs scan, public/irt-worker.js:541-558; the cell
  * semantics of src/shaders/chemistry.wgsl:66-80 with one axis): 0 = autotune together with

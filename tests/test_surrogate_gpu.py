@@ -76,6 +76,28 @@ def test_gradient_matches_fp64_reference(net):
     assert np.all(Wg[0][70:, :] == 0.0)
 
 
+def test_fused_forward_equals_per_layer_path(ctx, net):
+    """The fused forward kernel (DNA_OPT_MLP_FUSED = 1) runs k_gemm's MFMA sequence per
+    output tile: raw outputs and gradients equal the one-GEMM-per-layer path bit for bit,
+    on a batch that is not a multiple of the 64-row tile."""
+    from dna_amd.abi import DNA_OPT_MLP_FUSED
+    rng = np.random.default_rng(15)
+    X = rng.normal(0, 1, (1000, 70)).astype(np.float32)
+    T = rng.normal(0, 1, (1000, 80)).astype(np.float32)
+    out = {}
+    for fused in (0, 1):
+        ctx.set_option(DNA_OPT_MLP_FUSED, fused)
+        try:
+            print(f"surrogate forward and gradient, fused={fused}", flush=True)
+            out[fused] = (net.forward(X),) + net.grad(X, T)
+        finally:
+            ctx.set_option(DNA_OPT_MLP_FUSED, 1)
+    assert np.array_equal(out[0][0], out[1][0])
+    # the loss is a sum of FP64 atomics (any order): equal to rounding
+    assert abs(out[0][1] - out[1][1]) <= 1e-12 * abs(out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_training_fits_smooth_function(ctx):
     rng = np.random.default_rng(7)
     N = 8192
@@ -165,6 +187,25 @@ def test_features_match_numpy(net, mixed):
     assert np.array_equal(F, F2)
 
 
+def test_fused_predict_equals_per_layer_path(ctx, net, mixed):
+    """Surrogate chemistry's prediction (mean, sigma, fallback flags and count) from the
+    one-launch fused kernel equals the per-layer path with k_predict."""
+    from dna_amd.abi import DNA_OPT_MLP_FUSED
+    E, t = mixed
+    out = {}
+    for fused in (0, 1):
+        ctx.set_option(DNA_OPT_MLP_FUSED, fused)
+        try:
+            print(f"surrogate predict, fused={fused}", flush=True)
+            mu, sg, fb, nf, _ = net.predict(E, 0.05)
+            out[fused] = (mu, sg, fb, nf)
+        finally:
+            ctx.set_option(DNA_OPT_MLP_FUSED, 1)
+    for a, b in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(a, b)
+    assert out[0][3] == out[1][3] == int(out[1][2].sum())
+
+
 def test_surrogate_replaces_irt_on_held_out_tracks(ctx):
     """Train on 48k generated tracks; on a held-out batch of 4096 tracks the surrogate
     chemistry (IRT fallback for uncertain tracks) matches plain IRT's aggregate G(t)
@@ -196,5 +237,10 @@ def test_surrogate_replaces_irt_on_held_out_tracks(ctx):
         rmse_net = np.sqrt(((res["mu"] - G_irt_tr) ** 2).mean())
         rmse_const = np.sqrt(((d["G"].mean(0) - G_irt_tr) ** 2).mean())
         assert rmse_net < 0.8 * rmse_const, (rmse_net, rmse_const)
+        # sigma calibration per track and output (outputs with mean G >= 0.05): the share of
+        # |mu - G_IRT| within 1 and 2 sigma, 68.3% and 95.4% for a calibrated Gaussian
+        cal = S.sigma_calibration(res["mu"], res["sigma"], G_irt_tr)
+        print(f"surrogate: sigma calibration {cal}", flush=True)
+        assert 0.55 <= cal["within_1sigma"] <= 0.85 and 0.85 <= cal["within_2sigma"] <= 0.995, cal
     finally:
         s.close()

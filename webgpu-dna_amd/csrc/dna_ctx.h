@@ -79,6 +79,7 @@ struct dna_ctx {
     int irt_final = 0;
     // transport knobs (0 = autotune per (log2 primaries, log2 mean energy) class) and the
     // values of the last run
+    int mlp_fused = 1;                 // DNA_OPT_MLP_FUSED: surrogate forward in one launch
     int pa_lanes = 0, rad_chunk = 0, work_chunk = 0;
     int last_pa_lanes = 0, last_rad_chunk = 0, last_work_chunk = 0;
     std::map<std::pair<int, int>, std::array<uint32_t, 3>> tr_tuned;

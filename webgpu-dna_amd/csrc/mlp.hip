@@ -5,18 +5,25 @@
 // of its initial radical distribution; tracks whose predicted sigma is too large go to IRT.
 //
 // Kernels:
+//   k_mlp_fused   the whole forward pass for 64 rows per workgroup: input normalisation,
+//                 every layer from an LDS activation tile and L2-resident transposed BF16
+//                 weights, and the prediction epilogue (one launch for any batch)
 //   k_gemm        C[MxN] = epilogue(A[MxK] B[KxN]); A and B are BF16 with arbitrary strides,
-//                 so the backward products dX = dY W^T and dW = X^T dY reuse the kernel.
-//                 64x64 block tile, K chunks of 32 staged in LDS as [row][k] (16-byte
-//                 vector loads whenever a dimension is contiguous), so each MFMA operand
-//                 fragment is one 16-byte LDS read; 4 wavefronts, each a 32x32 sub-tile of
-//                 2x2 MFMA 16x16 tiles.  Epilogues: bias, ReLU, ReLU' mask, FP32 and/or
-//                 BF16 output.
-//   k_nll         Gaussian NLL 0.5 (s + (t - mu)^2 e^-s) per output and its gradient
-//   k_colsum      bias gradients
-//   k_adam        Adam on FP32 master weights, refreshing the BF16 copies
+//                 so the backward products dX = dY W^T and dW = X^T dY use the kernel (and
+//                 the forward when the fused kernel is switched off).  64x64 block tile, K
+//                 chunks of 32 staged in LDS as [row][k] (16-byte vector loads whenever a
+//                 dimension is contiguous), so each MFMA operand fragment is one 16-byte LDS
+//                 read; 4 wavefronts, each a 32x32 sub-tile of 2x2 MFMA 16x16 tiles.
+//                 Epilogues: bias, ReLU, ReLU' mask, FP32 and/or BF16 output, and the column
+//                 sums of B (the bias gradient of dW = X^T dY).
+//   k_nll         Gaussian NLL 0.5 (s + (t - mu)^2 e^-s) per output and its gradient, with
+//                 the mini-batch gather and normalisation of the targets
+//   k_adam        Adam on FP32 master weights, refreshing both BF16 copies
 //   k_features    per-track input features from a radical list
-//   k_predict     de-normalised mean / sigma and the per-track IRT fallback flag
+//   k_predict     de-normalised mean / sigma and the per-track IRT fallback flag (the
+//                 per-layer path)
+// A training step is four launches plus the backward GEMMs (one per layer for dW and the
+// bias gradient, one per hidden layer for dX): fused forward, NLL, backward, Adam.
 #include "dna_internal.h"
 #include "dna_ctx.h"
 #include <vector>
@@ -44,6 +51,8 @@ struct GemmArgs {
     bf16 *Cb; long sbm;               // optional BF16 output
     const float *bias;                // [N] or null
     const bf16 *mask; long smm;       // ReLU' mask source (post-activation of the forward)
+    float *colsum;                    // optional: column sums of B over K (the bias gradient of a
+                                      // dW = X^T dY product), written by the blocks of row 0
     int M, N, K;
     int act;
     int amode, bmode;
@@ -89,10 +98,14 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
     // MFMA 16x16x32 operands: lane l supplies row l%16 of A (column l%16 of B) at the
     // k-slice (l/16)*8 .. +7; A and B use the same slice, so the sum runs over all 32 k
     const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const bool csum = g.colsum && blockIdx.y == 0;
+    float cs = 0.f;
     for (int k0 = 0; k0 < g.K; k0 += GK) {
         stage_tile(As, g.A, g.sam, g.sak, g.M, g.K, m0, k0, g.amode, tid);
         stage_tile(Bs, g.B, g.sbn, g.sbk, g.N, g.K, n0, k0, g.bmode, tid);
         __syncthreads();
+        if (csum && tid < GT)
+            for (int kk = 0; kk < GK; kk++) cs += (float)Bs[tid][kk];
         bf16x8 a[2], b[2];
         for (int i = 0; i < 2; i++) a[i] = *(const bf16x8 *)&As[wm + i * 16 + fr][fk];
         for (int j = 0; j < 2; j++) b[j] = *(const bf16x8 *)&Bs[wn + j * 16 + fr][fk];
@@ -101,6 +114,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         __syncthreads();
     }
+    if (csum && tid < GT && n0 + tid < g.N) g.colsum[n0 + tid] = cs;
     // accumulator: lane l holds column l%16, rows (l/16)*4 + 0..3 of each 16x16 tile
     const int oc = lane & 15, orow = (lane >> 4) * 4;
     for (int i = 0; i < 2; i++)
@@ -119,17 +133,21 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
 
 #define S_CLAMP 12.0f   // log-variance range used by the loss (gradient zero outside)
 
-// Gaussian NLL over a batch: Y = [mu (no) | s (no)] per row, T normalised targets.
-__global__ void k_nll(const float *Y, const float *T, int B, int no, float inv_count, bf16 *dY, double *loss)
+// Gaussian NLL over a batch: Y = [mu (no) | s (no)] per row; the target of row b is
+// (T[src(b)][o] - mean[o]) * inv_std[o], src(b) = perm[b] when given (mini-batch gather).
+__global__ void k_nll(const float *Y, const float *T, const int *perm, const float *tmean, const float *tinv, int B,
+                      int no, float inv_count, bf16 *dY, double *loss)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double l = 0;
     if (i < B * no) {
         int b = i / no, o = i % no;
+        const long r = perm ? perm[b] : b;
+        const float tn = (T[r * no + o] - tmean[o]) * tinv[o];
         float mu = Y[(long)b * 2 * no + o], s = Y[(long)b * 2 * no + no + o];
         bool in = s > -S_CLAMP && s < S_CLAMP;
         float sc = fminf(fmaxf(s, -S_CLAMP), S_CLAMP);
-        float d = mu - T[i];
+        float d = mu - tn;
         float e = expf(-sc);
         l = 0.5 * ((double)sc + (double)d * d * e);
         dY[(long)b * 2 * no + o] = (bf16)(d * e * inv_count);
@@ -139,22 +157,26 @@ __global__ void k_nll(const float *Y, const float *T, int B, int no, float inv_c
     if ((threadIdx.x & 63) == 0 && l != 0) atomicAdd(loss, l);
 }
 
-// column sums of a BF16 matrix [M x N] (row stride ld) -> FP32 out[N]; 64 columns per block
-__global__ void __launch_bounds__(256) k_colsum(const bf16 *X, int M, int N, long ld, float *out)
+// layer shapes for the kernels that map a flat parameter index to (layer, row, column):
+// weights of layer l at woff[l], din[l] x dout[l] row-major; biases after all weights
+struct LayerMap { long woff[9]; int din[9], dout[9]; int nl; long nw; };
+
+// BF16 copies of FP32 parameter i: wb (the flat layout) and, for weights, wt (each layer
+// transposed to [dout][din], the fused forward's operand layout)
+__device__ __forceinline__ void store_bf16(const LayerMap &L, long i, float x, bf16 *wb, bf16 *wt)
 {
-    __shared__ float red[4][64];
-    int c = threadIdx.x & 63, g = threadIdx.x >> 6;
-    int n = blockIdx.x * 64 + c;
-    float s = 0;
-    if (n < N)
-        for (int m = g; m < M; m += 4) s += (float)X[(long)m * ld + n];
-    red[g][c] = s;
-    __syncthreads();
-    if (g == 0 && n < N) out[n] = red[0][c] + red[1][c] + red[2][c] + red[3][c];
+    const bf16 v = (bf16)x;
+    wb[i] = v;
+    if (i >= L.nw) return;
+    int l = 0;
+    while (l + 1 < L.nl && i >= L.woff[l + 1]) l++;
+    const long j = i - L.woff[l];
+    const int r = (int)(j / L.dout[l]), c = (int)(j - (long)r * L.dout[l]);
+    wt[L.woff[l] + (long)c * L.din[l] + r] = v;
 }
 
-__global__ void k_adam(float *w, const float *gr, float *m1, float *m2, bf16 *wb, long n, float lr, float b1,
-                       float b2, float eps, float c1, float c2, float wd)
+__global__ void k_adam(float *w, const float *gr, float *m1, float *m2, bf16 *wb, bf16 *wt, LayerMap L, long n,
+                       float lr, float b1, float b2, float eps, float c1, float c2, float wd)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -164,13 +186,13 @@ __global__ void k_adam(float *w, const float *gr, float *m1, float *m2, bf16 *wb
     m1[i] = a; m2[i] = v;
     float x = w[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
     w[i] = x;
-    wb[i] = (bf16)x;
+    store_bf16(L, i, x, wb, wt);
 }
 
-__global__ void k_f32_to_bf16(const float *x, bf16 *y, long n)
+__global__ void k_f32_to_bf16(const float *x, bf16 *wb, bf16 *wt, LayerMap L, long n)
 {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < n) y[i] = (bf16)x[i];
+    if (i < n) store_bf16(L, i, x[i], wb, wt);
 }
 
 // xb[b][k] = bf16((x[src(b)][k] - mean[k]) * inv_std[k]), zero in the padding columns;
@@ -185,14 +207,160 @@ __global__ void k_pack_inputs(const float *x, const int *perm, int B, int n_in, 
     xb[i] = k < n_in ? (bf16)((x[r * n_in + k] - mean[k]) * inv_std[k]) : (bf16)0.0f;
 }
 
-__global__ void k_pack_targets(const float *t, const int *perm, int B, int no, const float *mean,
-                               const float *inv_std, float *tn)
+// ---------------------------------------------------------------- fused forward
+// The whole network for 64 rows per workgroup (4 wavefronts) in one launch: the rows'
+// FP32 inputs are normalised into a BF16 tile in LDS, then each layer reads its input tile
+// from LDS (MFMA A fragments, one 16-byte read each) and its weights from the transposed
+// BF16 copy wt ([dout][din] per layer, one 16-byte global read per B fragment; the weights,
+// 0.4 MB for 70-256-256-256-160, stay in L2), keeps the layer's output in registers
+// (wavefront w owns the 16-column tiles w, w + 4, ...), and, after every wavefront has read
+// the tile, writes the ReLU'd BF16 output over it.  The MFMA sequence per output tile is
+// k_gemm's (16x16x32 in increasing k, bias, ReLU, BF16 rounding), so every value equals the
+// per-layer path.  The output layer's epilogue writes the raw outputs and/or the
+// prediction: mean, sigma and the per-row fallback flag (k_predict's arithmetic).
+// Shapes: padded input, hidden width and 2 n_out multiples of 16 (32 for the input), at most
+// FT_W (FT_OUT for 2 n_out).
+#define FT_M 64
+#define FT_W 256
+#define FT_LD (FT_W + 8)     // LDS row pitch in bf16 (528 B: 16-byte aligned fragments)
+#define FT_OUT 192           // 2 n_out at most: the FP32 output tile stays under 64 KB of LDS
+#define FT_YLD (FT_OUT + 4)  // LDS row pitch in floats of the output tile
+
+struct FusedArgs {
+    const float *X; const int *perm; int B;       // FP32 inputs [*][n_in], rows perm[b] or b
+    int n_in, n_in_pad, n_hid, n_layers, n_out;
+    const float *xm, *xinv;                       // input normalisation
+    const bf16 *wt; const float *w;               // transposed BF16 weights; FP32 parameters (biases)
+    bf16 *xb;                                     // optional: the BF16 input tile [B][n_in_pad]
+    bf16 *acts;                                   // optional: hidden activations [l][B][n_hid]
+    float *Y;                                     // optional: raw outputs [B][2 n_out]
+    const float *ym, *ys; float sigma_max;        // optional prediction (mu set): de-normalisation,
+    const uint64_t *off;                          // fallback gate, track offsets (empty tracks
+    float *mu, *sig; uint8_t *fb; uint32_t *n_fb; // never fall back)
+};
+
+__global__ void __launch_bounds__(256) k_mlp_fused(FusedArgs a)
 {
-    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= (long)B * no) return;
-    int b = (int)(i / no), o = (int)(i % no);
-    long r = perm ? perm[b] : b;
-    tn[i] = (t[r * no + o] - mean[o]) * inv_std[o];
+    // one LDS tile: the BF16 activations [FT_M][FT_LD], and after the last layer's MFMAs its
+    // FP32 outputs [FT_M][FT_YLD]
+    extern __shared__ __attribute__((aligned(16))) unsigned char ft_lds[];
+    bf16 (*H)[FT_LD] = (bf16 (*)[FT_LD])ft_lds;
+    __shared__ uint32_t rowbad[FT_M];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blockIdx.x * FT_M;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    const int oc = lane & 15, orow = (lane >> 4) * 4;
+    const int rows = a.B - r0 < FT_M ? a.B - r0 : FT_M;
+    if (tid < FT_M) rowbad[tid] = 0u;
+    for (int e = tid; e < FT_M * a.n_in_pad; e += 256) {
+        const int r = e / a.n_in_pad, k = e - r * a.n_in_pad;
+        float v = 0.f;
+        if (r < rows && k < a.n_in) {
+            const long src = a.perm ? a.perm[r0 + r] : r0 + r;
+            v = (a.X[src * a.n_in + k] - a.xm[k]) * a.xinv[k];
+        }
+        const bf16 b = (bf16)v;
+        H[r][k] = b;
+        if (a.xb && r < rows) a.xb[(long)(r0 + r) * a.n_in_pad + k] = b;
+    }
+    __syncthreads();
+    int K = a.n_in_pad;
+    const long hh = (long)a.n_hid * a.n_hid, w0 = (long)a.n_in_pad * a.n_hid;
+    const int n2 = 2 * a.n_out;
+    const long nw = w0 + (long)(a.n_layers - 1) * hh + (long)a.n_hid * n2;
+#pragma unroll 1
+    for (int l = 0; l <= a.n_layers; l++) {
+        const bool last = l == a.n_layers;
+        const int N = last ? n2 : a.n_hid, nct = N / 16;
+        // the flat parameter layout: weights of every layer, then every bias
+        const bf16 *W = a.wt + (l == 0 ? 0 : w0 + (long)(l - 1) * hh);
+        const float *bias = a.w + nw + (long)l * a.n_hid;
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            bf16x8 af[4];
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++) af[rt] = *(const bf16x8 *)&H[rt * 16 + fr][k0 + fk];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int ct = wave + 4 * j;
+                if (ct < nct) {
+                    const bf16x8 bv = *(const bf16x8 *)(W + (long)(ct * 16 + fr) * K + k0 + fk);
+#pragma unroll
+                    for (int rt = 0; rt < 4; rt++)
+                        acc[rt][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rt], bv, acc[rt][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();                          // the input tile has been read by every wavefront
+        // bias (and ReLU + BF16 for a hidden layer) into the LDS tile: lane l holds column
+        // l % 16, rows (l / 16) * 4 + q of each 16 x 16 tile
+        float *Yf = (float *)ft_lds;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int ct = wave + 4 * j;
+            if (ct >= nct) continue;
+            const int n = ct * 16 + oc;
+            const float bn = bias[n];
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int m = rt * 16 + orow + q;
+                    const float v = acc[rt][j][q] + bn;
+                    if (last) Yf[m * FT_YLD + n] = v;
+                    else H[m][n] = (bf16)(v > 0.f ? v : 0.f);
+                }
+        }
+        __syncthreads();
+        if (!last && a.acts) {                    // the activations kept for the backward pass
+            const int cpr = a.n_hid / 8;
+            for (int e = tid; e < rows * cpr; e += 256) {
+                const int r = e / cpr, c = (e - r * cpr) * 8;
+                *(bf16x8 *)(a.acts + ((long)l * a.B + r0 + r) * a.n_hid + c) = *(const bf16x8 *)&H[r][c];
+            }
+        }
+        K = N;
+    }
+    // the output layer from LDS: raw outputs and/or the prediction
+    const float *Yf = (const float *)ft_lds;
+    for (int e = tid; e < rows * n2; e += 256) {
+        const int r = e / n2, n = e - r * n2, gr = r0 + r;
+        const float v = Yf[r * FT_YLD + n];
+        if (a.Y) a.Y[(long)gr * n2 + n] = v;
+        if (a.mu) {
+            const int no = a.n_out;
+            if (n < no) a.mu[(long)gr * no + n] = a.ym[n] + a.ys[n] * v;
+            else {
+                const int o = n - no;
+                const float sc = fminf(fmaxf(v, -S_CLAMP), S_CLAMP);
+                const float sg = a.ys[o] * expf(0.5f * sc);
+                a.sig[(long)gr * no + o] = sg;
+                if (!(sg <= a.sigma_max)) rowbad[r] = 1u;
+            }
+        }
+    }
+    __syncthreads();
+    if (a.mu && tid < FT_M) {
+        bool f = false;
+        if (tid < rows) {
+            const int gr = r0 + tid;
+            f = rowbad[tid] && a.off[gr + 1] != a.off[gr];
+            a.fb[gr] = f ? 1 : 0;
+        }
+        const uint64_t m = __ballot(f);
+        if (tid == 0 && m) atomicAdd(a.n_fb, (uint32_t)__popcll(m));
+    }
+}
+
+static size_t fused_lds_bytes()
+{
+    const size_t h = (size_t)FT_M * FT_LD * 2, y = (size_t)FT_M * FT_YLD * 4;
+    return h > y ? h : y;
 }
 
 // Per-track features (one workgroup per track, entries [off[t], off[t+1]) of rad):
@@ -298,11 +466,13 @@ struct dna_surrogate {
     size_t n_params = 0;
     float *w = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;   // FP32 master, gradient, Adam moments
     bf16 *wb = nullptr;                                               // BF16 weights used by the GEMMs
+    bf16 *wt = nullptr;                                               // BF16 weights, each layer transposed
+    LayerMap lmap;
     float *norm = nullptr;   // x_mean[n_in], x_inv_std[n_in], y_mean[n_out], y_std[n_out], y_inv_std[n_out]
     long step = 0;
     int cap = 0;             // batch capacity of the buffers below
     bf16 *xb = nullptr, *acts = nullptr, *dh = nullptr, *dh2 = nullptr;
-    float *y = nullptr, *tn = nullptr, *xin = nullptr, *tin = nullptr;
+    float *y = nullptr, *xin = nullptr, *tin = nullptr;
     double *loss = nullptr;
     // prediction buffers
     size_t pcap = 0;
@@ -346,7 +516,6 @@ static int ensure_batch(dna_surrogate *s, int B)
     DNA_HIP(sg_alloc(s->dh, (size_t)B * wmax));
     DNA_HIP(sg_alloc(s->dh2, (size_t)B * wmax));
     DNA_HIP(sg_alloc(s->y, (size_t)B * 2 * s->n_out));
-    DNA_HIP(sg_alloc(s->tn, (size_t)B * s->n_out));
     DNA_HIP(sg_alloc(s->xin, (size_t)B * s->n_in));
     DNA_HIP(sg_alloc(s->tin, (size_t)B * s->n_out));
     s->cap = B;
@@ -368,11 +537,32 @@ static int pack_inputs(dna_surrogate *s, const float *x_dev, const int *perm, in
     return DNA_OK;
 }
 
-static int pack_targets(dna_surrogate *s, const float *t_dev, const int *perm, int B)
+static bool fused_ok(const dna_surrogate *s)
 {
-    long n = (long)B * s->n_out;
-    hipLaunchKernelGGL(k_pack_targets, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s->ctx->stream, t_dev, perm, B,
-                       s->n_out, y_mean(s), y_inv(s), s->tn);
+    return s->ctx->mlp_fused && s->n_in_pad <= FT_W && s->n_hid <= FT_W && s->n_hid % 16 == 0 &&
+           2 * s->n_out <= FT_OUT && (2 * s->n_out) % 16 == 0;
+}
+
+// The fused forward over B rows of FP32 inputs x_dev (row perm[b] when perm is given):
+// writes the BF16 input tile and the hidden activations when training (want_acts), the raw
+// outputs to Y when given, and the prediction when p_mu is given.
+static int forward_fused(dna_surrogate *s, const float *x_dev, const int *perm, int B, bool want_acts, float *Y,
+                         float sigma_max = 0.f, const uint64_t *off = nullptr, float *p_mu = nullptr,
+                         float *p_sig = nullptr, uint8_t *fb = nullptr, uint32_t *n_fb = nullptr)
+{
+    FusedArgs a; memset(&a, 0, sizeof(a));
+    a.X = x_dev; a.perm = perm; a.B = B;
+    a.n_in = s->n_in; a.n_in_pad = s->n_in_pad; a.n_hid = s->n_hid; a.n_layers = s->n_layers; a.n_out = s->n_out;
+    a.xm = x_mean(s); a.xinv = x_inv(s);
+    a.wt = s->wt; a.w = s->w;
+    if (want_acts) { a.xb = s->xb; a.acts = s->acts; }
+    a.Y = Y;
+    if (p_mu) {
+        a.ym = y_mean(s); a.ys = y_std(s); a.sigma_max = sigma_max; a.off = off;
+        a.mu = p_mu; a.sig = p_sig; a.fb = fb; a.n_fb = n_fb;
+    }
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((B + FT_M - 1) / FT_M)), dim3(256), fused_lds_bytes(),
+                       s->ctx->stream, a);
     DNA_HIP(hipGetLastError());
     return DNA_OK;
 }
@@ -417,10 +607,8 @@ static int backward(dna_surrogate *s, int B)
         gw.B = dcur; gw.sbk = ld_d; gw.sbn = 1;
         gw.M = s->din[l]; gw.N = s->dout[l]; gw.K = B;
         gw.C = s->g + s->woff[l]; gw.scm = s->dout[l];
+        gw.colsum = s->g + s->boff[l];            // the bias gradient: column sums of dcur
         int rc = launch_gemm(st, gw); if (rc) return rc;
-        hipLaunchKernelGGL(k_colsum, dim3((s->dout[l] + 63) / 64), dim3(256), 0, st, dcur, B, s->dout[l], ld_d,
-                           s->g + s->boff[l]);
-        DNA_HIP(hipGetLastError());
         if (l == 0) break;
         // gradient at the previous layer's pre-activation: (dcur W^T) * ReLU'(in)
         GemmArgs gd; memset(&gd, 0, sizeof(gd));
@@ -436,14 +624,23 @@ static int backward(dna_surrogate *s, int B)
     return DNA_OK;
 }
 
-static int loss_and_backward(dna_surrogate *s, int B)
+// loss and gradients of rows perm[b] (or b) of the raw targets t_dev
+static int loss_and_backward(dna_surrogate *s, const float *t_dev, const int *perm, int B)
 {
     hipStream_t st = s->ctx->stream;
     int no = s->n_out;
-    hipLaunchKernelGGL(k_nll, dim3((B * no + 255) / 256), dim3(256), 0, st, s->y, s->tn, B, no, 1.0f / (float)(B * no),
-                       s->dh, s->loss);
+    hipLaunchKernelGGL(k_nll, dim3((B * no + 255) / 256), dim3(256), 0, st, s->y, t_dev, perm, y_mean(s), y_inv(s), B,
+                       no, 1.0f / (float)(B * no), s->dh, s->loss);
     DNA_HIP(hipGetLastError());
     return backward(s, B);
+}
+
+// forward of a training batch: the fused kernel, or packing and one GEMM per layer
+static int train_forward(dna_surrogate *s, const float *x_dev, const int *perm, int B)
+{
+    if (fused_ok(s)) return forward_fused(s, x_dev, perm, B, true, s->y);
+    int rc = pack_inputs(s, x_dev, perm, B); if (rc) return rc;
+    return forward(s, B);
 }
 
 static uint64_t sm64(uint64_t &z)
@@ -474,6 +671,9 @@ extern "C" int dna_surrogate_create(dna_ctx *ctx, int n_in, int n_hidden, int n_
     }
     for (int l = 0; l <= n_layers; l++) { s->boff.push_back(off); off += s->dout[l]; }
     s->n_params = off;
+    memset(&s->lmap, 0, sizeof(s->lmap));
+    s->lmap.nl = n_layers + 1; s->lmap.nw = (long)s->boff[0];
+    for (int l = 0; l <= n_layers; l++) { s->lmap.woff[l] = (long)s->woff[l]; s->lmap.din[l] = s->din[l]; s->lmap.dout[l] = s->dout[l]; }
     // He-normal weights (Box-Muller over a SplitMix64 stream), zero biases, zero rows for the
     // padded inputs; the output layer starts 10x smaller
     std::vector<float> h(off, 0.f);
@@ -497,6 +697,7 @@ extern "C" int dna_surrogate_create(dna_ctx *ctx, int n_in, int n_hidden, int n_
     if (e == hipSuccess) e = sg_alloc(s->m1, off);
     if (e == hipSuccess) e = sg_alloc(s->m2, off);
     if (e == hipSuccess) e = sg_alloc(s->wb, off);
+    if (e == hipSuccess) e = sg_alloc(s->wt, off);
     if (e == hipSuccess) e = sg_alloc(s->norm, nrm.size());
     if (e == hipSuccess) e = sg_alloc(s->loss, 1);
     if (e == hipSuccess) e = sg_alloc(s->d_nfb, 1);
@@ -507,7 +708,7 @@ extern "C" int dna_surrogate_create(dna_ctx *ctx, int n_in, int n_hidden, int n_
     if (e == hipSuccess) e = hipMemset(s->g, 0, off * 4);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((off + 255) / 256)), dim3(256), 0, ctx->stream, s->w, s->wb,
-                           (long)off);
+                           s->wt, s->lmap, (long)off);
         e = hipStreamSynchronize(ctx->stream);
     }
     if (e != hipSuccess) {
@@ -523,7 +724,7 @@ extern "C" void dna_surrogate_destroy(dna_surrogate *s)
 {
     if (!s) return;
     (void)hipStreamSynchronize(s->ctx->stream);
-    void *ptrs[] = {s->w, s->g, s->m1, s->m2, s->wb, s->norm, s->loss, s->xb, s->acts, s->dh, s->dh2, s->y, s->tn,
+    void *ptrs[] = {s->w, s->g, s->m1, s->m2, s->wb, s->wt, s->norm, s->loss, s->xb, s->acts, s->dh, s->dh2, s->y,
                     s->xin, s->tin, s->d_off, s->d_E, s->d_F, s->d_mu, s->d_sig, s->d_fb, s->d_nfb};
     for (void *p : ptrs) if (p) (void)hipFree(p);
     dna_ctx *c = s->ctx;
@@ -557,7 +758,7 @@ extern "C" int dna_surrogate_set_params(dna_surrogate *s, const float *in)
     DNA_HIP(hipStreamSynchronize(s->ctx->stream));
     DNA_HIP(hipMemcpy(s->w, in, s->n_params * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((s->n_params + 255) / 256)), dim3(256), 0, s->ctx->stream, s->w,
-                       s->wb, (long)s->n_params);
+                       s->wb, s->wt, s->lmap, (long)s->n_params);
     DNA_HIP(hipStreamSynchronize(s->ctx->stream));
     return DNA_OK;
 }
@@ -589,8 +790,11 @@ extern "C" int dna_surrogate_forward(dna_surrogate *s, const float *X, int B, fl
     int rc = ensure_batch(s, B); if (rc) return rc;
     hipStream_t st = s->ctx->stream;
     DNA_HIP(hipMemcpyAsync(s->xin, X, (size_t)B * s->n_in * 4, hipMemcpyHostToDevice, st));
-    rc = pack_inputs(s, s->xin, nullptr, B); if (rc) return rc;
-    rc = forward(s, B); if (rc) return rc;
+    if (fused_ok(s)) { rc = forward_fused(s, s->xin, nullptr, B, false, s->y); if (rc) return rc; }
+    else {
+        rc = pack_inputs(s, s->xin, nullptr, B); if (rc) return rc;
+        rc = forward(s, B); if (rc) return rc;
+    }
     DNA_HIP(hipMemcpyAsync(Y, s->y, (size_t)B * 2 * s->n_out * 4, hipMemcpyDeviceToHost, st));
     DNA_HIP(hipStreamSynchronize(st));
     return DNA_OK;
@@ -605,11 +809,9 @@ extern "C" int dna_surrogate_grad(dna_surrogate *s, const float *X, const float 
     hipStream_t st = s->ctx->stream;
     DNA_HIP(hipMemcpyAsync(s->xin, X, (size_t)B * s->n_in * 4, hipMemcpyHostToDevice, st));
     DNA_HIP(hipMemcpyAsync(s->tin, T, (size_t)B * s->n_out * 4, hipMemcpyHostToDevice, st));
-    rc = pack_inputs(s, s->xin, nullptr, B); if (rc) return rc;
-    rc = pack_targets(s, s->tin, nullptr, B); if (rc) return rc;
-    rc = forward(s, B); if (rc) return rc;
+    rc = train_forward(s, s->xin, nullptr, B); if (rc) return rc;
     DNA_HIP(hipMemsetAsync(s->loss, 0, sizeof(double), st));
-    rc = loss_and_backward(s, B); if (rc) return rc;
+    rc = loss_and_backward(s, s->tin, nullptr, B); if (rc) return rc;
     double h_loss = 0;
     DNA_HIP(hipMemcpyAsync(&h_loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, st));
     if (grad) DNA_HIP(hipMemcpyAsync(grad, s->g, s->n_params * 4, hipMemcpyDeviceToHost, st));
@@ -650,15 +852,13 @@ extern "C" int dna_surrogate_train(dna_surrogate *s, const float *X, const float
         DNA_HIP(hipMemsetAsync(s->loss, 0, sizeof(double), st));
         for (int bi = 0; bi < nb; bi++) {
             const int *pb = dP + (size_t)bi * batch;
-            rc = pack_inputs(s, dX, pb, batch); if (rc) break;
-            rc = pack_targets(s, dT, pb, batch); if (rc) break;
-            rc = forward(s, batch); if (rc) break;
-            rc = loss_and_backward(s, batch); if (rc) break;
+            rc = train_forward(s, dX, pb, batch); if (rc) break;
+            rc = loss_and_backward(s, dT, pb, batch); if (rc) break;
             s->step++;
             float c1 = 1.f - (float)std::pow((double)b1, (double)s->step);
             float c2 = 1.f - (float)std::pow((double)b2, (double)s->step);
             hipLaunchKernelGGL(k_adam, dim3((unsigned)((s->n_params + 255) / 256)), dim3(256), 0, st, s->w, s->g, s->m1,
-                               s->m2, s->wb, (long)s->n_params, lr, b1, b2, 1e-8f, c1, c2, weight_decay);
+                               s->m2, s->wb, s->wt, s->lmap, (long)s->n_params, lr, b1, b2, 1e-8f, c1, c2, weight_decay);
         }
         if (rc) break;
         // the host permutation is reshuffled next epoch, so this epoch's upload must have
@@ -755,15 +955,21 @@ extern "C" int dna_surrogate_predict(dna_surrogate *s, const float *E, uint32_t 
     rc = features_dev(s, c->d_rad, n); if (rc) return rc;
     DNA_HIP(hipEventRecord(c->ev[1], st));
     DNA_HIP(hipMemsetAsync(s->d_nfb, 0, 4, st));
-    for (uint32_t b0 = 0; b0 < n; b0 += chunk) {
-        int B = (int)std::min<uint32_t>(chunk, n - b0);
-        rc = pack_inputs(s, s->d_F + (size_t)b0 * NF_TOTAL, nullptr, B); if (rc) return rc;
-        rc = forward(s, B); if (rc) return rc;
-        hipLaunchKernelGGL(k_predict, dim3((B + 3) / 4), dim3(256), 0, st, s->y, B, s->n_out, y_mean(s), y_std(s),
-                           s->d_off + b0, sigma_max, s->d_mu + (size_t)b0 * s->n_out, s->d_sig + (size_t)b0 * s->n_out,
-                           s->d_fb + b0, s->d_nfb);
-        DNA_HIP(hipGetLastError());
-    }
+    if (fused_ok(s) && n <= 0x7FFFFFFFu) {
+        // one launch for every track: normalisation, the layers and the prediction epilogue
+        rc = forward_fused(s, s->d_F, nullptr, (int)n, false, nullptr, sigma_max, s->d_off, s->d_mu, s->d_sig, s->d_fb,
+                           s->d_nfb);
+        if (rc) return rc;
+    } else
+        for (uint32_t b0 = 0; b0 < n; b0 += chunk) {
+            int B = (int)std::min<uint32_t>(chunk, n - b0);
+            rc = pack_inputs(s, s->d_F + (size_t)b0 * NF_TOTAL, nullptr, B); if (rc) return rc;
+            rc = forward(s, B); if (rc) return rc;
+            hipLaunchKernelGGL(k_predict, dim3((B + 3) / 4), dim3(256), 0, st, s->y, B, s->n_out, y_mean(s), y_std(s),
+                               s->d_off + b0, sigma_max, s->d_mu + (size_t)b0 * s->n_out, s->d_sig + (size_t)b0 * s->n_out,
+                               s->d_fb + b0, s->d_nfb);
+            DNA_HIP(hipGetLastError());
+        }
     DNA_HIP(hipEventRecord(c->ev[2], st));
     DNA_HIP(hipMemcpyAsync(fallback, s->d_fb, n, hipMemcpyDeviceToHost, st));
     uint32_t nf = 0;

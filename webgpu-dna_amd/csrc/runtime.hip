@@ -168,6 +168,10 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
         c->pa_lanes = value;
         c->tr_tuned.clear();
         return DNA_OK;
+    case DNA_OPT_MLP_FUSED:
+        if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_MLP_FUSED must be 0 or 1"); return DNA_ERR_ARG; }
+        c->mlp_fused = value;
+        return DNA_OK;
     case DNA_OPT_RAD_CHUNK:
         if (value != 0 && (value < 256 || value > 4096 || (value & (value - 1)))) {
             set_error("dna_set_option: DNA_OPT_RAD_CHUNK must be 0 (autotune) or a power of two from 256 to 4096");
@@ -201,6 +205,7 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     case DNA_OPT_IRT_FINAL: *value = c->irt_final; return DNA_OK;
     case DNA_OPT_PA_LANES: *value = c->pa_lanes; return DNA_OK;
     case DNA_OPT_RAD_CHUNK: *value = c->rad_chunk; return DNA_OK;
+    case DNA_OPT_MLP_FUSED: *value = c->mlp_fused; return DNA_OK;
     case DNA_OPT_WORK_CHUNK: *value = c->work_chunk; return DNA_OK;
     case DNA_OPT_PA_LANES_LAST: *value = c->last_pa_lanes; return DNA_OK;
     case DNA_OPT_RAD_CHUNK_LAST: *value = c->last_rad_chunk; return DNA_OK;

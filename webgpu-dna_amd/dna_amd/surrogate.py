@@ -56,6 +56,20 @@ def g_error(pred, ref, floor=0.05):
             "mean_rel": float(rel.mean()) if rel.size else 0.0, "n_outputs": int(m.sum())}
 
 
+def sigma_calibration(mu, sigma, G_ref, floor=0.05):
+    """Per-track calibration of the predicted sigma against reference G (IRT): over every
+    (track, output) whose mean reference G is at least floor, the share of |mu - G| within
+    1 and 2 sigma (68.3% and 95.4% for a calibrated Gaussian), the per-track RMSE and mean
+    absolute error of mu, and the mean predicted sigma."""
+    mu, sigma, G_ref = (np.asarray(a, np.float64) for a in (mu, sigma, G_ref))
+    m = G_ref.mean(0) >= floor
+    d = np.abs(mu[:, m] - G_ref[:, m])
+    s = sigma[:, m]
+    return {"within_1sigma": round(float((d <= s).mean()), 4), "within_2sigma": round(float((d <= 2 * s).mean()), 4),
+            "rmse": round(float(np.sqrt((d ** 2).mean())), 5), "mae": round(float(d.mean()), 5),
+            "mean_sigma": round(float(s.mean()), 5), "outputs": int(m.sum()), "tracks": int(mu.shape[0])}
+
+
 def norm_stats(X, G):
     """Per-column mean and std of features and targets (std floored so that constant
     columns normalise to zero rather than dividing by zero)."""
@@ -155,7 +169,8 @@ def surrogate_chemistry(ctx: Context, surr: Surrogate, rad_n, E, sigma_max, seed
     fbm = fb.astype(bool)
     Gt = np.where(fbm[:, None], per_track_G(tally_fb, E), mu)
     return {"G": aggregate_G(Gt, E), "mu": mu, "sigma": sg, "fallback": fbm, "n_fallback": int(nf),
-            "t_predict_ms": float(t_pred[2]), "t_irt_ms": float(o.t_ms) if o is not None else 0.0,
+            "t_predict_ms": float(t_pred[2]), "t_features_ms": float(t_pred[0]), "t_network_ms": float(t_pred[1]),
+            "t_irt_ms": float(o.t_ms) if o is not None else 0.0,
             "wall_predict_s": t1 - t0, "wall_irt_s": t2 - t1}
 
 
