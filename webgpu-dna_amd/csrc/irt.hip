@@ -813,17 +813,29 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
     const uint64_t below = (1ull << lane) - 1ull;
     volatile uint32_t *q = sh.rq[w];
     uint32_t qn = 0;                                   // wavefront-uniform
+    // The list entries of the next 64 flattened indices are loaded before the current 64
+    // are screened, so their latency overlaps the screen.  A lane's index only grows, so its
+    // segment is found by moving forward from the previous one.
+    int sg = 0;
+    uint32_t jj_n = 0, ps_n = 0;
+    float4 pj_n = make_float4(0.f, 0.f, 0.f, 0.f);
+    auto fetch = [&](uint32_t u) {
+        while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
+        ps_n = sh.seg_ps[sg];
+        const size_t slot = (size_t)(ps_n & 15u) * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
+        jj_n = lst[slot];
+        pj_n = lps[slot];                              // x, y, z, birth time rounded down
+    };
+    if (w * 64u + lane < tot) fetch(w * 64u + lane);
     for (uint32_t u0 = w * 64u; u0 < tot; u0 += IRT_NT) {
         const uint32_t u = u0 + lane;
+        const uint32_t jj = jj_n, psg = ps_n;
+        const float4 pj = pj_n;
+        if (u + IRT_NT < tot) fetch(u + IRT_NT);
         bool keep = false;
         uint32_t ent = 0;
         if (u < tot) {
-            int sg = 0;
-            while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
-            const uint32_t p = sh.seg_ps[sg] >> 4, sj = sh.seg_ps[sg] & 15u;
-            const size_t slot = (size_t)sj * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
-            const uint32_t jj = lst[slot];
-            const float4 pj = lps[slot];          // x, y, z, birth time rounded down
+            const uint32_t p = psg >> 4, sj = psg & 15u;
             const uint32_t idx = sh.prod_idx[p];
             if (al[jj] && jj != idx && !(p == 0 && np == 2 && jj == pidx1)) {
                 const uint32_t pc = sh.prod_sp[p] * 6u + sj;
