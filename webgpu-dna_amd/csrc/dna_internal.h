@@ -70,6 +70,7 @@ enum { C_OH = 0, C_EAQ = 1, C_H = 2, C_H3O = 3, C_DNA = 4, C_H2 = 5, C_SEC = 6, 
        C_SST = 8 /* 8..15 sec_stats */, C_DBG = 16 /* 16..23 dbg */, C_PNEXT = 24, C_SNEXT = 25,
        C_WRAP = 26 /* set when the radical slot counter would pass 2^32 */, C_NCTR = 32 };
 
+int num_cu();                                  // compute units of the current device
 uint32_t phase_a_lanes(uint32_t n, int opt);
 size_t transport_slot_headroom(uint32_t n, uint32_t lanes, uint32_t rad_chunk);
 hipError_t launch_phase_a(const TransportDev &d, uint32_t lanes, hipStream_t st);

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <type_traits>
 
 namespace dna {
 
@@ -367,11 +368,14 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 // ------------------------------------------------------------ per-track kernels
 // Two launches per batch, both in launch order (largest tracks first):
 //   k_irt_pairs  one workgroup per track (64, 128 or 256 threads, the size autotuned): the
-//                displacement of the initial species, the species-sorted copy (in LDS), the
-//                initial pairs of every reaction channel and their FP32 screens; the pairs
-//                that may react are queued in HBM with their FP64 distance
+//                displacement of the initial species, the sorted copy (in LDS, by species
+//                and hash cell), the initial pairs of every reaction channel and their FP32
+//                screens; the pairs that may react are queued in HBM with their FP64
+//                distance, then the whole workgroup samples them (the drain: the rest of
+//                sampleIRT, SamplePDC's rejection loops included) into the event list.  A
+//                track's drain overlaps the pair enumeration of the other workgroups on its CU.
 //   k_irt_loop   one wavefront per track, 16 tracks per CU (128 VGPRs, under 10 KB of LDS):
-//                the FP64 sampling of the queued pairs (the drain), then the event loop.  The
+//                the event loop.  The
 //                next event comes from per-block minima of the event list kept in LDS: the
 //                global minimum is popped, and when it is stale (a reactant has reacted) it is
 //                dropped and only its block is rescanned (lazy deletion).  The events that one
@@ -389,6 +393,7 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
                                      // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n,
                                      // [12] windows on, [13..15] axis, [16] umin, [17] cinv,
                                      // [18] cells, [19] cells per bucket, [20] buckets,
+                                     // [21] events of the initial pairs (drained by k_irt_pairs),
                                      // [32..127] bucket ends, 2 x u16 per word, 16 words per species
 #define WB_MAX 32u                   // buckets of the re-pairing windows
 
@@ -441,6 +446,12 @@ struct PairShared {
     float    rlo[4], rhi[4];
     uint32_t cs[6 * NX_MAX + 1];
     uint32_t cu[NX_MAX + 1];              // units of the current channel block before each cell
+    // the drain of the queued pairs (pdc_drain): event count, queue and long-list cursors,
+    // the long list (in the dynamic LDS of the sorted copy, no longer needed by then)
+    uint32_t n_ev, pend_next, long_n, long_next, long_cap;
+    uint32_t *long_item;
+    int8_t   rmap[36];
+    double   Dd[6];
 };
 
 // shared state of k_irt_loop (one wavefront)
@@ -502,15 +513,22 @@ struct LoopShared {
 
 extern __shared__ float4 dyn_lds[];
 
+// capacity of the long list of the SamplePDC drain
+__device__ __forceinline__ uint32_t long_cap(const LoopShared &) { return LoopShared::LONG_CAP; }
+__device__ __forceinline__ uint32_t long_cap(const PairShared &sh) { return sh.long_cap; }
+
 // An event of the loop kernel: stored in HBM and staged for the block minima.
-__device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, LoopShared &sh,
-                                           double t, uint32_t a, uint32_t b)
+template <class SH>
+__device__ __forceinline__ void push_event(const IrtArgs &A, const IrtTrack &T, SH &sh, double t, uint32_t a,
+                                           uint32_t b)
 {
     const uint32_t e = atomicAdd(&sh.n_ev, 1u);
     if (e < T.ev_cap) {
         A.ev_t[T.ev_off + e] = t; A.ev_ab[T.ev_off + e] = make_uint2(a, b);
-        const uint32_t s = atomicAdd(&sh.nt_n, 1u);
-        if (s < STAGE_MAX) { sh.nt_e[s] = e; sh.nt_t[s] = t; sh.nt_ab[s] = ((uint64_t)a << 32) | b; }
+        if constexpr (std::is_same<SH, LoopShared>::value) {
+            const uint32_t s = atomicAdd(&sh.nt_n, 1u);
+            if (s < STAGE_MAX) { sh.nt_e[s] = e; sh.nt_t[s] = t; sh.nt_ab[s] = ((uint64_t)a << 32) | b; }
+        }
     } else atomicOr(A.overflow, 2u);
 }
 
@@ -540,7 +558,8 @@ __device__ __forceinline__ void push_pending(const IrtArgs &A, const IrtTrack &T
 struct PendPair { uint32_t a, b; double r0; int ri; double D; uint64_t ps; };
 
 // A queued pair with its r0 (r0 < 0: a re-pairing beyond R_CUT, no event).
-__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, const LoopShared &sh,
+template <class SH>
+__device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrack &T, const SH &sh,
                                                  uint64_t tk, uint32_t item)
 {
     uint4 e = A.pend[T.ev_off + item];
@@ -554,75 +573,154 @@ __device__ __forceinline__ PendPair load_pending(const IrtArgs &A, const IrtTrac
     return q;
 }
 
-__device__ __forceinline__ void pdc_setup(const PendPair &q, double &a, double &b, double &p, double &pq, double &M)
+// SamplePDC constants of one pair (irt-worker.js:91-94), with the per-trial subexpressions
+// that do not depend on the trial computed once: S = p + q M, the branch threshold p / S
+// and 2 b / a (the same operations on the same operands as inside the trial, so the values
+// are identical)
+struct PdcC { double a, b, p, q, M, S, thr, tba; };
+__device__ __forceinline__ PdcC pdc_setup(const PendPair &pr)
 {
-    pdc_ab(q.r0, q.ri, q.D, a, b);
-    p = 2.0 * sqrt(2.0 * b / a);
-    double qq = 2.0 / sqrt(2.0 * b / a);
-    double M1 = 1.0 / (a * a), M2 = 3.0 * b / a;
-    M = M1 > M2 ? M1 : M2;
-    pq = qq;
+    PdcC c;
+    pdc_ab(pr.r0, pr.ri, pr.D, c.a, c.b);
+    c.p = 2.0 * sqrt(2.0 * c.b / c.a);
+    c.q = 2.0 / sqrt(2.0 * c.b / c.a);
+    const double M1 = 1.0 / (c.a * c.a), M2 = 3.0 * c.b / c.a;
+    c.M = M1 > M2 ? M1 : M2;
+    c.S = c.p + c.q * c.M;
+    c.thr = c.p / c.S;
+    c.tba = 2.0 * c.b / c.a;
+    return c;
 }
 
-// one SamplePDC trial (irt-worker.js:95-111); returns accept.  The acceptance value is
-// lam = E f with E = exp(-b^2/X), f = 1 - a sqrt(pi X) erfcx(z), z = b/sqrt(X) + a sqrt(X).
-// Most trials are decided in FP32 first, with margins far above FP32 rounding, so only
-// trials whose uniform lies within the margins of lam reach the FP64 exp and erfcx:
+// One SamplePDC trial (irt-worker.js:95-111), first part: the proposal X and the second
+// uniform U, and a decision by bounds in FP32: 0 reject, 1 accept, 2 undecided (pdc_exact
+// decides).  The acceptance value is lam = E f with E = exp(-b^2/X),
+// f = 1 - a sqrt(pi X) erfcx(z), z = b/sqrt(X) + a sqrt(X).  Most trials are decided by the
+// bounds, whose margins are far above FP32 rounding:
 //  * always lam <= E (a > 0, erfcx > 0): E is taken with a 1e-4 relative and 1e-37
 //    absolute margin;
 //  * for z >= 0, 1/(z + sqrt(z^2 + 2)) < (sqrt(pi)/2) erfcx(z) <= 1/(z + sqrt(z^2 + 4/pi))
 //    (Abramowitz & Stegun 7.1.13) bracket f; z and a sqrt(pi X) are taken with 1e-5
 //    relative margins (the bounds decrease in z) and f with 1e-5 absolute ones.
-// (The bracket below takes its FP32 divisions, square roots and the reciprocal of X with
-// the hardware instructions, each within a few ulp (< 1e-6 relative): far inside its
-// 1e-5 and 1e-4 margins, and the exact path decides every trial they do not.)
+// (The bracket takes its FP32 divisions, square roots and the reciprocal of X with the
+// hardware instructions, each within a few ulp (< 1e-6 relative): far inside its 1e-5 and
+// 1e-4 margins, and the exact path decides every trial they do not.)
 __device__ __forceinline__ float fdiv_bound(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
 
-__device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q, double M, uint64_t ps,
-                                          uint32_t trial, double &X)
+__device__ __forceinline__ int pdc_fast(const PdcC &c, uint64_t ps, uint32_t trial, double &X, double &U2)
 {
     // draws 7 + 2 trial and 8 + 2 trial of the pair stream: dna_u01(ps, n) is the finaliser
     // of ps + G (n + 1), so the second argument is the first plus G
     const uint64_t x0 = ps + DNA_GOLDEN * (uint64_t)(8u + 2u * trial);
-    double U = (double)(dna_mix64(x0) >> 11) * 1.1102230246251565404e-16;
+    const double U = (double)(dna_mix64(x0) >> 11) * 1.1102230246251565404e-16;
     double v;
-    if (U < p / (p + q * M)) { v = U * (p + q * M) / 2; X = v * v; }
-    else { v = 2 / ((1 - U) * (p + q * M) / M); X = v * v; }
-    U = (double)(dna_mix64(x0 + DNA_GOLDEN) >> 11) * 1.1102230246251565404e-16;
-    {
-        const bool c1 = X <= 2.0 * b / a, c2 = X >= 2.0 * b / a;
-        const float Xf = (float)X, rX = __builtin_amdgcn_rcpf(Xf);
-        // b^2 / X to a few ulp: the exponent's error (< 3e-7 |b^2/X|) moves E by under 3e-5
-        // of itself while E is above the 1e-37 absolute margin (|b^2/X| < 87)
-        const float E = expf(-((float)(b * b) * rX));
-        const float Ehi = E * 1.0001f + 1e-37f;
-        const float fU = (float)U;
-        // the value compared with lam: U (X below 2b/a) or U M / X (above); X = 2b/a exactly
-        // (both tests) and NaN go to the exact path
-        const float w = c1 ? fU : fU * (float)M * rX;
-        const float whi = w * 1.0001f, wlo = w * 0.9999f;
-        if (c1 != c2) {
-            if (wlo > Ehi) return false;
-            const float bf = (float)b;
-            if (bf >= 0.f) {
-                const float sq = __builtin_amdgcn_sqrtf(Xf);
-                const float z = fdiv_bound(bf, sq) + (float)a * sq;
-                const float K = (float)a * sq * 1.7724539f;                 // a sqrt(pi X)
-                const float zl = z * 0.99999f, zh = z * 1.00001f;
-                const float Lh = fdiv_bound(1.1283792f, zh + __builtin_amdgcn_sqrtf(zh * zh + 2.f));        // erfcx(z) >  Lh
-                const float Ul = fdiv_bound(1.1283792f, zl + __builtin_amdgcn_sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
-                const float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
-                const float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
-                const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
-                const float llo = (E * 0.9999f - 1e-37f) * fmaxf(flo, 0.f);
-                if (wlo > lhi) return false;
-                if (whi < llo && llo > 0.f) return true;
-            }
-        }
+    if (U < c.thr) v = U * c.S / 2;
+    else v = 2 / ((1 - U) * c.S / c.M);
+    X = v * v;
+    U2 = (double)(dna_mix64(x0 + DNA_GOLDEN) >> 11) * 1.1102230246251565404e-16;
+    const bool c1 = X <= c.tba, c2 = X >= c.tba;
+    const float Xf = (float)X, rX = __builtin_amdgcn_rcpf(Xf);
+    // b^2 / X to a few ulp: the exponent's error (< 3e-7 |b^2/X|) moves E by under 3e-5
+    // of itself while E is above the 1e-37 absolute margin (|b^2/X| < 87)
+    const float E = expf(-((float)(c.b * c.b) * rX));
+    const float Ehi = E * 1.0001f + 1e-37f;
+    const float fU = (float)U2;
+    // the value compared with lam: U (X below 2b/a) or U M / X (above); X = 2b/a exactly
+    // (both tests) and NaN go to the exact path
+    const float w = c1 ? fU : fU * (float)c.M * rX;
+    const float whi = w * 1.0001f, wlo = w * 0.9999f;
+    const float bf = (float)c.b;
+    const float sq = __builtin_amdgcn_sqrtf(Xf);
+    const float z = fdiv_bound(bf, sq) + (float)c.a * sq;
+    const float K = (float)c.a * sq * 1.7724539f;                       // a sqrt(pi X)
+    const float zl = z * 0.99999f, zh = z * 1.00001f;
+    const float Lh = fdiv_bound(1.1283792f, zh + __builtin_amdgcn_sqrtf(zh * zh + 2.f));        // erfcx(z) >  Lh
+    const float Ul = fdiv_bound(1.1283792f, zl + __builtin_amdgcn_sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
+    const float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
+    const float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
+    const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
+    const float llo = (E * 0.9999f - 1e-37f) * fmaxf(flo, 0.f);
+    const bool one = c1 != c2, rejE = wlo > Ehi, zok = bf >= 0.f;
+    const bool rej = one && (rejE || (zok && wlo > lhi));
+    const bool acc = one && !rejE && zok && !(wlo > lhi) && whi < llo && llo > 0.f;
+    return rej ? 0 : acc ? 1 : 2;
+}
+
+// exp(z^2) erfc(z) for z >= 0 in FP32 with a relative bound m on its distance to dna_erfcx
+// (include/dna_irt_math.h): below 6 the same Chebyshev series in FP32 (largest error
+// 3.9e-7 over 4e6 points against the FP64 routine, from coefficient and operation rounding),
+// from 6 on the alternating asymptotic series sqrt(pi) z erfcx(z) = 1 - 1/(2z^2) + 3/(4z^4)
+// - 15/(8z^6) + 105/(16z^8), whose remainder is below the first omitted term 945/(32z^10)
+// (at most 4.9e-7 from z = 6; the FP64 routine's 30-step continued fraction is far closer to
+// the function than that).  m = 1e-5 on top covers the FP32 rounding and expf.
+__constant__ float c_cheb32[28] = {
+    -1.3026537197817094f, 6.4196979235649026e-1f, 1.9476473204185836e-2f,
+    -9.561514786808631e-3f, -9.46595344482036e-4f, 3.66839497852761e-4f,
+    4.2523324806907e-5f, -2.0278578112534e-5f, -1.624290004647e-6f,
+    1.303655835580e-6f, 1.5626441722e-8f, -8.5238095915e-8f, 6.529054439e-9f,
+    5.059343495e-9f, -9.91364156e-10f, -2.27365122e-10f, 9.6467911e-11f,
+    2.394038e-12f, -6.886027e-12f, 8.94487e-13f, 3.13092e-13f, -1.12708e-13f,
+    3.81e-16f, 7.106e-15f, -1.523e-15f, -9.4e-17f, 1.21e-16f, -2.8e-17f};
+__device__ __forceinline__ void erfcx_bounds(float z, float &lo, float &hi)
+{
+    if (z < 6.f) {
+        const float t = 2.f / (2.f + z), ty = 4.f * t - 2.f;
+        float d = 0.f, dd = 0.f;
+        for (int j = 27; j > 0; j--) { const float tmp = d; d = ty * d - dd + c_cheb32[j]; dd = tmp; }
+        const float e = t * expf(0.5f * (c_cheb32[0] + ty * d) - dd);
+        lo = e * 0.99999f; hi = e * 1.00001f;
+    } else {
+        const float r = 1.f / (z * z);
+        const float sm = 1.f - r * (0.5f - r * (0.75f - r * (1.875f - r * 6.5625f)));
+        const float rem = 29.53125f * r * r * r * r * r;
+        const float q = 0.56418958f / z;                   // 1 / (sqrt(pi) z)
+        lo = (sm - rem) * q * 0.99999f; hi = (sm + rem) * q * 1.00001f;
     }
+}
+
+// Second decision for a trial the first bounds left undecided (pdc_fast returned 2): the
+// exact path's own FP64 quantities b^2/X, z and a sqrt(pi X) rounded to FP32 (relative
+// error 6e-8: taken with 1e-6 margins; the exponent b^2/X below 87 moves E by under 5.5e-6
+// with expf, taken with 5e-5), erfcx from erfcx_bounds at the ends of z's interval (erfcx
+// decreases), and U (M / X) in FP64.  0 reject, 1 accept, 2 still undecided (pdc_exact).
+__device__ __forceinline__ int pdc_refine(const PdcC &c, double X, double U)
+{
+    const double a = c.a, b = c.b;
+    const bool c1 = X <= 2.0 * b / a, c2 = X >= 2.0 * b / a;
+    if (c1 == c2 || !(b >= 0.0)) return 2;
+    const double sqX = sqrt(X);
+    const float E = expf(-(float)(b * b / X));
+    const float Ehi = E * 1.00005f + 1e-37f, Elo = E * 0.99995f - 1e-37f;
+    const float z = (float)(b / sqX + a * sqX), K = (float)(a * sqrt(DNA_PI * X));
+    float lo, hi, lo2, hi2;
+    erfcx_bounds(z * 1.000001f, lo, hi2);
+    erfcx_bounds(z * 0.999999f, lo2, hi);
+    const float fhi = 1.f - K * 0.999999f * lo + 1e-6f;
+    const float flo = 1.f - K * 1.000001f * hi - 1e-6f;
+    const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
+    const float llo = Elo * fmaxf(flo, 0.f);
+    const float w = (float)(c1 ? U : U * c.M / X);
+    const float whi = w * 1.000001f, wlo = w * 0.999999f;
+    if (wlo > lhi) return 0;
+    if (whi < llo && llo > 0.f) return 1;
+    return 2;
+}
+
+// the exact acceptance test of a trial that the bounds left undecided
+__device__ __forceinline__ bool pdc_exact(const PdcC &c, double X, double U)
+{
+    const double a = c.a, b = c.b, M = c.M;
     double sqX = sqrt(X);
     double lam = dna_exp(-b * b / X) * (1.0 - a * sqrt(DNA_PI * X) * dna_erfcx(b / sqX + a * sqX));
     return (X <= 2.0 * b / a && U <= lam) || (X >= 2.0 * b / a && U * M / X <= lam);
+}
+
+__device__ __forceinline__ bool pdc_trial(const PdcC &c, uint64_t ps, uint32_t trial, double &X)
+{
+    double U;
+    int d = pdc_fast(c, ps, trial, X, U);
+    if (d == 2) d = pdc_refine(c, X, U);
+    return d == 2 ? pdc_exact(c, X, U) : d == 1;
 }
 
 #define LANE_TRIALS 64u
@@ -633,10 +731,11 @@ __device__ __forceinline__ bool pdc_trial(double a, double b, double p, double q
 // refill, so the first stage and the SamplePDC setup (a global load and FP64 code) run once
 // per refill for all idle lanes instead of in almost every trial.
 #define REFILL 8u
-__device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
+template <class SH>
+__device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &sh,
                                 uint32_t npend, double t_base)
 {
-    double a = 1, b = 1, p = 0, q = 0, M = 0;
+    PdcC c; c.a = 1; c.b = 1; c.p = 0; c.q = 0; c.M = 0; c.S = 1; c.thr = 0; c.tba = 0;
     PendPair pp; pp.a = 0; pp.b = 0; pp.D = 1; pp.ps = 0; pp.r0 = 0; pp.ri = 0;
     uint32_t item = 0, trial = 0;
     bool have = false, more = true;
@@ -647,7 +746,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
                 if (item >= npend) { more = false; break; }
                 pp = load_pending(A, T, sh, tk, item);
                 const double t1 = pp.r0 < 0 ? -1.0 : irt_stage1(pp.r0, pp.ri, pp.D, pp.ps);
-                if (t1 == STAGE_PDC) { pdc_setup(pp, a, b, p, q, M); trial = 0; have = true; break; }
+                if (t1 == STAGE_PDC) { c = pdc_setup(pp); trial = 0; have = true; break; }
                 if (t1 >= 0) { double ta = t_base + t1; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
             }
         }
@@ -655,7 +754,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
         for (uint32_t k = 0; k < REFILL; k++) {
             if (!have) continue;
             double X = -1;
-            const bool acc = pdc_trial(a, b, p, q, M, pp.ps, trial, X);
+            const bool acc = pdc_trial(c, pp.ps, trial, X);
             trial++;
             if (acc) {
                 if (X > 0) { double ta = t_base + X / pp.D; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
@@ -663,7 +762,7 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
             } else if (trial == LANE_TRIALS) {
                 // to the wavefront schedule; when its list is full the lane goes on itself
                 const uint32_t l = atomicAdd(&sh.long_n, 1u);
-                if (l < LoopShared::LONG_CAP) { sh.long_item[l] = item; have = false; }
+                if (l < long_cap(sh)) { sh.long_item[l] = item; have = false; }
             } else if (trial == 10000u) have = false;
         }
     }
@@ -671,11 +770,12 @@ __device__ void pdc_drain_lanes(const IrtArgs &A, const IrtTrack &T, uint64_t tk
 
 // wavefront schedule over items [from the queue, or the long list when use_long; those
 // resume at trial long_start]
-__device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
+template <class SH>
+__device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &sh,
                                 uint32_t npend, double t_base, bool use_long, uint32_t long_start = LANE_TRIALS)
 {
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t nlong = sh.long_n < LoopShared::LONG_CAP ? sh.long_n : LoopShared::LONG_CAP;
+    const uint32_t nlong = sh.long_n < long_cap(sh) ? sh.long_n : long_cap(sh);
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(use_long ? &sh.long_next : &sh.pend_next, 1u);
@@ -691,14 +791,13 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
                 continue;
             }
         } else start = long_start;
-        double a, b, p, q, M;
-        pdc_setup(pp, a, b, p, q, M);
+        const PdcC c = pdc_setup(pp);
         double X = -1;
         for (uint32_t base = start; base < 10000u; base += 64u) {
             uint32_t trial = base + lane;
             double Xl = 0;
             bool acc = false;
-            if (trial < 10000u) acc = pdc_trial(a, b, p, q, M, pp.ps, trial, Xl);
+            if (trial < 10000u) acc = pdc_trial(c, pp.ps, trial, Xl);
             unsigned long long m = __ballot(acc);
             if (m) { int first = __ffsll((long long)m) - 1; X = __shfl(Xl, first, 64); break; }
         }
@@ -706,10 +805,11 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     }
 }
 
-// small queues (fewer than 2 x 64 items, so at most LoopShared::LONG_CAP - 1): the first
+// small queues (fewer than twice the workgroup's threads, so within the long list): the first
 // stage of every pair on the first wavefront, a lane per pair; the pairs that need SamplePDC
 // trials are listed and run on the wavefront schedule from trial 0
-__device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh,
+template <class SH>
+__device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &sh,
                                 uint32_t npend, double t_base)
 {
     if (threadIdx.x < 64u) {
@@ -725,7 +825,8 @@ __device__ void pdc_drain_small(const IrtArgs &A, const IrtTrack &T, uint64_t tk
 }
 
 // drain the whole queue; all threads call it after a barrier
-__device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, LoopShared &sh, double t_base)
+template <class SH>
+__device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &sh, double t_base)
 {
     const uint32_t npend = sh.npend < T.ev_cap ? sh.npend : T.ev_cap;
     if (npend >= 2u * IRT_NT) {
@@ -1338,6 +1439,21 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
     }
     __syncthreads();
     const unsigned long long tp2 = wall_clock64();
+    // 4. the queued pairs: the rest of sampleIRT (irt-worker.js:116-184) in FP64, SamplePDC's
+    // rejection loops included, into the track's event list, by the whole workgroup; a track's
+    // drain overlaps the pair enumeration of the other workgroups on its CU.  The long list
+    // takes the dynamic LDS of the sorted copy.
+    if (tid == 0) {
+        sh.n_ev = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
+        sh.long_cap = A.lds_slots * 4u < 1024u ? A.lds_slots * 4u : 1024u;
+        sh.long_item = (uint32_t *)dyn_lds;
+    }
+    if (tid < 36) sh.rmap[tid] = (int8_t)C.rxnMap[tid];
+    if (tid < 6) sh.Dd[tid] = C.D[tid];
+    __syncthreads();
+    pdc_drain(A, T, tk, sh, 0.0);
+    __syncthreads();
+    const unsigned long long tp3 = wall_clock64();
     for (int o = 32; o > 0; o >>= 1) { wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); }
     if ((tid & 63) == 0) {
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 11;
@@ -1365,13 +1481,14 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
         st[7] = (uint32_t)sh.c[0]; st[8] = (uint32_t)sh.c[1]; st[9] = (uint32_t)sh.c[2];
         st[10] = T.h2; st[11] = n;
         st[12] = win ? 1u : 0u;
+        st[21] = sh.n_ev;
         if (win) {
             st[13] = __float_as_uint(sh.ax[0]); st[14] = __float_as_uint(sh.ax[1]); st[15] = __float_as_uint(sh.ax[2]);
             st[16] = __float_as_uint(sh.umin); st[17] = __float_as_uint(sh.cinv);
             st[18] = nx; st[19] = wg; st[20] = wnb;
         }
         unsigned long long *g = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
-        atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tp2 - tp1);
+        atomicAdd(&g[0], tp1 - tp0); atomicAdd(&g[1], tp2 - tp1); atomicAdd(&g[6], tp3 - tp2);
         // the worker's own enumeration: every pair of alive initial particles with a channel
         const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
         unsigned long long all = 0;
@@ -1418,8 +1535,8 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     const uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
     const unsigned long long tp0 = wall_clock64();
     if (lane == 0) {
-        sh.npend = st[0]; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
-        sh.n_ev = 0; sh.n_total = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
+        sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
+        sh.n_ev = st[21]; sh.n_total = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
         for (int s = 0; s < 6; s++) { sh.llen[s] = st[1 + s]; sh.ldead[s] = 0; }
         sh.win = st[12];
         for (int s = 0; s < 6; s++) sh.n0[s] = sh.win ? st[1 + s] : 0u;
@@ -1461,12 +1578,9 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         __syncthreads();
         al = la;
     }
-    // 1. the queued initial pairs: the rest of sampleIRT (irt-worker.js:116-184) in FP64
-    pdc_drain(A, T, tk, sh, 0.0);
-    __syncthreads();
+    // 1. the events of the initial pairs (k_irt_drain) into the block minima
     const unsigned long long n_init_ev = sh.n_ev;
     if (lane == 0) {
-        sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; sh.nt_n = 0;
         const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
         sh.nblk = (ne + sh.bs - 1) / sh.bs;
     }
@@ -1939,6 +2053,12 @@ static size_t bytes_scratch(uint64_t part, uint64_t rad_n)
 {
     return std::max<size_t>(lpos_offset(part) + (size_t)part * 6 * 16 + 256, rad_n / 1024 * 4 + 64);
 }
+// events: times, pairs and the SamplePDC queue (32 bytes per event slot), then the per-track
+// hand-over words
+static size_t bytes_events(uint64_t evs, uint32_t ntr)
+{
+    return evs * 32 + (size_t)ntr * TSTATE * 4 + 256;
+}
 static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
 {
     return tally_bytes + 256 + G.tr.size() * sizeof(IrtTrack) + (G.K + 1) * 4 + (size_t)G.run * 4 + tt_bytes + 1536;
@@ -2011,8 +2131,9 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
     A.nc = nc; A.seed = seed; A.seed_ptr = nullptr;
     // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in global
-    // memory above; the event loop reuses that LDS for alive flags and events
-    A.lds_slots = std::min<uint32_t>(G.maxn_all, 3072u);
+    // memory above; the drain's long list reuses that LDS afterwards (at least 128 slots:
+    // 512 list entries, twice the largest workgroup, as the small-queue drain needs)
+    A.lds_slots = std::max<uint32_t>(std::min<uint32_t>(G.maxn_all, 3072u), 128u);
     // the loop kernel keeps a track's alive flags in LDS when its slots fit in AL_LDS_MAX
     // bytes (16 one-wavefront workgroups per CU with LoopShared)
     uint32_t capmax = 0;
@@ -2022,17 +2143,15 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     return A;
 }
 
+static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st);
+
+// one IRT launch: the pairs kernel (with the drain of the queued pairs), the event loop
 static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
 {
-    const size_t shmem = (size_t)A.lds_slots * 16;
+    irt_launch_pairs(A, grid, block, st);
     // the death-time stores are compiled only into the variants that keep the final state
-    if (A.tdeath) {
-        hipLaunchKernelGGL((k_irt_pairs<true>), dim3(grid), dim3(block), shmem, st, A);
-        hipLaunchKernelGGL((k_irt_loop<true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
-    } else {
-        hipLaunchKernelGGL((k_irt_pairs<false>), dim3(grid), dim3(block), shmem, st, A);
-        hipLaunchKernelGGL((k_irt_loop<false>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
-    }
+    if (A.tdeath) hipLaunchKernelGGL((k_irt_loop<true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+    else hipLaunchKernelGGL((k_irt_loop<false>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
 }
 
 static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
@@ -2172,7 +2291,7 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         uint64_t part = 0, evs = 0;
         irt_size(G, attempt, ev_mult, part, evs);
         DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch, bytes_scratch(part, rad_n)));
-        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, evs * 32 + (size_t)ntr * TSTATE * 4 + 256));
+        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, bytes_events(evs, ntr)));
         const size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
         const size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
         DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, bytes_block(G, tally_bytes, tt_bytes)));
@@ -2338,7 +2457,7 @@ static int plan_build(dna_irt_plan *P)
     irt_size(G, P->attempt, P->ev_mult, part, evs);
     P->tally_bytes = (size_t)G.n_inst * TALLY_STRIDE * 8;
     const size_t s_bytes = (bytes_scratch(part, G.rad_n) + 255) & ~(size_t)255;
-    const size_t e_bytes = (evs * 32 + (size_t)ntr * TSTATE * 4 + 256 + 255) & ~(size_t)255;
+    const size_t e_bytes = (bytes_events(evs, ntr) + 255) & ~(size_t)255;
     const size_t b_bytes = bytes_block(G, P->tally_bytes, 0);
     DNA_HIP(hipMalloc((void **)&P->buf, s_bytes + e_bytes + b_bytes + 256));
     IrtBufs b;

@@ -869,7 +869,7 @@ __global__ void __launch_bounds__(256) k_phase_b(TransportDev d, uint32_t sec_n)
 }
 
 static int g_num_cu = 0;
-static int num_cu()
+int num_cu()
 {
     if (!g_num_cu) {
         int dev = 0, n = 0;
