@@ -245,6 +245,13 @@ def main():
                     help="chain steps in flight: independent batches on that many contexts (each its "
                          "own HIP stream and buffers), driven by as many host threads, so one batch's "
                          "latency-bound transport overlaps another's IRT")
+    ap.add_argument("--irt-drain", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="where the IRT drains its initial pairs' queue (DNA_OPT_IRT_DRAIN): 0 the "
+                         "event-loop kernel, 1 the pairs kernel, 2 a kernel of its own; -1 the "
+                         "library default")
+    ap.add_argument("--irt-order", type=int, default=-1, choices=(-1, 0, 1),
+                    help="the IRT event-loop workgroups heaviest track first (1) or in launch "
+                         "order (0) (DNA_OPT_IRT_ORDER); -1 the library default")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (a plain copy kernel of known "
                          "size that calibrates the FETCH_SIZE / WRITE_SIZE counters)")
@@ -268,6 +275,14 @@ def main():
     from dna_amd import Context, DNATarget, default_params
     K = max(1, args.inflight)
     ctxs = [Context(0) for _ in range(K)]
+    if args.irt_drain >= 0:
+        from dna_amd.abi import DNA_OPT_IRT_DRAIN
+        for c in ctxs:
+            c.set_option(DNA_OPT_IRT_DRAIN, args.irt_drain)
+    if args.irt_order >= 0:
+        from dna_amd.abi import DNA_OPT_IRT_ORDER
+        for c in ctxs:
+            c.set_option(DNA_OPT_IRT_ORDER, args.irt_order)
     ctx = ctxs[0]
     dna = DNATarget()
     p = default_params(E_EV, N_PRIM, dna=dna)

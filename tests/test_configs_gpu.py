@@ -120,7 +120,8 @@ def test_g_1us_against_e10(ctx, E):
 def test_captured_irt_batch_replays_match_eager(ctx):
     """The 512-instance batch captured as one hipGraph (dna_irt_plan): every replay, with
     the program's own eager work and synchronisations in between, equals the eager
-    dna_run_irt_batch result for the same seed."""
+    dna_run_irt_batch result for the same seed, and the first replay equals the oracle's
+    batch (oracle_irt_batch) instance by instance."""
     n = 512
     E = S.log_uniform_energies(n, E10_IRT_SEED, 1000.0, 20000.0)
     p = S.transport_params(n, seed=E10_IRT_SEED)
@@ -133,6 +134,14 @@ def test_captured_irt_batch_replays_match_eager(ctx):
     print("irt plan from the resident radicals", flush=True)
     plan = ctx.irt_plan(None, off, nth, E, tc)
     try:
+        print("oracle batch, seed 21", flush=True)
+        ob = O.irt_batch(rad, off, nth, E, tc, seed=21, n_threads=8)
+        first = plan.run(21)
+        for i in range(n):
+            a = [[first[i].tally[c][s] for s in range(5)] for c in range(16)]
+            b = [[ob[i].tally[c][s] for s in range(5)] for c in range(16)]
+            assert a == b, i
+            assert list(first[i].rxn_counts) == list(ob[i].rxn_counts), i
         for k, seed in enumerate((21, 22, 21, 23)):
             print(f"replay {k} seed {seed}", flush=True)
             gp = plan.run(seed)

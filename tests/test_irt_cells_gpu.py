@@ -142,3 +142,64 @@ def test_cell_option_is_checked(ctx):
         with pytest.raises(DnaError):
             ctx.set_option(DNA_OPT_IRT_CELL, bad)
     assert ctx.get_option(DNA_OPT_IRT_CELL) == 0
+
+
+@pytest.mark.parametrize("drain", [0, 1, 2])
+def test_drain_placement_keeps_the_results(ctx, rad20k, drain):
+    """The initial pairs' SamplePDC drain in the loop kernel (0), the pairs kernel (1) or a
+    kernel of its own (2): the same events, so the same results as the oracle."""
+    from dna_amd.abi import DNA_OPT_IRT_DRAIN
+    o = O.irt(rad20k, 24, 20000.0, CK, seed=13, n_threads=8)
+    ctx.set_option(DNA_OPT_IRT_DRAIN, drain)
+    try:
+        print(f"irt 24 x 20 keV, drain placement {drain}", flush=True)
+        g = ctx.irt(rad20k, None, 24, 20000.0, CK, seed=13)
+        assert ctx.get_option(DNA_OPT_IRT_DRAIN) == drain
+    finally:
+        ctx.set_option(DNA_OPT_IRT_DRAIN, 0)
+    _same(g, o)
+
+
+def test_drain_option_is_checked(ctx):
+    from dna_amd.abi import DNA_OPT_IRT_DRAIN
+    from dna_amd.lib import DnaError
+    for bad in (-1, 3):
+        with pytest.raises(DnaError):
+            ctx.set_option(DNA_OPT_IRT_DRAIN, bad)
+    assert ctx.get_option(DNA_OPT_IRT_DRAIN) == 0
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_loop_order_keeps_the_results(ctx, rad20k, order):
+    """The event-loop workgroups in launch order (0) or heaviest track first (1): each
+    track's chain is the same, so the results are the oracle's either way."""
+    from dna_amd.abi import DNA_OPT_IRT_ORDER
+    o = O.irt(rad20k, 24, 20000.0, CK, seed=17, n_threads=8)
+    ctx.set_option(DNA_OPT_IRT_ORDER, order)
+    try:
+        print(f"irt 24 x 20 keV, loop order {order}", flush=True)
+        g = ctx.irt(rad20k, None, 24, 20000.0, CK, seed=17)
+        assert ctx.get_option(DNA_OPT_IRT_ORDER) == order
+    finally:
+        ctx.set_option(DNA_OPT_IRT_ORDER, 1)
+    _same(g, o)
+
+
+def test_loop_order_over_several_chunks(ctx):
+    """5000 tracks: two runs of the ordering sort (4096 tracks each at most), the second one
+    partial; the results equal the launch-order run's and the oracle's."""
+    from dna_amd.abi import DNA_OPT_IRT_ORDER
+    print("transport 5000 x 1 keV for IRT input", flush=True)
+    rad = ctx.transport(default_params(1000.0, 5000))["rad"]
+    o = O.irt(rad, 5000, 1000.0, CK, seed=19, n_threads=8)
+    runs = []
+    for order in (1, 0):
+        ctx.set_option(DNA_OPT_IRT_ORDER, order)
+        try:
+            print(f"irt 5000 x 1 keV, loop order {order}", flush=True)
+            runs.append(ctx.irt(rad, None, 5000, 1000.0, CK, seed=19))
+        finally:
+            ctx.set_option(DNA_OPT_IRT_ORDER, 1)
+    assert runs[0].n_tracks > 4096
+    for g in runs:
+        _same(g, o)

@@ -68,9 +68,13 @@ struct IrtArgs {
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy (pairs kernel)
+    int drain_at;                 // the initial pairs' drain: 0 in k_irt_loop, 1 in k_irt_pairs,
+                                  // 2 in k_irt_drain
     int cell_m;                   // initial-pair hash: 0 = one cell per species, m = cells of
                                   // CELL_R / m along the track's principal axis
     uint32_t *tstate;             // TSTATE u32 per track (launch order): pairs kernel -> loop kernel
+    uint32_t *order;              // optional: the track (launch index) of each loop-kernel
+                                  // workgroup, heaviest first; null = launch order
     uint32_t al_lds;              // bytes of dynamic LDS for the alive flags (loop kernel)
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
@@ -393,7 +397,7 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
                                      // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n,
                                      // [12] windows on, [13..15] axis, [16] umin, [17] cinv,
                                      // [18] cells, [19] cells per bucket, [20] buckets,
-                                     // [21] events of the initial pairs (drained by k_irt_pairs),
+                                     // [21] events of the initial pairs drained by k_irt_pairs,
                                      // [32..127] bucket ends, 2 x u16 per word, 16 words per species
 #define WB_MAX 32u                   // buckets of the re-pairing windows
 
@@ -511,11 +515,20 @@ struct LoopShared {
     unsigned long long acc[7];
 };
 
+// shared state of k_irt_drain (the initial pairs' drain in a kernel of its own)
+struct DrainShared {
+    uint32_t n_ev, npend, pend_next, long_n, long_next;
+    uint32_t long_item[1024];                   // pairs handed from the per-lane to the wavefront schedule
+    int8_t   rmap[36];
+    double   Dd[6];
+};
+
 extern __shared__ float4 dyn_lds[];
 
 // capacity of the long list of the SamplePDC drain
 __device__ __forceinline__ uint32_t long_cap(const LoopShared &) { return LoopShared::LONG_CAP; }
 __device__ __forceinline__ uint32_t long_cap(const PairShared &sh) { return sh.long_cap; }
+__device__ __forceinline__ uint32_t long_cap(const DrainShared &) { return 1024u; }
 
 // An event of the loop kernel: stored in HBM and staged for the block minima.
 template <class SH>
@@ -599,13 +612,37 @@ __device__ __forceinline__ PdcC pdc_setup(const PendPair &pr)
 // bounds, whose margins are far above FP32 rounding:
 //  * always lam <= E (a > 0, erfcx > 0): E is taken with a 1e-4 relative and 1e-37
 //    absolute margin;
-//  * for z >= 0, 1/(z + sqrt(z^2 + 2)) < (sqrt(pi)/2) erfcx(z) <= 1/(z + sqrt(z^2 + 4/pi))
+//  * for z < 6, 1/(z + sqrt(z^2 + 2)) < (sqrt(pi)/2) erfcx(z) <= 1/(z + sqrt(z^2 + 4/pi))
 //    (Abramowitz & Stegun 7.1.13) bracket f; z and a sqrt(pi X) are taken with 1e-5
-//    relative margins (the bounds decrease in z) and f with 1e-5 absolute ones.
-// (The bracket takes its FP32 divisions, square roots and the reciprocal of X with the
-// hardware instructions, each within a few ulp (< 1e-6 relative): far inside its 1e-5 and
+//    relative margins (the bounds decrease in z) and f with 1e-5 absolute ones;
+//  * for z >= 6, f is small (1 - a sqrt(pi X) erfcx(z) cancels to 1e-5 .. 1e-14 in most
+//    trials there) and is bounded without the cancellation (f_asym): 1e-5 relative and
+//    1e-13 absolute margins.
+// (The bounds take their FP32 divisions, square roots and the reciprocal of X with the
+// hardware instructions, each within a few ulp (< 1e-6 relative): far inside the 1e-5 and
 // 1e-4 margins, and the exact path decides every trial they do not.)
 __device__ __forceinline__ float fdiv_bound(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
+// f = 1 - a sqrt(pi X) erfcx(z) for large z without its cancellation: with
+// S = sqrt(pi) z erfcx(z) and z sqrt(X) = b + a X,
+//   f = 1 - S a X / (b + a X) = (b + a X (1 - S)) / (b + a X),
+// and from the asymptotic series of erfc (Abramowitz & Stegun 7.1.23; for real z its
+// remainder has the sign of, and is at most, the first omitted term)
+//   1 - S = r/2 - 3r^2/4 + 15r^3/8 - 105r^4/16 + R,  0 <= R <= 945 r^5/32,  r = 1/z^2.
+// Every term is positive (b >= 0), so FP32 keeps f to a few 1e-7 relative even where f is
+// 1e-14 (the FP32 evaluation of the bracket [lo, hi] adds at most ~2e-6 relative; the
+// callers' margins cover it).  The FP64 routine's f carries the rounding of its own
+// cancellation, at most ~1e-15 absolute (its erfcx and a sqrt(pi X) to a few ulp): the
+// callers' 1e-13 absolute margin covers that.
+__device__ __forceinline__ void f_asym(float aX, float b, float z, float &lo, float &hi)
+{
+    const float r = __builtin_amdgcn_rcpf(z * z);
+    const float t4 = r * (0.5f - r * (0.75f - r * (1.875f - r * 6.5625f)));
+    const float rm = 29.53125f * r * r * r * r * r;
+    const float den = __builtin_amdgcn_rcpf(aX + b);
+    lo = (b + aX * t4) * den;
+    hi = (b + aX * (t4 + rm)) * den;
+}
 
 __device__ __forceinline__ int pdc_fast(const PdcC &c, uint64_t ps, uint32_t trial, double &X, double &U2)
 {
@@ -636,8 +673,11 @@ __device__ __forceinline__ int pdc_fast(const PdcC &c, uint64_t ps, uint32_t tri
     const float zl = z * 0.99999f, zh = z * 1.00001f;
     const float Lh = fdiv_bound(1.1283792f, zh + __builtin_amdgcn_sqrtf(zh * zh + 2.f));        // erfcx(z) >  Lh
     const float Ul = fdiv_bound(1.1283792f, zl + __builtin_amdgcn_sqrtf(zl * zl + 1.2732395f)); // erfcx(z) <= Ul
-    const float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
-    const float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
+    float fhi = 1.f - K * 0.99999f * Lh * 0.99999f + 1e-5f;
+    float flo = 1.f - K * 1.00001f * Ul * 1.00001f - 1e-5f;
+    float gl, gh;
+    f_asym((float)c.a * Xf, bf, z, gl, gh);
+    if (z >= 6.f) { fhi = gh * 1.00001f + 1e-13f; flo = gl * 0.99999f - 1e-13f; }
     const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
     const float llo = (E * 0.9999f - 1e-37f) * fmaxf(flo, 0.f);
     const bool one = c1 != c2, rejE = wlo > Ehi, zok = bf >= 0.f;
@@ -679,10 +719,11 @@ __device__ __forceinline__ void erfcx_bounds(float z, float &lo, float &hi)
 }
 
 // Second decision for a trial the first bounds left undecided (pdc_fast returned 2): the
-// exact path's own FP64 quantities b^2/X, z and a sqrt(pi X) rounded to FP32 (relative
+// exact path's own FP64 quantities b^2/X, z, a X and a sqrt(pi X) rounded to FP32 (relative
 // error 6e-8: taken with 1e-6 margins; the exponent b^2/X below 87 moves E by under 5.5e-6
-// with expf, taken with 5e-5), erfcx from erfcx_bounds at the ends of z's interval (erfcx
-// decreases), and U (M / X) in FP64.  0 reject, 1 accept, 2 still undecided (pdc_exact).
+// with expf, taken with 5e-5); for z < 6 erfcx from erfcx_bounds at the ends of z's
+// interval (erfcx decreases), for z >= 6 f from f_asym with 4e-6 relative and 1e-13
+// absolute margins; U (M / X) in FP64.  0 reject, 1 accept, 2 still undecided (pdc_exact).
 __device__ __forceinline__ int pdc_refine(const PdcC &c, double X, double U)
 {
     const double a = c.a, b = c.b;
@@ -691,12 +732,20 @@ __device__ __forceinline__ int pdc_refine(const PdcC &c, double X, double U)
     const double sqX = sqrt(X);
     const float E = expf(-(float)(b * b / X));
     const float Ehi = E * 1.00005f + 1e-37f, Elo = E * 0.99995f - 1e-37f;
-    const float z = (float)(b / sqX + a * sqX), K = (float)(a * sqrt(DNA_PI * X));
-    float lo, hi, lo2, hi2;
-    erfcx_bounds(z * 1.000001f, lo, hi2);
-    erfcx_bounds(z * 0.999999f, lo2, hi);
-    const float fhi = 1.f - K * 0.999999f * lo + 1e-6f;
-    const float flo = 1.f - K * 1.000001f * hi - 1e-6f;
+    const float z = (float)(b / sqX + a * sqX);
+    float fhi, flo;
+    if (z >= 6.f) {
+        float gl, gh;
+        f_asym((float)(a * X), (float)b, z, gl, gh);
+        fhi = gh * 1.000004f + 1e-13f; flo = gl * 0.999996f - 1e-13f;
+    } else {
+        const float K = (float)(a * sqrt(DNA_PI * X));
+        float lo, hi, lo2, hi2;
+        erfcx_bounds(z * 1.000001f, lo, hi2);
+        erfcx_bounds(z * 0.999999f, lo2, hi);
+        fhi = 1.f - K * 0.999999f * lo + 1e-6f;
+        flo = 1.f - K * 1.000001f * hi - 1e-6f;
+    }
     const float lhi = Ehi * fmaxf(fhi, 0.f) + 1e-37f;
     const float llo = Elo * fmaxf(flo, 0.f);
     const float w = (float)(c1 ? U : U * c.M / X);
@@ -1451,8 +1500,10 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
     if (tid < 36) sh.rmap[tid] = (int8_t)C.rxnMap[tid];
     if (tid < 6) sh.Dd[tid] = C.D[tid];
     __syncthreads();
-    pdc_drain(A, T, tk, sh, 0.0);
-    __syncthreads();
+    if (A.drain_at == 1) {
+        pdc_drain(A, T, tk, sh, 0.0);
+        __syncthreads();
+    }
     const unsigned long long tp3 = wall_clock64();
     for (int o = 32; o > 0; o >>= 1) { wcand += __shfl_xor(wcand, o, 64); wsamp += __shfl_xor(wsamp, o, 64); }
     if ((tid & 63) == 0) {
@@ -1521,18 +1572,82 @@ __device__ __forceinline__ void block_min(const IrtArgs &A, const IrtTrack &T, L
     if (lane == 0) { sh.bm_t[j] = bt; sh.bm_ab[j] = bab; }
 }
 
+// The initial pairs' drain as a kernel of its own (DNA_OPT_IRT_DRAIN = 2): one
+// DRAIN_NT-thread workgroup per track, light registers and 4 KB of LDS, between the pairs
+// and the loop kernels; the event count goes to the loop kernel through tstate[21].
+#define DRAIN_NT 128
+__global__ void __launch_bounds__(DRAIN_NT) k_irt_drain(IrtArgs A)
+{
+    __shared__ DrainShared sh;
+    const IrtTrack T = A.tracks[blockIdx.x];
+    const uint32_t tid = threadIdx.x;
+    uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
+    const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
+    const unsigned long long tp0 = wall_clock64();
+    if (tid == 0) { sh.n_ev = 0; sh.npend = st[0]; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; }
+    if (tid < 36) sh.rmap[tid] = (int8_t)c_irt.rxnMap[tid];
+    if (tid < 6) sh.Dd[tid] = c_irt.D[tid];
+    __syncthreads();
+    pdc_drain(A, T, tk, sh, 0.0);
+    __syncthreads();
+    if (tid == 0) {
+        st[21] = sh.n_ev;
+        unsigned long long *pf = A.tally + (size_t)T.inst * TALLY_STRIDE + DNA_MAX_CKPT * 5 + 14;
+        atomicAdd(&pf[6], wall_clock64() - tp0);
+    }
+}
+
+// The loop kernel's workgroup order (DNA_OPT_IRT_ORDER): within each run of ORDER_CHUNK
+// tracks (about one resident wave of loop workgroups: 16 per CU), the tracks by decreasing
+// queued pairs + initial particles, ties by launch index.  A SIMD's arbiter issues for its
+// oldest wavefront first, so a workgroup's dispatch rank sets how much of its SIMD it gets
+// while the others still run: the heaviest chains take the first ranks.  Bitonic sort of
+// (~weight, index) in LDS, one workgroup per chunk; the order changes only when each
+// track's wavefront runs, not what it computes.
+#define ORDER_CHUNK 4096u
+#define ORDER_NT 1024
+__global__ void __launch_bounds__(ORDER_NT) k_irt_order(const uint32_t *tstate, uint32_t ntr, uint32_t *order)
+{
+    __shared__ unsigned long long key[ORDER_CHUNK];
+    const uint32_t base = blockIdx.x * ORDER_CHUNK;
+    const uint32_t len = ntr - base < ORDER_CHUNK ? ntr - base : ORDER_CHUNK;
+    for (uint32_t i = threadIdx.x; i < ORDER_CHUNK; i += ORDER_NT) {
+        unsigned long long k = ~0ull;
+        if (i < len) {
+            const uint32_t *st = tstate + (size_t)TSTATE * (base + i);
+            const uint32_t w = st[0] + st[11];
+            k = ((unsigned long long)(~w) << 32) | (base + i);
+        }
+        key[i] = k;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= ORDER_CHUNK; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < ORDER_CHUNK; i += ORDER_NT) {
+                const uint32_t l = i ^ j;
+                if (l > i) {
+                    const unsigned long long a = key[i], b = key[l];
+                    if ((a > b) == ((i & k) == 0)) { key[i] = b; key[l] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < len; i += ORDER_NT) order[base + i] = (uint32_t)key[i];
+}
+
 template <bool DEATH>
 __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
 {
     const IrtConst &C = c_irt;
     __shared__ LoopShared sh;
-    const IrtTrack T = A.tracks[blockIdx.x];
+    const uint32_t li = A.order ? A.order[blockIdx.x] : blockIdx.x;     // the track's launch index
+    const IrtTrack T = A.tracks[li];
     const uint32_t lane = threadIdx.x;
     float4 *P = A.pos + T.part_off;
     uint8_t *al = A.alive + T.part_off;
     double *tb = A.tb + T.part_off;
     const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
-    const uint32_t *st = A.tstate + (size_t)TSTATE * blockIdx.x;
+    const uint32_t *st = A.tstate + (size_t)TSTATE * li;
     const unsigned long long tp0 = wall_clock64();
     if (lane == 0) {
         sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
@@ -1578,7 +1693,16 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         __syncthreads();
         al = la;
     }
-    // 1. the events of the initial pairs (k_irt_drain) into the block minima
+    // 1. the queued initial pairs (unless k_irt_pairs drained them): the rest of sampleIRT
+    // (irt-worker.js:116-184) in FP64, then every event into the block minima
+    if (A.drain_at == 0) {
+        if (lane == 0) sh.npend = st[0];
+        __syncthreads();
+        pdc_drain(A, T, tk, sh, 0.0);
+        __syncthreads();
+        if (lane == 0) { sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0; sh.nt_n = 0; }
+        __syncthreads();
+    }
     const unsigned long long n_init_ev = sh.n_ev;
     if (lane == 0) {
         const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
@@ -2054,10 +2178,10 @@ static size_t bytes_scratch(uint64_t part, uint64_t rad_n)
     return std::max<size_t>(lpos_offset(part) + (size_t)part * 6 * 16 + 256, rad_n / 1024 * 4 + 64);
 }
 // events: times, pairs and the SamplePDC queue (32 bytes per event slot), then the per-track
-// hand-over words
+// hand-over words and the loop kernel's workgroup order
 static size_t bytes_events(uint64_t evs, uint32_t ntr)
 {
-    return evs * 32 + (size_t)ntr * TSTATE * 4 + 256;
+    return evs * 32 + (size_t)ntr * (TSTATE + 1) * 4 + 256;
 }
 static size_t bytes_block(const IrtGroup &G, size_t tally_bytes, size_t tt_bytes)
 {
@@ -2140,15 +2264,20 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     for (auto &t : G.tr) capmax = t.cap > capmax ? t.cap : capmax;
     A.al_lds = (std::min<uint32_t>(capmax, AL_LDS_MAX) + 15u) & ~15u;
     A.tstate = (uint32_t *)(b.events + evs * 32);
+    A.order = A.tstate + (size_t)G.tr.size() * TSTATE;
     return A;
 }
 
 static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st);
 
-// one IRT launch: the pairs kernel (with the drain of the queued pairs), the event loop
+// one IRT launch: the pairs kernel, the drain kernel when the drain runs on its own, the
+// loop kernel's workgroup order, the event loop
 static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
 {
     irt_launch_pairs(A, grid, block, st);
+    if (A.drain_at == 2) hipLaunchKernelGGL(k_irt_drain, dim3(grid), dim3(DRAIN_NT), 0, st, A);
+    if (A.order) hipLaunchKernelGGL(k_irt_order, dim3((grid + ORDER_CHUNK - 1) / ORDER_CHUNK), dim3(ORDER_NT), 0,
+                                    st, (const uint32_t *)A.tstate, grid, A.order);
     // the death-time stores are compiled only into the variants that keep the final state
     if (A.tdeath) hipLaunchKernelGGL((k_irt_loop<true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
     else hipLaunchKernelGGL((k_irt_loop<false>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
@@ -2300,6 +2429,8 @@ static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const
         carve_block(G, (char *)c->irt.tally, tally_bytes, track_tally != nullptr, b);
         rc = irt_prepare(c, G, b); if (rc) return rc;
         IrtArgs A = irt_args(G, d_rad, b, part, evs, t_ckpt, nc, seed);
+        A.drain_at = c->irt_drain;
+        if (!c->irt_order) A.order = nullptr;
         if (c->irt_final) {
             A.final_ntot = c->d_ntot;
             A.tdeath = (double *)((char *)c->irt.scratch + ((part * 65 + 7) & ~(uint64_t)7));
@@ -2466,6 +2597,8 @@ static int plan_build(dna_irt_plan *P)
     P->d_seed = (uint64_t *)(P->buf + s_bytes + e_bytes + b_bytes);
     rc = irt_prepare(c, G, b); if (rc) return rc;
     IrtArgs A = irt_args(G, P->rad, b, part, evs, P->tck.data(), P->nc, 0);
+    A.drain_at = c->irt_drain;
+    if (!c->irt_order) A.order = nullptr;
     if (ntr) { rc = irt_pick_launch(c, G, A, b, P->block); if (rc) return rc; }
     P->cell = A.cell_m ? A.cell_m : -1;
     A.seed_ptr = P->d_seed;

@@ -10,6 +10,8 @@ DNA_OPT_IRT_CELL, DNA_OPT_IRT_CELL_LAST = 5, 6
 DNA_OPT_RAD_CHUNK, DNA_OPT_WORK_CHUNK = 7, 8
 DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK_LAST, DNA_OPT_WORK_CHUNK_LAST = 9, 10, 11
 DNA_OPT_MLP_FUSED = 12
+DNA_OPT_IRT_DRAIN = 13
+DNA_OPT_IRT_ORDER = 14
 
 
 class DnaParams(C.Structure):
