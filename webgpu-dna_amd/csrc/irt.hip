@@ -75,7 +75,7 @@ struct IrtArgs {
     uint32_t *tstate;             // TSTATE u32 per track (launch order): pairs kernel -> loop kernel
     uint32_t *order;              // optional: the track (launch index) of each loop-kernel
                                   // workgroup, heaviest first; null = launch order
-    uint32_t al_lds;              // bytes of dynamic LDS for the alive flags (loop kernel)
+    uint32_t al_lds;              // bytes of dynamic LDS for the loop kernel's alive bits (0: global)
     unsigned long long *tally;    // per instance: [ckpt][5], then 9 rxn, n_reacted, n_events
     uint32_t *track_tally;        // optional per-track [ckpt][5] species counts (track order)
     uint32_t *final_ntot;         // optional: particles per track at the end (track order); the
@@ -391,7 +391,7 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 #define IRT_NT (blockDim.x)
 #define BM_MAX 128u                  // event blocks per track (a block holds 64 k events)
 #define STAGE_MAX 64u                // events staged per reaction (more: blocks rescanned)
-#define AL_LDS_MAX 3584u             // alive flags in LDS up to this many particle slots
+#define AL_LDS_MAX 3584u             // bytes of LDS for the loop kernel's alive bits (28,672 slots)
 #define TSTATE 128                   // u32 per track handed from k_irt_pairs to k_irt_loop:
                                      // [0] queued pairs, [1..6] species list lengths,
                                      // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n,
@@ -524,6 +524,26 @@ struct DrainShared {
 };
 
 extern __shared__ float4 dyn_lds[];
+typedef __attribute__((address_space(3))) volatile uint32_t lds_vu32;
+
+// The loop kernel's alive flags, one bit per particle slot of its track (the species of a
+// slot is P[k].w).  LDS_BITS: in the dynamic LDS, so that every test and update is an LDS
+// access (a flag array reachable through either address space compiles to flat accesses,
+// whose waits also wait for every global load in flight); else in global memory, in the
+// sorted-copy region, which the pairs kernel no longer needs (launches whose largest track
+// exceeds the LDS budget).  Updated by one lane at a time, read by all after the wavefront
+// synchronises.
+template <bool LDS_BITS>
+struct AliveBits {
+    uint32_t *g;                                      // the global words (LDS_BITS false)
+    __device__ __forceinline__ uint32_t *w() const { return LDS_BITS ? (uint32_t *)dyn_lds : g; }
+    __device__ __forceinline__ bool operator[](uint32_t k) const { return (w()[k >> 5] >> (k & 31u)) & 1u; }
+    __device__ __forceinline__ void clear(uint32_t k) const { w()[k >> 5] &= ~(1u << (k & 31u)); }
+    __device__ __forceinline__ void set(uint32_t k) const { w()[k >> 5] |= 1u << (k & 31u); }
+};
+
+// words of the alive bits of a track of cap slots (whole 64-slot chunks)
+__host__ __device__ __forceinline__ uint32_t alive_words(uint32_t cap) { return 2u * ((cap + 63u) / 64u); }
 
 // capacity of the long list of the SamplePDC drain
 __device__ __forceinline__ uint32_t long_cap(const LoopShared &) { return LoopShared::LONG_CAP; }
@@ -889,16 +909,17 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &
 
 // The exact part for one staged re-pairing (product p, partner jj): the partner's
 // Brownian step, r0, the first sampling stage, and the queueing of pairs that can react.
-__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
+template <class AL>
+__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al,
                                              uint64_t tk, double t_new, uint32_t e)
 {
     const float4 *P = A.pos + T.part_off;
     const double *tb = A.tb + T.part_off;
     const uint32_t p = e >> 31, jj = e & 0x7FFFFFFFu;
-    const uint32_t sj = (uint32_t)al[jj] - 1u, sp = sh.prod_sp[p], pc = sp * 6u + sj;
+    const float4 pi = sh.prod_pos[p], pj = P[jj];
+    const uint32_t sj = (uint32_t)pj.w, sp = sh.prod_sp[p], pc = sp * 6u + sj;
     const int rj = sh.rmap[pc];
     const uint32_t idx = sh.prod_idx[p];
-    const float4 pi = sh.prod_pos[p], pj = P[jj];
     const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
     const uint64_t ps = dna_pair_state(tk, a, b);
     const double dt = t_new - tb[jj];
@@ -951,7 +972,8 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
 // the survivors are staged per wavefront and finished 64 at a time (the FP64 step,
 // r0 = |product - partner| (irt-worker.js:443-456), the first sampling stage), so that
 // part runs on full wavefronts; pairs that can react are queued for the drain.
-__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
+template <class AL>
+__device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al,
                               uint64_t tk, double t_new)
 {
     const uint32_t *lst = A.lst + 6 * T.part_off;
@@ -961,7 +983,10 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
     const uint32_t pidx1 = sh.prod_idx[1];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     const uint64_t below = (1ull << lane) - 1ull;
-    volatile uint32_t *q = sh.rq[w];
+    // written and read by different lanes between wave barriers: volatile, and typed as LDS
+    // so that the accesses stay LDS instructions (a generic volatile pointer compiles to
+    // flat accesses, which wait for every global load in flight)
+    lds_vu32 *q = (lds_vu32 *)sh.rq[w];
     uint32_t qn = 0;                                   // wavefront-uniform
     // The list entries of the next 64 flattened indices are loaded before the current 64
     // are screened, so their latency overlaps the screen.  A lane's index only grows, so its
@@ -1054,7 +1079,8 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
 // cell order, and the ends of its buckets are moved with it.  In place: a chunk of 64
 // entries is read (the loads complete before the ballot that the stores depend on) before
 // any of its slots is rewritten, and an entry only ever moves to a lower position.
-__device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al)
+template <class AL>
+__device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al)
 {
     uint32_t *lst = A.lst + 6 * T.part_off;
     float4 *lps = A.lpos + 6 * T.part_off;
@@ -1553,7 +1579,8 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
 
 // The minimum of the live events of block j into the block minima (all lanes of the
 // wavefront; lane 0 writes it).  Events of reacted particles are skipped.
-__device__ __forceinline__ void block_min(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const uint8_t *al,
+template <class AL>
+__device__ __forceinline__ void block_min(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al,
                                           uint32_t j)
 {
     const uint32_t lane = threadIdx.x & 63u;
@@ -1635,7 +1662,7 @@ __global__ void __launch_bounds__(ORDER_NT) k_irt_order(const uint32_t *tstate, 
     for (uint32_t i = threadIdx.x; i < len; i += ORDER_NT) order[base + i] = (uint32_t)key[i];
 }
 
-template <bool DEATH>
+template <bool DEATH, bool LDS_BITS>
 __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
 {
     const IrtConst &C = c_irt;
@@ -1644,10 +1671,12 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     const IrtTrack T = A.tracks[li];
     const uint32_t lane = threadIdx.x;
     float4 *P = A.pos + T.part_off;
-    uint8_t *al = A.alive + T.part_off;
+    uint8_t *const al_global = A.alive + T.part_off;
     double *tb = A.tb + T.part_off;
     const uint64_t tk = dna_track_key(A.seed_ptr ? *A.seed_ptr : A.seed, T.inst, T.pid);
     const uint32_t *st = A.tstate + (size_t)TSTATE * li;
+    AliveBits<LDS_BITS> al;
+    al.g = (uint32_t *)(A.sorted + T.part_off);
     const unsigned long long tp0 = wall_clock64();
     if (lane == 0) {
         sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
@@ -1684,14 +1713,16 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     for (uint32_t i = lane; i < DNA_MAX_CKPT * 5; i += 64u) sh.snap[i] = 0u;
     for (uint32_t j = lane; j < BM_MAX; j += 64u) { sh.bm_t[j] = 1e300; sh.bm_ab[j] = ~0ull; }
     __syncthreads();
-    // alive bytes in LDS when the track's slots fit (the loop reads them at random)
-    uint8_t *const al_global = al;
-    if (T.cap <= A.al_lds) {
-        uint8_t *la = (uint8_t *)dyn_lds;
-        const uint32_t nt0 = sh.n_total;
-        for (uint32_t k = lane; k < T.cap; k += 64u) la[k] = k < nt0 ? al[k] : 0;
+    // the alive bits from the pairs kernel's alive bytes
+    {
+        const uint32_t nt0 = sh.n_total, nw = alive_words(T.cap);
+        uint32_t *aw = al.w();
+        for (uint32_t c0 = 0; 2u * (c0 / 64u) < nw; c0 += 64u) {
+            const uint32_t k = c0 + lane;
+            const uint64_t m = __ballot(k < nt0 && al_global[k] != 0);
+            if (lane == 0) { aw[c0 / 32u] = (uint32_t)m; aw[c0 / 32u + 1u] = (uint32_t)(m >> 32); }
+        }
         __syncthreads();
-        al = la;
     }
     // 1. the queued initial pairs (unless k_irt_pairs drained them): the rest of sampleIRT
     // (irt-worker.js:116-184) in FP64, then every event into the block minima
@@ -1747,9 +1778,10 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             sh.nt_n = 0; sh.e0 = sh.n_ev;
             while (cp < A.nc && t0 >= A.tc[cp]) { for (int s = 0; s < 5; s++) sh.snap[cp * 5 + s] = (uint32_t)sh.cnt[s]; cp++; }
             uint32_t a = (uint32_t)(ab0 >> 32), b = (uint32_t)ab0;
-            int si = (int)al[a] - 1, sj = (int)al[b] - 1;
+            const float4 pa = P[a], pb = P[b];
+            int si = (int)pa.w, sj = (int)pb.w;
             int ri = sh.rmap[(si < sj ? si : sj) * 6 + (si < sj ? sj : si)];
-            al[a] = 0; al[b] = 0;
+            al.clear(a); al.clear(b);
             if (DEATH) { A.tdeath[T.part_off + a] = t0; A.tdeath[T.part_off + b] = t0; }
             sh.ldead[si]++; sh.ldead[sj]++;
             sh.n_re++; sh.rxc[ri]++;
@@ -1758,7 +1790,6 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
             int np = sh.nprod[ri];
             if (np > 0) {
-                float4 pa = P[a], pb = P[b];
                 double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
                 double wsum = sqDi + sqDj;
                 double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
@@ -1775,7 +1806,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                     if (np == 1) q = make_float4((float)mx, (float)my, (float)mz, (float)ps);
                     else if (pp == 0) q = make_float4(pa.x, pa.y, pa.z, (float)ps);
                     else q = make_float4(pb.x, pb.y, pb.z, (float)ps);
-                    P[id] = q; al[id] = (uint8_t)(ps + 1); tb[id] = t0;
+                    P[id] = q; al.set(id); tb[id] = t0;
                     if (DEATH) A.tdeath[T.part_off + id] = __longlong_as_double(0x7FF0000000000000ll);
                     const size_t slot = 6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++;
                     A.lst[slot] = id;
@@ -1844,7 +1875,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     const unsigned long long tp2 = wall_clock64();
     if (A.final_ntot) {                       // the final state for dna_irt_oh_final
         const uint32_t nt = sh.n_total;
-        if (al != al_global) for (uint32_t k = lane; k < nt; k += 64u) al_global[k] = al[k];
+        for (uint32_t k = lane; k < nt; k += 64u) al_global[k] = al[k] ? (uint8_t)((uint32_t)P[k].w + 1u) : (uint8_t)0;
         if (lane == 0) A.final_ntot[T.idx] = nt;
     }
     if (lane == 0) {
@@ -2258,11 +2289,13 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     // memory above; the drain's long list reuses that LDS afterwards (at least 128 slots:
     // 512 list entries, twice the largest workgroup, as the small-queue drain needs)
     A.lds_slots = std::max<uint32_t>(std::min<uint32_t>(G.maxn_all, 3072u), 128u);
-    // the loop kernel keeps a track's alive flags in LDS when its slots fit in AL_LDS_MAX
-    // bytes (16 one-wavefront workgroups per CU with LoopShared)
+    // the loop kernel keeps its track's alive bits in LDS when the largest track's fit in
+    // AL_LDS_MAX bytes (16 one-wavefront workgroups per CU with LoopShared), else (al_lds 0)
+    // in global memory
     uint32_t capmax = 0;
     for (auto &t : G.tr) capmax = t.cap > capmax ? t.cap : capmax;
-    A.al_lds = (std::min<uint32_t>(capmax, AL_LDS_MAX) + 15u) & ~15u;
+    const uint32_t alb = 4u * alive_words(capmax);
+    A.al_lds = alb <= AL_LDS_MAX ? ((alb + 15u) & ~15u) : 0u;
     A.tstate = (uint32_t *)(b.events + evs * 32);
     A.order = A.tstate + (size_t)G.tr.size() * TSTATE;
     return A;
@@ -2278,9 +2311,16 @@ static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int b
     if (A.drain_at == 2) hipLaunchKernelGGL(k_irt_drain, dim3(grid), dim3(DRAIN_NT), 0, st, A);
     if (A.order) hipLaunchKernelGGL(k_irt_order, dim3((grid + ORDER_CHUNK - 1) / ORDER_CHUNK), dim3(ORDER_NT), 0,
                                     st, (const uint32_t *)A.tstate, grid, A.order);
-    // the death-time stores are compiled only into the variants that keep the final state
-    if (A.tdeath) hipLaunchKernelGGL((k_irt_loop<true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
-    else hipLaunchKernelGGL((k_irt_loop<false>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+    // the death-time stores are compiled only into the variants that keep the final state;
+    // alive bits in LDS when the largest track's fit in al_lds bytes, else in global memory
+    const bool lb = A.al_lds > 0;
+    if (A.tdeath) {
+        if (lb) hipLaunchKernelGGL((k_irt_loop<true, true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+        else hipLaunchKernelGGL((k_irt_loop<true, false>), dim3(grid), dim3(64), 0, st, A);
+    } else {
+        if (lb) hipLaunchKernelGGL((k_irt_loop<false, true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
+        else hipLaunchKernelGGL((k_irt_loop<false, false>), dim3(grid), dim3(64), 0, st, A);
+    }
 }
 
 static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
