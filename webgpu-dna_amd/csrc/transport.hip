@@ -249,10 +249,13 @@ struct Sink {
 // the canonical scatter.  Work items come in chunks of d.work_chunk (>= 64).  Both sizes
 // are launch options (autotuned by the host); results do not depend on them.
 // Different lanes lead successive calls, so every access goes through a volatile view:
-// no value may be kept in a register from one call to the next.
+// no value may be kept in a register from one call to the next.  The view is typed as LDS:
+// through a generic volatile pointer the accesses compile to flat instructions, and each of
+// those waits for every global load and store of the wavefront still in flight.
 __shared__ uint32_t s_wrad_[12], s_wwork_[12];
-#define s_wrad ((volatile uint32_t *)s_wrad_)
-#define s_wwork ((volatile uint32_t *)s_wwork_)
+typedef __attribute__((address_space(3))) volatile uint32_t lds_vu32;
+#define s_wrad ((lds_vu32 *)s_wrad_)
+#define s_wwork ((lds_vu32 *)s_wwork_)
 
 __device__ __forceinline__ void wave_state_init(uint32_t rad_chunk, uint32_t work_chunk)
 {
