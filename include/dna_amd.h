@@ -287,9 +287,9 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * layer.  Both give the same values. */
 #define DNA_OPT_MLP_FUSED       12
 /* DNA_OPT_IRT_DRAIN: where the IRT samples its initial pairs' queue (the rest of sampleIRT,
- * SamplePDC's rejection loops included): 0 (default) in the one-wavefront event-loop kernel
- * before its loop, 1 in the pairs kernel's workgroup after the enumeration, 2 in a kernel of
- * its own between the two.  Results do not depend on it. */
+ * SamplePDC's rejection loops included): 0 in the one-wavefront event-loop kernel before
+ * its loop, 1 (default) in the pairs kernel's workgroup after the enumeration, 2 in a kernel
+ * of its own between the two.  Results do not depend on it. */
 #define DNA_OPT_IRT_DRAIN       13
 /* DNA_OPT_IRT_ORDER: 1 (default) runs the IRT's event-loop workgroups heaviest track first
  * (queued initial pairs + initial particles, within runs of 4096 tracks), 0 in launch

@@ -156,7 +156,7 @@ def test_drain_placement_keeps_the_results(ctx, rad20k, drain):
         g = ctx.irt(rad20k, None, 24, 20000.0, CK, seed=13)
         assert ctx.get_option(DNA_OPT_IRT_DRAIN) == drain
     finally:
-        ctx.set_option(DNA_OPT_IRT_DRAIN, 0)
+        ctx.set_option(DNA_OPT_IRT_DRAIN, 1)
     _same(g, o)
 
 
@@ -166,7 +166,7 @@ def test_drain_option_is_checked(ctx):
     for bad in (-1, 3):
         with pytest.raises(DnaError):
             ctx.set_option(DNA_OPT_IRT_DRAIN, bad)
-    assert ctx.get_option(DNA_OPT_IRT_DRAIN) == 0
+    assert ctx.get_option(DNA_OPT_IRT_DRAIN) == 1
 
 
 @pytest.mark.parametrize("order", [0, 1])

@@ -72,7 +72,7 @@ struct dna_ctx {
     // class; cells: -1 = none, m = cells of R_CUT / m)
     int irt_block = 0;
     int irt_cell = 0;
-    int irt_drain = 0;                 // DNA_OPT_IRT_DRAIN: where the initial pairs are drained
+    int irt_drain = 1;                 // DNA_OPT_IRT_DRAIN: where the initial pairs are drained
     int irt_order = 1;                 // DNA_OPT_IRT_ORDER: loop workgroups heaviest track first
     int irt_last_block = 0;            // workgroup size of the last IRT launch
     int irt_last_cell = 0;             // cells of the last IRT launch (-1 = none)
