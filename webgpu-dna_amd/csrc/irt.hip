@@ -1749,10 +1749,12 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     for (;;) {
         const unsigned long long q0 = wall_clock64();
         if (lane == 0) sh.acc[4]++;
-        // the next event: the least block minimum whose reactants are both alive; a stale
-        // minimum is replaced by its block's rescan
+        // the next event: the least block minimum whose reactants are both alive.  Each
+        // reaction refreshes the minima it makes stale (below), so the first candidate is
+        // live; a stale one would be replaced by its block's rescan
         double t0;
         uint64_t ab0;
+        uint32_t bjw = 0;
         for (;;) {
             double bt = 1e300;
             uint64_t bab = ~0ull;
@@ -1763,15 +1765,23 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                 if (ev_less(t, k, bt, bab)) { bt = t; bab = k; bj = j; }
             }
             bj = __shfl(bj, (int)wave_min_ev(bt, bab), 64);
-            t0 = bt; ab0 = bab;
+            t0 = bt; ab0 = bab; bjw = bj;
             if (!(bt < 1e300)) break;
             if (al[(uint32_t)(bab >> 32)] && al[(uint32_t)bab]) break;
             block_min(A, T, sh, al, bj);
             __syncthreads();
         }
+        if (!(t0 < 1e300)) break;
+        // The reaction takes the minimum of block bjw: that block's first 64 events are
+        // loaded now, while lane 0 runs the reaction, and reduced right after it
+        const uint32_t bs_w = sh.bs, ne_w = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
+        const uint32_t end_w = (bjw + 1u) * bs_w < ne_w ? (bjw + 1u) * bs_w : ne_w;
+        const uint32_t ew = bjw * bs_w + lane;
+        uint2 pab = make_uint2(0u, 0u);
+        double pt = 1e300;
+        if (ew < end_w) { pab = A.ev_ab[T.ev_off + ew]; pt = A.ev_t[T.ev_off + ew]; }
         const unsigned long long q1 = wall_clock64();
         if (lane == 0) sh.acc[0] += q1 - q0;
-        if (!(t0 < 1e300)) break;
         if (lane == 0) {
             sh.win_t = t0; sh.win_ab = ab0; sh.np = 0;
             sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
@@ -1830,6 +1840,40 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             sh.lcomp = lc;
         }
         __syncthreads();
+        // the block minima this reaction made stale: block bjw (its minimum was the reacting
+        // pair), and any other block whose minimum has either reactant, so the next
+        // selection finds live minima
+        {
+            double bt = 1e300;
+            uint64_t bab = ~0ull;
+            if (ew < end_w && al[pab.x] && al[pab.y]) { bt = pt; bab = ((uint64_t)pab.x << 32) | pab.y; }
+            for (uint32_t e = ew + 64u; e < end_w; e += 64u) {
+                const uint2 q = A.ev_ab[T.ev_off + e];
+                const double t = A.ev_t[T.ev_off + e];
+                if (!al[q.x] || !al[q.y]) continue;
+                const uint64_t k = ((uint64_t)q.x << 32) | q.y;
+                if (ev_less(t, k, bt, bab)) { bt = t; bab = k; }
+            }
+            wave_min_ev(bt, bab);
+            if (lane == 0) { sh.bm_t[bjw] = bt; sh.bm_ab[bjw] = bab; }
+            const uint32_t ra = (uint32_t)(ab0 >> 32), rb = (uint32_t)ab0;
+            for (uint32_t j0 = 0; j0 < sh.nblk; j0 += 64u) {
+                const uint32_t j = j0 + lane;
+                bool stale = false;
+                if (j < sh.nblk && j != bjw) {
+                    const uint64_t k = sh.bm_ab[j];
+                    const uint32_t ka = (uint32_t)(k >> 32), kb = (uint32_t)k;
+                    stale = ka == ra || ka == rb || kb == ra || kb == rb;
+                }
+                uint64_t m = __ballot(stale);
+                while (m) {
+                    const uint32_t jj = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+                    block_min(A, T, sh, al, j0 + jj);
+                    m &= m - 1ull;
+                }
+            }
+            __syncthreads();
+        }
         const unsigned long long q2 = wall_clock64();
         if (lane == 0) sh.acc[1] += q2 - q1;
         if (sh.np) {
