@@ -171,7 +171,7 @@ def test_irt_interleaved_large_primary(ctx):
 
 def test_irt_track_beyond_lds_alive_bits(ctx):
     """A 15,000-entry primary beside a small one: its 30,512 particle slots need 3,816 bytes
-    of alive bits, more than the loop kernel's LDS budget (3,584), so that launch keeps the
+    of alive bits, more than the loop kernel's LDS budget (2,048), so that launch keeps the
     bits in global memory; tallies still equal the oracle's."""
     rng = np.random.default_rng(22)
 
@@ -183,7 +183,7 @@ def test_irt_track_beyond_lds_alive_bits(ctx):
         r[:, 3] = pid * 8 + rng.choice([0, 0, 1, 1, 2, 3, 6], n)
         return r
     rad = np.concatenate([blob(2, 15000, 600.0), blob(5, 400, 60.0)])
-    assert 4 * 2 * ((2 * 15000 + 512 + 63) // 64) > 3584
+    assert 4 * 2 * ((2 * 15000 + 512 + 63) // 64) > 2048
     print("irt: a 15000-entry primary (alive bits in global memory)", flush=True)
     g = ctx.irt(rad, rad.shape[0], 2, 10000.0, CK, seed=9)
     o = O.irt(rad, 2, 10000.0, CK, seed=9, n_threads=4)

@@ -59,9 +59,12 @@ struct IrtArgs {
     const IrtTrack *tracks;
     float4 *pos;                  // (x, y, z, species) per particle slot
     uint8_t *alive;               // per particle slot: species + 1 while alive, 0 once reacted
-    uint32_t *lst;                // 6 species lists of capacity cap per track (may hold reacted particles)
-    float4 *lpos;                 // beside each list entry: the particle's x, y, z and its birth time
-                                  // rounded down to FP32 (contiguous reads for the re-pairing screen)
+    // 6 species lists of capacity cap per track (may hold reacted particles): each entry is
+    // the particle's x, y, z and index (lpos; contiguous reads for the re-pairing screen) and
+    // its birth time rounded down to FP32, bits in lst (0 for the initial particles; read only
+    // for the appended part of a list)
+    uint32_t *lst;
+    float4 *lpos;
     double *tb;
     double *ev_t;
     uint2 *ev_ab;
@@ -391,7 +394,7 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 #define IRT_NT (blockDim.x)
 #define BM_MAX 128u                  // event blocks per track (a block holds 64 k events)
 #define STAGE_MAX 64u                // events staged per reaction (more: blocks rescanned)
-#define AL_LDS_MAX 3584u             // bytes of LDS for the loop kernel's alive bits (28,672 slots)
+#define AL_LDS_MAX 2048u             // bytes of LDS for the loop kernel's alive bits (16,384 slots)
 #define TSTATE 128                   // u32 per track handed from k_irt_pairs to k_irt_loop:
                                      // [0] queued pairs, [1..6] species list lengths,
                                      // [7..9] initial OH, e-aq, H, [10] H2 markers, [11] n,
@@ -496,9 +499,11 @@ struct LoopShared {
     // re-pairing segments (product p, partner species sj, a list range) flattened
     int      nseg;
     uint32_t seg_u0[25], seg_lo[24];      // first flattened index, first list entry
-    uint8_t  seg_ps[24];                  // p << 4 | sj
-    uint32_t rq[1][128];                  // re-pairings that passed the bound
-    uint32_t rqn[1];
+    uint8_t  seg_ps[24];                  // p << 4 | part << 3 | sj (part 1: appended entries)
+    // re-pairings that passed the bound: the partner's list entry (x, y, z) and
+    // p << 31 | sj << 28 | index
+    uint32_t rq[4][128];
+    uint32_t n_init;                      // particles 0 .. n_init - 1 are initial (born at 0)
     // event selection: blocks of bs events, the minimum live (t, a << 32 | b) of each block
     // (a minimum may have gone stale since), and the events pushed in the current reaction
     uint32_t bs, nblk, e0;
@@ -907,22 +912,22 @@ __device__ void pdc_drain(const IrtArgs &A, const IrtTrack &T, uint64_t tk, SH &
     }
 }
 
-// The exact part for one staged re-pairing (product p, partner jj): the partner's
-// Brownian step, r0, the first sampling stage, and the queueing of pairs that can react.
-template <class AL>
-__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al,
-                                             uint64_t tk, double t_new, uint32_t e)
+// The exact part for one staged re-pairing (product p, partner jj of species sj at the
+// list entry's position pj, the partner's own position): the partner's Brownian step, r0,
+// the first sampling stage, and the queueing of pairs that can react.  Only a partner born
+// in a reaction reads its birth time; the initial particles are born at 0.
+__device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, uint64_t tk,
+                                             double t_new, float4 pj, uint32_t e)
 {
-    const float4 *P = A.pos + T.part_off;
     const double *tb = A.tb + T.part_off;
-    const uint32_t p = e >> 31, jj = e & 0x7FFFFFFFu;
-    const float4 pi = sh.prod_pos[p], pj = P[jj];
-    const uint32_t sj = (uint32_t)pj.w, sp = sh.prod_sp[p], pc = sp * 6u + sj;
+    const uint32_t p = e >> 31, sj = (e >> 28) & 7u, jj = e & 0x0FFFFFFFu;
+    const float4 pi = sh.prod_pos[p];
+    const uint32_t sp = sh.prod_sp[p], pc = sp * 6u + sj;
     const int rj = sh.rmap[pc];
     const uint32_t idx = sh.prod_idx[p];
     const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
     const uint64_t ps = dna_pair_state(tk, a, b);
-    const double dt = t_new - tb[jj];
+    const double dt = t_new - (jj < sh.n_init ? 0.0 : tb[jj]);
     {
         // FP32 screen of the same step: each normal g = sqrt(-2 ln(u + 1e-30)) cos(6.2831853 u')
         // is within 2e-3 + 1e-4 |g| of the FP64 one (hardware log2/cos in revolutions: the
@@ -976,7 +981,7 @@ template <class AL>
 __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &sh, const AL &al,
                               uint64_t tk, double t_new)
 {
-    const uint32_t *lst = A.lst + 6 * T.part_off;
+    const float *lbr = (const float *)A.lst + 6 * T.part_off;
     const float4 *lps = A.lpos + 6 * T.part_off;
     const int ns = sh.nseg, np = sh.np;
     const uint32_t tot = sh.seg_u0[ns];
@@ -986,31 +991,35 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
     // written and read by different lanes between wave barriers: volatile, and typed as LDS
     // so that the accesses stay LDS instructions (a generic volatile pointer compiles to
     // flat accesses, which wait for every global load in flight)
-    lds_vu32 *q = (lds_vu32 *)sh.rq[w];
+    lds_vu32 *qx = (lds_vu32 *)sh.rq[0], *qy = (lds_vu32 *)sh.rq[1], *qz = (lds_vu32 *)sh.rq[2];
+    lds_vu32 *qe = (lds_vu32 *)sh.rq[3];
     uint32_t qn = 0;                                   // wavefront-uniform
     // The list entries of the next 64 flattened indices are loaded before the current 64
     // are screened, so their latency overlaps the screen.  A lane's index only grows, so its
     // segment is found by moving forward from the previous one.
     int sg = 0;
-    uint32_t jj_n = 0, ps_n = 0;
+    uint32_t ps_n = 0;
+    float bt_n = 0.f;
     float4 pj_n = make_float4(0.f, 0.f, 0.f, 0.f);
     auto fetch = [&](uint32_t u) {
         while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
         ps_n = sh.seg_ps[sg];
-        const size_t slot = (size_t)(ps_n & 15u) * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
-        jj_n = lst[slot];
-        pj_n = lps[slot];                              // x, y, z, birth time rounded down
+        const size_t slot = (size_t)(ps_n & 7u) * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
+        pj_n = lps[slot];                              // x, y, z, the particle index
+        bt_n = (ps_n & 8u) ? lbr[slot] : 0.f;          // birth time rounded down (appended part)
     };
     if (w * 64u + lane < tot) fetch(w * 64u + lane);
     for (uint32_t u0 = w * 64u; u0 < tot; u0 += IRT_NT) {
         const uint32_t u = u0 + lane;
-        const uint32_t jj = jj_n, psg = ps_n;
+        const uint32_t psg = ps_n;
         const float4 pj = pj_n;
+        const float bt = bt_n;
         if (u + IRT_NT < tot) fetch(u + IRT_NT);
+        const uint32_t jj = __float_as_uint(pj.w);
         bool keep = false;
         uint32_t ent = 0;
         if (u < tot) {
-            const uint32_t p = psg >> 4, sj = psg & 15u;
+            const uint32_t p = psg >> 4, sj = psg & 7u;
             const uint32_t idx = sh.prod_idx[p];
             if (al[jj] && jj != idx && !(p == 0 && np == 2 && jj == pidx1)) {
                 const uint32_t pc = sh.prod_sp[p] * 6u + sj;
@@ -1019,7 +1028,7 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
                 const uint64_t ps = dna_pair_state(tk, a, b);
                 // t_new - (birth time rounded down) >= the exact step time, so the bound's
                 // step is at least the true one: the screen stays conservative
-                const double dt = t_new - (double)pj.w;
+                const double dt = t_new - (double)bt;
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
                 const float r = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
                 const float sgm = dt > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt)) : 0.f;
@@ -1028,7 +1037,7 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
                 // partner beyond R_CUT by that much cannot pair, and the draw of the first
                 // sampling stage alone may already rule the pair out
                 const float rloose = r - sgm * 20.4f;
-                ent = (p << 31) | jj;
+                ent = (p << 31) | (sj << 28) | jj;
                 if (!(rloose > c_irt.R_CUT_F)) {
                     const float U = u01_lo(ps, 6u);
                     if (!cannot_react(sh.bnd[pc], rloose, U)) {
@@ -1049,29 +1058,30 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
             }
         }
         const uint64_t m = __ballot(keep);
-        if (keep) q[qn + (uint32_t)__popcll(m & below)] = ent;
+        if (keep) {
+            const uint32_t k = qn + (uint32_t)__popcll(m & below);
+            qx[k] = __float_as_uint(pj.x); qy[k] = __float_as_uint(pj.y); qz[k] = __float_as_uint(pj.z); qe[k] = ent;
+        }
         qn += (uint32_t)__popcll(m);
         __builtin_amdgcn_wave_barrier();
         if (qn >= 64u) {
-            repair_exact(A, T, sh, al, tk, t_new, q[lane]);
+            repair_exact(A, T, sh, tk, t_new,
+                         make_float4(__uint_as_float(qx[lane]), __uint_as_float(qy[lane]), __uint_as_float(qz[lane]), 0.f),
+                         qe[lane]);
             __builtin_amdgcn_wave_barrier();
-            if (lane < qn - 64u) q[lane] = q[64u + lane];
+            if (lane < qn - 64u) {
+                qx[lane] = qx[64u + lane]; qy[lane] = qy[64u + lane]; qz[lane] = qz[64u + lane]; qe[lane] = qe[64u + lane];
+            }
             qn -= 64u;
             __builtin_amdgcn_wave_barrier();
         }
     }
-    // the wavefronts' remainders (< 64 each) are finished together, 64 per wavefront, so a
-    // few survivors cost one pass instead of one per wavefront
-    if (lane == 0) sh.rqn[w] = qn;
+    // the remainder (< 64)
+    if (lane < qn)
+        repair_exact(A, T, sh, tk, t_new,
+                     make_float4(__uint_as_float(qx[lane]), __uint_as_float(qy[lane]), __uint_as_float(qz[lane]), 0.f),
+                     qe[lane]);
     __syncthreads();
-    const uint32_t nw = IRT_NT / 64u;
-    uint32_t tot_r = 0;
-    for (uint32_t k = 0; k < nw; k++) tot_r += sh.rqn[k];
-    for (uint32_t v = w * 64u + lane; v < tot_r; v += IRT_NT) {
-        uint32_t k = 0, off = v;
-        while (off >= sh.rqn[k]) { off -= sh.rqn[k]; k++; }
-        repair_exact(A, T, sh, al, tk, t_new, sh.rq[k][off]);
-    }
 }
 
 // Drop reacted entries from the species lists flagged in sh.lcomp (the loop kernel's one
@@ -1100,11 +1110,11 @@ __device__ void compact_lists(const IrtArgs &A, const IrtTrack &T, LoopShared &s
             for (uint32_t c0 = i0; c0 < i1; c0 += 64u) {
                 const uint32_t i = c0 + lane;
                 const bool in = i < i1;
-                const uint32_t k = in ? L[i] : 0u;
+                const uint32_t bt = in ? L[i] : 0u;                 // birth time bits
                 const float4 q = in ? LP[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                const bool live = in && al[k];
+                const bool live = in && al[__float_as_uint(q.w)];
                 const uint64_t m = __ballot(live);
-                if (live) { const uint32_t d = dst + (uint32_t)__popcll(m & below); L[d] = k; LP[d] = q; }
+                if (live) { const uint32_t d = dst + (uint32_t)__popcll(m & below); L[d] = bt; LP[d] = q; }
                 dst += (uint32_t)__popcll(m);
             }
             __builtin_amdgcn_wave_barrier();
@@ -1174,7 +1184,7 @@ __device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
     const uint64_t m = __ballot(len > 0);
     if (len > 0) {
         const uint32_t k = (uint32_t)__popcll(m & below);
-        sh.seg_u0[k] = x - len; sh.seg_lo[k] = lo; sh.seg_ps[k] = (uint8_t)((p << 4) | sj);
+        sh.seg_u0[k] = x - len; sh.seg_lo[k] = lo; sh.seg_ps[k] = (uint8_t)((p << 4) | (part << 3) | sj);
     }
     uint32_t na = nalive;
     for (int o = 32; o > 0; o >>= 1) na += __shfl_xor(na, o, 64);
@@ -1508,8 +1518,8 @@ __global__ void __launch_bounds__(IRT_BLOCK, 4) k_irt_pairs(IrtArgs A)
         if (g < sh.goff[spc] + sh.gcnt[spc]) {
             const float4 q = Sg[g];
             const size_t slot = 6 * T.part_off + (size_t)spc * T.cap + (g - sh.goff[spc]);
-            A.lst[slot] = (uint32_t)__float_as_int(q.w);
-            A.lpos[slot] = make_float4(q.x, q.y, q.z, 0.0f);
+            A.lst[slot] = 0u;                          // born at 0
+            A.lpos[slot] = q;                          // x, y, z, the particle index
         }
     }
     __syncthreads();
@@ -1680,7 +1690,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     const unsigned long long tp0 = wall_clock64();
     if (lane == 0) {
         sh.npend = 0; sh.pend_next = 0; sh.long_n = 0; sh.long_next = 0;
-        sh.n_ev = st[21]; sh.n_total = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
+        sh.n_ev = st[21]; sh.n_total = st[11]; sh.n_init = st[11]; sh.n_re = 0; sh.nt_n = 0; sh.lcomp = 0;
         for (int s = 0; s < 6; s++) { sh.llen[s] = st[1 + s]; sh.ldead[s] = 0; }
         sh.win = st[12];
         for (int s = 0; s < 6; s++) sh.n0[s] = sh.win ? st[1 + s] : 0u;
@@ -1819,8 +1829,8 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                     P[id] = q; al.set(id); tb[id] = t0;
                     if (DEATH) A.tdeath[T.part_off + id] = __longlong_as_double(0x7FF0000000000000ll);
                     const size_t slot = 6 * T.part_off + (size_t)ps * T.cap + sh.llen[ps]++;
-                    A.lst[slot] = id;
-                    A.lpos[slot] = make_float4(q.x, q.y, q.z, __double2float_rd(t0));
+                    A.lst[slot] = __float_as_uint(__double2float_rd(t0));
+                    A.lpos[slot] = make_float4(q.x, q.y, q.z, __uint_as_float(id));
                     if (ps == 0) sh.cnt[0]++; else if (ps == 1) sh.cnt[1]++; else if (ps == 2) sh.cnt[2]++; else if (ps == 4) sh.cnt[3]++;
                     sh.prod_idx[made] = id; sh.prod_sp[made] = (uint32_t)ps; sh.prod_pos[made] = q;
                     made++;
