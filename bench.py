@@ -61,10 +61,12 @@ def chain_step_surrogate(ctx, p, dna, surr, sigma_max, seed):
     return t, r, s
 
 
-def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
+def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log, irt_chain_value, chains=4):
     """Learned chemistry: 200k one-primary tracks (E log-uniform 1-20 keV) from this GPU
     code, 95% for training on the GPU and 5% held out; then a 65,536-track inference
-    batch timed against plain IRT, and the 10 keV chain with surrogate chemistry."""
+    batch timed against plain IRT, and the 10 keV chain with surrogate chemistry with
+    `chains` chains in flight (its transport is latency-bound, so it gains from more chains
+    than the GPU-bound IRT chain), compared with the IRT chain's value."""
     import numpy as np
     from dna_amd import Surrogate
     from dna_amd import surrogate as S
@@ -138,9 +140,12 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
                         "g_err_mean": round(err["mean_rel"], 4),
                         "network_bf16_gflop": round(flops / 1e9, 2)}
     log(f"surrogate: inference {out['inference']}")
-    # the 10 keV chain with surrogate chemistry, as many chains in flight as the IRT chain
-    # (each context gets its own copy of the trained network: the activations are per copy)
+    # the 10 keV chain with surrogate chemistry (each context gets its own copy of the trained
+    # network: the activations are per copy)
     from concurrent.futures import ThreadPoolExecutor
+    from dna_amd import Context
+    extra = [Context(0) for _ in range(max(0, chains - len(ctxs)))]
+    ctxs = list(ctxs)[:max(1, chains)] + extra
     K = len(ctxs)
     nets = [s] + [Surrogate(c, 70, 256, 3, 80, seed=1) for c in ctxs[1:]]
     for n in nets[1:]:
@@ -161,12 +166,15 @@ def surrogate_leg(ctxs, dna, p10, G_irt_1us, steps, log):
     rel = [abs(G1[i] - G_irt_1us[i]) / G_irt_1us[i] for i in range(5) if G_irt_1us[i] >= 0.05]
     out["chain_10keV"] = {"primaries_per_s": round(N_PRIM * steps / wall, 2),
                           "ms_per_step": round(1e3 * wall / steps, 3), "chains_in_flight": K,
+                          "vs_irt_chain": round(N_PRIM * steps / wall / irt_chain_value, 2),
                           "fallback": rr["n_fallback"],
                           "G_1us": [round(float(x), 4) for x in G1],
                           "g_err_1us_vs_irt_chain": round(max(rel), 4)}
     log(f"surrogate: chain {out['chain_10keV']}")
     for n in nets:
         n.close()
+    for c in extra:
+        c.close()
     return out
 
 
@@ -241,6 +249,9 @@ def main():
                     help="primaries of the single-thread CPU run (the reference worker is single-threaded)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--surrogate-inflight", type=int, default=4,
+                    help="chains in flight for the 10 keV chain with surrogate chemistry (4 = the "
+                         "hardware queues a process gets by default; more streams share queues)")
     ap.add_argument("--inflight", type=int, default=2,
                     help="chain steps in flight: independent batches on that many contexts (each its "
                          "own HIP stream and buffers), driven by as many host threads, so one batch's "
@@ -369,7 +380,7 @@ def main():
         def log(m):
             print(m, file=sys.stderr, flush=True)
         G1 = [c.G[6][s] for s in range(5)]
-        out["surrogate"] = surrogate_leg(ctxs, dna, p, G1, args.steps, log)
+        out["surrogate"] = surrogate_leg(ctxs, dna, p, G1, args.steps, log, value, args.surrogate_inflight)
         out["surrogate_speedup"] = out["surrogate"]["inference"]["speedup"]
     if not args.no_cpu:
         threads, host = host_cores()
