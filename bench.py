@@ -40,11 +40,21 @@ def measured_traffic(kernel):
     return None
 
 
+# With several chains in flight, one IRT at a time (set by main): each chain's transport and
+# scoring then run beside another chain's IRT, and two IRTs never contend for the wave slots
+# (the event-loop kernel fills every CU's slots with its own one-wavefront tracks).
+IRT_GATE = None
+
+
 def chain_step(ctx, p, dna, seed):
     from dna_amd.abi import DNA_COPY_RESULTS
     t = ctx.transport(p, copy=DNA_COPY_RESULTS)
     n_therm = int((t["results"]["esc"] == 0).sum())
-    c = ctx.irt(None, t["rad_n"], n_therm, E_EV, CK, seed=seed)
+    if IRT_GATE is not None:
+        with IRT_GATE:
+            c = ctx.irt(None, t["rad_n"], n_therm, E_EV, CK, seed=seed)
+    else:
+        c = ctx.irt(None, t["rad_n"], n_therm, E_EV, CK, seed=seed)
     s = ctx.score(dna, None, t["rad_n"], lcg_state=0x12345678)
     return t, c, s, n_therm
 
@@ -256,6 +266,9 @@ def main():
                     help="chain steps in flight: independent batches on that many contexts (each its "
                          "own HIP stream and buffers), driven by as many host threads, so one batch's "
                          "latency-bound transport overlaps another's IRT")
+    ap.add_argument("--irt-gate", type=int, default=1, choices=(0, 1),
+                    help="1: one chain's IRT at a time, the other chains' transport and scoring "
+                         "beside it; 0: the chains run free")
     ap.add_argument("--irt-drain", type=int, default=-1, choices=(-1, 0, 1, 2),
                     help="where the IRT drains its initial pairs' queue (DNA_OPT_IRT_DRAIN): 0 the "
                          "event-loop kernel, 1 the pairs kernel, 2 a kernel of its own; -1 the "
@@ -285,6 +298,10 @@ def main():
     from concurrent.futures import ThreadPoolExecutor
     from dna_amd import Context, DNATarget, default_params
     K = max(1, args.inflight)
+    global IRT_GATE
+    if args.irt_gate and K > 1:
+        import threading
+        IRT_GATE = threading.Lock()
     ctxs = [Context(0) for _ in range(K)]
     if args.irt_drain >= 0:
         from dna_amd.abi import DNA_OPT_IRT_DRAIN
@@ -366,7 +383,8 @@ def main():
         "data": "synthetic (reference geometry and seeding, no external data)",
         "config": {"workload": "full reference chain: 4096 primaries at 10 keV, transport + prechemistry, "
                                "IRT to 1 us (FP64), SSB/DSB on the 21x21 B-DNA fiber grid",
-                   "primaries": N_PRIM, "energy_eV": E_EV, "chains_in_flight": K},
+                   "primaries": N_PRIM, "energy_eV": E_EV, "chains_in_flight": K,
+                   "irt_gate": IRT_GATE is not None},
         "roofline": roof,
         "breakdown_ms": {"phase_a": round(sum(k_prim) / len(k_prim), 3),
                          "phase_b": round(sum(k_sec) / len(k_sec), 3),
