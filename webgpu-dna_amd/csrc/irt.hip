@@ -215,11 +215,12 @@ __device__ __forceinline__ bool cannot_react(const RxB &b, float rmin, float U)
     return a * U * 0.99999f > b.S;
 }
 
-// Lower bound of draw n, dna_u01(ps, n) = (x >> 11) 2^-53: its top 24 bits, exact in FP32.
+// Lower bound of draw n, dna_u01(ps, n) = (x >> 11) 2^-53: its top 23 bits, taken as the FP32
+// 1.m - 1 with m those bits (exact, and no integer-to-float conversion).
 __device__ __forceinline__ float u01_lo(uint64_t ps, uint32_t n)
 {
     const uint64_t x = dna_mix64(ps + DNA_GOLDEN * (uint64_t)(n + 1u));
-    return (float)(uint32_t)(x >> 40) * 5.9604644775390625e-8f;
+    return __uint_as_float(0x3F800000u | ((uint32_t)(x >> 32) >> 9)) - 1.0f;
 }
 
 // Square root with the hardware instruction (1 ulp); only used inside bounds whose margins
@@ -1258,7 +1259,7 @@ __device__ __forceinline__ CellWin cell_window(const PairShared &sh, uint32_t sa
 // FP32 bound decides it: the FP32 squared distance is within 3.1e-7 (relative) of the FP64
 // one (five roundings of non-negative terms), so r2 above R_CUT^2 (1 + 1e-5) or below
 // R_CUT^2 (1 - 1e-5) in FP32 settles the cutoff test, and the bound of cannot_react takes
-// sqrt(r2) (1 - 1e-5) and the draw's 24-bit lower bound.  The exact FP64 distance is
+// sqrt(r2) (1 - 1e-5) and the draw's 23-bit lower bound.  The exact FP64 distance is
 // computed near the cutoff and for the pairs that are queued.  The cells leave out only
 // pairs farther apart than R_CUT (see the hash in k_irt_pairs), so the queued pairs are
 // those of the full enumeration.
