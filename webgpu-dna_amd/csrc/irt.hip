@@ -414,24 +414,56 @@ __device__ __forceinline__ bool ev_less(double t1, uint64_t ab1, double t2, uint
 // are non-negative, so their bit patterns order like the values: one 64-bit butterfly
 // finds the least time, and only lanes sharing it (a tie, e.g. contact reactions at the
 // same instant) take a second butterfly on ab.
+// Cross-lane steps of the wavefront (all 64 lanes active) with DPP moves and lane reads:
+// no LDS round trip per step, unlike shuffles.
+// A 64-bit value of the lane given by a DPP control (the lane's own where the pattern has no
+// source).
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x)
+{
+    const int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+    const uint32_t l = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    const uint32_t h = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return ((uint64_t)h << 32) | l;
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, int lane)
+{
+    const uint32_t l = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, lane);
+    const uint32_t h = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), lane);
+    return ((uint64_t)h << 32) | l;
+}
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+// the least value over the wavefront, in every lane: minima within quads, half-rows and rows
+// of 16 lanes by DPP, then over the four rows
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t x)
+{
+    x = umin64(x, dpp_u64<0xB1>(x));            // quad_perm [1, 0, 3, 2]
+    x = umin64(x, dpp_u64<0x4E>(x));            // quad_perm [2, 3, 0, 1]
+    x = umin64(x, dpp_u64<0x141>(x));           // row_half_mirror
+    x = umin64(x, dpp_u64<0x140>(x));           // row_mirror
+    return umin64(umin64(readlane_u64(x, 0), readlane_u64(x, 16)), umin64(readlane_u64(x, 32), readlane_u64(x, 48)));
+}
+// inclusive prefix sum over the wavefront's lanes: within rows of 16 by shifted DPP adds
+// (lanes without a source add 0), then row 0's and rows 0-1's totals into the rows above
+__device__ __forceinline__ uint32_t wave_scan_u32(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true);    // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true);    // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true);    // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true);    // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);   // row_bcast:15 -> rows 1, 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);   // row_bcast:31 -> rows 2, 3
+    return x;
+}
+
 __device__ __forceinline__ uint32_t wave_min_ev(double &t, uint64_t &ab)
 {
-    uint64_t tk = (uint64_t)__double_as_longlong(t);
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t ok = __shfl_xor(tk, o, 64);
-        tk = ok < tk ? ok : tk;
-    }
+    const uint64_t tk = wave_min_u64((uint64_t)__double_as_longlong(t));
     const bool at = (uint64_t)__double_as_longlong(t) == tk;
     const uint64_t m = __ballot(at);
-    uint64_t k = at ? ab : ~0ull;
-    if (m & (m - 1ull)) {                       // a tie on t: least ab among the tied lanes
-        for (int o = 32; o > 0; o >>= 1) {
-            const uint64_t ok = __shfl_xor(k, o, 64);
-            k = ok < k ? ok : k;
-        }
-    } else {
-        k = __shfl(k, (int)__ffsll((long long)m) - 1, 64);
-    }
+    uint64_t k;
+    if (m & (m - 1ull)) k = wave_min_u64(at ? ab : ~0ull);  // a tie on t: least ab among the tied lanes
+    else k = readlane_u64(ab, (int)__ffsll((long long)m) - 1);
     const uint32_t src = (uint32_t)__ffsll((long long)__ballot(at && ab == k)) - 1u;
     t = __longlong_as_double((long long)tk);
     ab = k;
@@ -854,7 +886,7 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
     for (;;) {
         uint32_t slot = 0;
         if (lane == 0) slot = atomicAdd(use_long ? &sh.long_next : &sh.pend_next, 1u);
-        slot = __shfl(slot, 0, 64);
+        slot = (uint32_t)__builtin_amdgcn_readlane((int)slot, 0);
         if (slot >= (use_long ? nlong : npend)) break;
         uint32_t item = use_long ? sh.long_item[slot] : slot;
         PendPair pp = load_pending(A, T, sh, tk, item);
@@ -874,7 +906,11 @@ __device__ void pdc_drain_waves(const IrtArgs &A, const IrtTrack &T, uint64_t tk
             bool acc = false;
             if (trial < 10000u) acc = pdc_trial(c, pp.ps, trial, Xl);
             unsigned long long m = __ballot(acc);
-            if (m) { int first = __ffsll((long long)m) - 1; X = __shfl(Xl, first, 64); break; }
+            if (m) {
+                const int first = __ffsll((long long)m) - 1;
+                X = __longlong_as_double((long long)readlane_u64((uint64_t)__double_as_longlong(Xl), first));
+                break;
+            }
         }
         if (lane == 0 && X > 0) { double ta = t_base + X / pp.D; if (ta < DNA_T_END_NS) push_event(A, T, sh, ta, pp.a, pp.b); }
     }
@@ -1177,19 +1213,14 @@ __device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
             }
         }
     }
-    uint32_t x = len;
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= (uint32_t)o) x += y;
-    }
+    const uint32_t x = wave_scan_u32(len);
     const uint64_t m = __ballot(len > 0);
     if (len > 0) {
         const uint32_t k = (uint32_t)__popcll(m & below);
         sh.seg_u0[k] = x - len; sh.seg_lo[k] = lo; sh.seg_ps[k] = (uint8_t)((p << 4) | (part << 3) | sj);
     }
-    uint32_t na = nalive;
-    for (int o = 32; o > 0; o >>= 1) na += __shfl_xor(na, o, 64);
-    const uint32_t total = __shfl(x, 63, 64);
+    const uint32_t na = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32(nalive), 63);
+    const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
     if (lane == 0) {
         const int ns = __popcll(m);
         sh.nseg = ns; sh.seg_u0[ns] = total;
@@ -1775,7 +1806,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                 const uint64_t k = sh.bm_ab[j];
                 if (ev_less(t, k, bt, bab)) { bt = t; bab = k; bj = j; }
             }
-            bj = __shfl(bj, (int)wave_min_ev(bt, bab), 64);
+            bj = (uint32_t)__builtin_amdgcn_readlane((int)bj, (int)wave_min_ev(bt, bab));
             t0 = bt; ab0 = bab; bjw = bj;
             if (!(bt < 1e300)) break;
             if (al[(uint32_t)(bab >> 32)] && al[(uint32_t)bab]) break;
