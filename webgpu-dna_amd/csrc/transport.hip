@@ -293,7 +293,7 @@ __device__ __forceinline__ void emit_n(Sink &k, int n, const float4 *e)
         base = s_wrad[3 * w] + f;
         s_wrad[3 * w + 1] = f + tot;
     }
-    base = __shfl(base, leader, 64) + rank;
+    base = (uint32_t)__builtin_amdgcn_readlane((int)base, leader) + rank;   // the leader's value
     for (int i = 0; i < n; i++) {
         uint32_t idx = base + (uint32_t)i;
         if (idx < k.rad_cap) { k.rad[idx] = e[i]; k.key[idx] = k.keybase | (uint64_t)(k.seq + i); }
@@ -322,7 +322,9 @@ __device__ __forceinline__ uint32_t wave_claim_chunked(uint32_t *ctr)
             s_wwork[3 * w + 1] = fresh + ch;
         }
     }
-    old = __shfl(old, leader, 64); avail = __shfl(avail, leader, 64); fresh = __shfl(fresh, leader, 64);
+    old = (uint32_t)__builtin_amdgcn_readlane((int)old, leader);
+    avail = (uint32_t)__builtin_amdgcn_readlane((int)avail, leader);
+    fresh = (uint32_t)__builtin_amdgcn_readlane((int)fresh, leader);
     return rank < avail ? old + rank : fresh + (rank - avail);
 }
 
@@ -334,7 +336,7 @@ __device__ __forceinline__ uint32_t wave_claim(uint32_t *ctr)
     const int leader = __ffsll((unsigned long long)m) - 1;
     uint32_t base = 0;
     if ((int)lane == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
-    return __shfl(base, leader, 64) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return (uint32_t)__builtin_amdgcn_readlane((int)base, leader) + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
 
 // The dose grid takes one atomic per voxel a lane leaves, not one per deposit: deposits
