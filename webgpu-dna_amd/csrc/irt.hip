@@ -1300,8 +1300,10 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
 {
     const float4 *Sg = SG_LDS ? (const float4 *)dyn_lds : A.sorted + T.part_off;
     // block loop outermost: the reaction and its constants are wavefront-uniform
-    // (readfirstlane -> scalar registers), so the per-pair path has no table loads
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    // (readfirstlane -> scalar registers), so the per-pair path has no table loads.  The
+    // wavefront index is a scalar too: the unit loop, its cell search and the unit's row and
+    // chunk then run on the scalar unit, not as per-lane copies on the VALU.
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
     const uint32_t nx = __builtin_amdgcn_readfirstlane(sh.nx);
     const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
