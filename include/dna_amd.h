@@ -191,6 +191,20 @@ int dna_run_irt(struct dna_ctx *ctx, const float *rad, uint64_t rad_n, uint32_t 
                 float E_eV, const double *t_ckpt, int n_ckpt, uint64_t seed,
                 dna_chem_out *out);
 
+/* dna_run_irt in two calls.  No counterpart in the reference (its worker runs one batch at
+ * a time): it is for callers that pipeline batches, grouping and preparing the next batch's
+ * IRT while the GPU still runs the current one's kernels.  dna_irt_stage does everything of
+ * dna_run_irt before its kernels (grouping of the entries into tracks, with host round trips
+ * for the counts; capacities; the track list on the device; the launch choice) on the
+ * context's stream; dna_irt_finish launches the kernels and fills *out as dna_run_irt does,
+ * with the same results for the same seed (t_ms counts from dna_irt_stage).  One IRT can be
+ * staged per context; any other IRT call on the context, and destroying it, drop it.  With
+ * rad == NULL, a dna_run_transport between the two calls makes dna_irt_finish fail
+ * (DNA_ERR_ARG). */
+int dna_irt_stage(struct dna_ctx *ctx, const float *rad, uint64_t rad_n, uint32_t n_therm,
+                  float E_eV, const double *t_ckpt, int n_ckpt);
+int dna_irt_finish(struct dna_ctx *ctx, uint64_t seed, dna_chem_out *out);
+
 /* Many independent IRT instances in one launch: instance i owns radical entries
  * [inst_off[i], inst_off[i+1]) of rad (host pointer, n_inst+1 offsets). */
 int dna_run_irt_batch(struct dna_ctx *ctx, const float *rad, const uint64_t *inst_off,

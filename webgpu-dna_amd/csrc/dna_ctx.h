@@ -97,4 +97,8 @@ struct dna_ctx {
     float4 *d_oh = nullptr;            size_t cap_oh = 0;      // alive OH at 1 us
     uint64_t oh_n = 0;
     void *na_scratch = nullptr;        // device address of the fixed scoring array (dna_score_noalloc)
+    // an IRT staged by dna_irt_stage for dna_irt_finish (dropped by any other IRT call), and
+    // the count of transport runs that rewrote the resident radicals
+    void *irt_staged = nullptr;
+    uint64_t rad_epoch = 0;
 };

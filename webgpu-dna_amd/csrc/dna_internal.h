@@ -17,6 +17,7 @@ void set_error(const std::string &msg);
 // give it back when destroyed; the last release frees the context.
 void ctx_retain(dna_ctx *c);
 void ctx_release(dna_ctx *c);
+void irt_staged_drop(dna_ctx *c);          // frees a staged IRT (irt.hip)
 
 // Device address of dna_score_noalloc's fixed array on `device` (the current device),
 // looked up once per device.

@@ -1020,10 +1020,12 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
 {
     const float *lbr = (const float *)A.lst + 6 * T.part_off;
     const float4 *lps = A.lpos + 6 * T.part_off;
-    const int ns = sh.nseg, np = sh.np;
-    const uint32_t tot = sh.seg_u0[ns];
-    const uint32_t pidx1 = sh.prod_idx[1];
-    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    // one wavefront (the loop kernel's workgroup): the segment count, the flattened total and
+    // the chunk index are scalars, so the chunk loop is a scalar loop
+    const int ns = __builtin_amdgcn_readfirstlane(sh.nseg), np = __builtin_amdgcn_readfirstlane(sh.np);
+    const uint32_t tot = __builtin_amdgcn_readfirstlane(sh.seg_u0[ns]);
+    const uint32_t pidx1 = __builtin_amdgcn_readfirstlane(sh.prod_idx[1]);
+    const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
     // written and read by different lanes between wave barriers: volatile, and typed as LDS
     // so that the accesses stay LDS instructions (a generic volatile pointer compiles to
@@ -1045,13 +1047,13 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
         pj_n = lps[slot];                              // x, y, z, the particle index
         bt_n = (ps_n & 8u) ? lbr[slot] : 0.f;          // birth time rounded down (appended part)
     };
-    if (w * 64u + lane < tot) fetch(w * 64u + lane);
-    for (uint32_t u0 = w * 64u; u0 < tot; u0 += IRT_NT) {
+    if (lane < tot) fetch(lane);
+    for (uint32_t u0 = 0; u0 < tot; u0 += 64u) {
         const uint32_t u = u0 + lane;
         const uint32_t psg = ps_n;
         const float4 pj = pj_n;
         const float bt = bt_n;
-        if (u + IRT_NT < tot) fetch(u + IRT_NT);
+        if (u + 64u < tot) fetch(u + 64u);
         const uint32_t jj = __float_as_uint(pj.w);
         bool keep = false;
         uint32_t ent = 0;
@@ -2517,87 +2519,164 @@ static void init_outs(uint32_t n_inst, const std::vector<uint64_t> &off, const d
     }
 }
 
-// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
-static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
-                          const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
-                          dna_chem_out *outs, const uint8_t *sel = nullptr, uint32_t n_sel = 0,
-                          uint32_t *track_tally = nullptr)
+// One IRT grouped and prepared on the context's stream, its kernels not yet launched: what
+// irt_stage leaves for irt_finish.  It refers to the context's IRT buffers and to the
+// radical entries it was staged from.
+struct IrtStaged {
+    IrtGroup G;
+    std::vector<uint64_t> off;
+    std::vector<uint32_t> n_therm;
+    std::vector<float> E;
+    std::vector<double> tck;
+    int nc = 0;
+    const float4 *d_rad = nullptr;
+    uint64_t rad_n = 0;
+    bool want_tt = false;
+    bool empty = false;                 // no entries or no tracks: nothing to launch
+    bool resident = false;              // staged from the context's resident radicals
+    uint64_t rad_epoch = 0;             // the context's radical epoch when staged
+    int attempt = 0, ev_mult = 8;
+    uint64_t part = 0, evs = 0;
+    IrtBufs b;
+    size_t tally_bytes = 0, tt_bytes = 0;
+    int block = 256, cell_m = 0;
+};
+
+namespace dna {
+void irt_staged_drop(dna_ctx *c)
+{
+    delete (IrtStaged *)c->irt_staged;
+    c->irt_staged = nullptr;
+}
+}  // namespace dna
+
+// the kernel arguments of a staged IRT (the launch-choice trials take seed 0: their output
+// is discarded)
+static IrtArgs staged_args(dna_ctx *c, const IrtStaged &S, uint64_t seed)
+{
+    IrtArgs A = irt_args(S.G, S.d_rad, S.b, S.part, S.evs, S.tck.data(), S.nc, seed);
+    A.drain_at = c->irt_drain;
+    if (!c->irt_order) A.order = nullptr;
+    if (c->irt_final) {
+        A.final_ntot = c->d_ntot;
+        A.tdeath = (double *)((char *)c->irt.scratch + ((S.part * 65 + 7) & ~(uint64_t)7));
+    }
+    return A;
+}
+
+// capacities, buffers, the track list on the device and the launch choice of S.attempt
+static int irt_stage_attempt(dna_ctx *c, IrtStaged &S)
+{
+    const uint32_t ntr = (uint32_t)S.G.tr.size();
+    irt_size(S.G, S.attempt, S.ev_mult, S.part, S.evs);
+    DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch, bytes_scratch(S.part, S.rad_n)));
+    DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, bytes_events(S.evs, ntr)));
+    S.tally_bytes = (size_t)S.G.n_inst * TALLY_STRIDE * 8;
+    S.tt_bytes = S.want_tt ? (size_t)ntr * S.nc * 5 * 4 : 0;
+    DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, bytes_block(S.G, S.tally_bytes, S.tt_bytes)));
+    S.b = IrtBufs();
+    S.b.scratch = (char *)c->irt.scratch; S.b.events = (char *)c->irt.events;
+    carve_block(S.G, (char *)c->irt.tally, S.tally_bytes, S.want_tt, S.b);
+    int rc = irt_prepare(c, S.G, S.b); if (rc) return rc;
+    IrtArgs A = staged_args(c, S, 0);
+    rc = irt_pick_launch(c, S.G, A, S.b, S.block); if (rc) return rc;
+    S.cell_m = A.cell_m;
+    return DNA_OK;
+}
+
+// Everything of an IRT before its kernels: grouping of the entries into tracks (host round
+// trips for the counts), capacities, the track list and bucket indices on the device, the
+// launch choice.  It replaces any earlier final state: the buffers are reused.
+static int irt_stage(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
+                     const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, bool want_tt,
+                     IrtStaged &S, const uint8_t *sel = nullptr, uint32_t n_sel = 0)
 {
     hipStream_t st = c->stream;
     const uint32_t n_inst = (uint32_t)off.size() - 1;
     int rc = ensure_const(c); if (rc) return rc;
-    init_outs(n_inst, off, t_ckpt, nc, outs);
     c->final_ready = false;                   // this call replaces any earlier final state
-    if (rad_n == 0) {                         // nothing to react: an empty final state
-        if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = seed; }
-        return DNA_OK;
-    }
+    S.off = off; S.n_therm.assign(n_therm, n_therm + n_inst); S.E.assign(E, E + n_inst);
+    S.tck.assign(t_ckpt, t_ckpt + nc); S.nc = nc;
+    S.d_rad = d_rad; S.rad_n = rad_n; S.want_tt = want_tt;
+    S.G.n_inst = n_inst;
     DNA_HIP(hipEventRecord(c->ev[5], st));
-    IrtGroup G;
-    rc = irt_group(c, d_rad, rad_n, off, sel, n_sel, G); if (rc) return rc;
-    const uint32_t ntr = (uint32_t)G.tr.size();
+    if (rad_n == 0) { S.empty = true; return DNA_OK; }
+    rc = irt_group(c, d_rad, rad_n, off, sel, n_sel, S.G); if (rc) return rc;
+    if (S.G.tr.empty()) { S.empty = true; return DNA_OK; }
+    if (c->irt_final) DNA_HIP(dgrow<uint32_t>(*(void **)&c->d_ntot, c->cap_ntot, (uint32_t)S.G.tr.size()));
+    return irt_stage_attempt(c, S);
+}
+
+// The IRT kernels of a staged IRT, the read-back of its tallies and the outputs (a launch
+// whose capacities overflowed is staged again with larger ones and repeated).
+// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
+static int irt_finish(dna_ctx *c, IrtStaged &S, uint64_t seed, dna_chem_out *outs,
+                      uint32_t *track_tally = nullptr, uint32_t n_sel = 0)
+{
+    hipStream_t st = c->stream;
+    const uint32_t n_inst = S.G.n_inst, nc = (uint32_t)S.nc;
+    init_outs(n_inst, S.off, S.tck.data(), S.nc, outs);
+    c->final_ready = false;
     for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
-    for (auto &t : G.tr) outs[t.inst].n_tracks++;
-    if (ntr == 0) {
+    for (auto &t : S.G.tr) outs[t.inst].n_tracks++;
+    if (S.empty) {
         if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = seed; }
         return DNA_OK;
     }
-    if (c->irt_final) DNA_HIP(dgrow<uint32_t>(*(void **)&c->d_ntot, c->cap_ntot, ntr));
-    int ev_mult = 8;
-    for (int attempt = 0; attempt < 4; attempt++) {
-        uint64_t part = 0, evs = 0;
-        irt_size(G, attempt, ev_mult, part, evs);
-        DNA_HIP(dgrow<char>(c->irt.scratch, c->irt.cap_scratch, bytes_scratch(part, rad_n)));
-        DNA_HIP(dgrow<char>(c->irt.events, c->irt.cap_events, bytes_events(evs, ntr)));
-        const size_t tally_bytes = (size_t)n_inst * TALLY_STRIDE * 8;
-        const size_t tt_bytes = track_tally ? (size_t)ntr * nc * 5 * 4 : 0;
-        DNA_HIP(dgrow<char>(c->irt.tally, c->irt.cap_tally, bytes_block(G, tally_bytes, tt_bytes)));
-        IrtBufs b;
-        b.scratch = (char *)c->irt.scratch; b.events = (char *)c->irt.events;
-        carve_block(G, (char *)c->irt.tally, tally_bytes, track_tally != nullptr, b);
-        rc = irt_prepare(c, G, b); if (rc) return rc;
-        IrtArgs A = irt_args(G, d_rad, b, part, evs, t_ckpt, nc, seed);
-        A.drain_at = c->irt_drain;
-        if (!c->irt_order) A.order = nullptr;
-        if (c->irt_final) {
-            A.final_ntot = c->d_ntot;
-            A.tdeath = (double *)((char *)c->irt.scratch + ((part * 65 + 7) & ~(uint64_t)7));
-        }
-        int block = 256;
-        rc = irt_pick_launch(c, G, A, b, block); if (rc) return rc;
-        c->irt_last_block = block; c->irt_last_cell = A.cell_m ? A.cell_m : -1;
+    const uint32_t ntr = (uint32_t)S.G.tr.size();
+    for (;;) {
+        IrtArgs A = staged_args(c, S, seed);
+        A.cell_m = S.cell_m;
+        c->irt_last_block = S.block; c->irt_last_cell = A.cell_m ? A.cell_m : -1;
         DNA_HIP(hipEventRecord(c->ev[7], st));
-        irt_launch(G, A, ntr, block, st);
+        irt_launch(S.G, A, ntr, S.block, st);
         DNA_HIP(hipEventRecord(c->ev[4], st));
         DNA_HIP(hipGetLastError());
-        uint32_t ovf = 0;
-        DNA_HIP(hipMemcpyAsync(&ovf, b.ovf, 4, hipMemcpyDeviceToHost, st));
-        std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE);
-        DNA_HIP(hipMemcpyAsync(ht.data(), b.tally, tally_bytes, hipMemcpyDeviceToHost, st));
+        // the tallies and the overflow word behind them in one copy
+        std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE + 1);
+        DNA_HIP(hipMemcpyAsync(ht.data(), S.b.tally, S.tally_bytes + 8, hipMemcpyDeviceToHost, st));
         std::vector<uint32_t> htt(track_tally ? (size_t)ntr * nc * 5 : 0);
-        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), b.tt, tt_bytes, hipMemcpyDeviceToHost, st));
+        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), S.b.tt, S.tt_bytes, hipMemcpyDeviceToHost, st));
         DNA_HIP(hipEventRecord(c->ev[6], st));
         DNA_HIP(hipStreamSynchronize(st));
-        if (ovf & 2u) { ev_mult *= 4; continue; }
-        if (ovf & 1u) { continue; }
+        const uint32_t ovf = (uint32_t)ht[(size_t)n_inst * TALLY_STRIDE];
+        if (ovf) {
+            if (ovf & 2u) S.ev_mult *= 4;
+            if (++S.attempt >= 4) break;
+            int rc = irt_stage_attempt(c, S); if (rc) return rc;
+            continue;
+        }
         if (track_tally) {
             memset(track_tally, 0, (size_t)n_sel * nc * 5 * 4);
             for (uint32_t j = 0; j < ntr; j++)
-                if (G.tr[j].inst == 0 && G.tr[j].pid < n_sel)
-                    memcpy(track_tally + (size_t)G.tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
+                if (S.G.tr[j].inst == 0 && S.G.tr[j].pid < n_sel)
+                    memcpy(track_tally + (size_t)S.G.tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
         }
         float ms = 0, kms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);     // timings only
         (void)hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
-        irt_outputs(G, ht.data(), n_therm, E, nc, ms, kms, outs);
+        irt_outputs(S.G, ht.data(), S.n_therm.data(), S.E.data(), S.nc, ms, kms, outs);
         if (c->irt_final) {
-            c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = b.tr;
+            c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = S.b.tr;
             c->final_pos = A.pos; c->final_alive = A.alive; c->final_tb = A.tb; c->final_td = A.tdeath;
         }
         return DNA_OK;
     }
     set_error("dna_run_irt: event/particle capacity kept overflowing");
     return DNA_ERR_CAPACITY;
+}
+
+// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
+static int run_irt_device(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std::vector<uint64_t> &off,
+                          const uint32_t *n_therm, const float *E, const double *t_ckpt, int nc, uint64_t seed,
+                          dna_chem_out *outs, const uint8_t *sel = nullptr, uint32_t n_sel = 0,
+                          uint32_t *track_tally = nullptr)
+{
+    irt_staged_drop(c);                       // its buffers are about to be reused
+    IrtStaged S;
+    int rc = irt_stage(c, d_rad, rad_n, off, n_therm, E, t_ckpt, nc, track_tally != nullptr, S, sel, n_sel);
+    if (rc) return rc;
+    return irt_finish(c, S, seed, outs, track_tally, n_sel);
 }
 
 static int stage_rad(dna_ctx *c, const float *rad, uint64_t rad_n, const float4 **d_rad)
@@ -2624,6 +2703,43 @@ extern "C" int dna_run_irt(dna_ctx *c, const float *rad, uint64_t rad_n, uint32_
     if (rc) return rc;
     std::vector<uint64_t> off = {0, rad_n};
     return run_irt_device(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, seed, out);
+}
+
+// dna_run_irt in two calls, so that a caller can group and prepare one batch while the GPU
+// still runs another batch's IRT kernels
+extern "C" int dna_irt_stage(dna_ctx *c, const float *rad, uint64_t rad_n, uint32_t n_therm, float E_eV,
+                             const double *t_ckpt, int n_ckpt)
+{
+    if (!c || !t_ckpt || n_ckpt < 1 || n_ckpt > DNA_MAX_CKPT) { set_error("dna_irt_stage: bad arguments"); return DNA_ERR_ARG; }
+    DNA_HIP(hipSetDevice(c->device));
+    irt_staged_drop(c);
+    const float4 *d_rad = nullptr;
+    int rc = stage_rad(c, rad, rad_n, &d_rad);
+    if (rc) return rc;
+    IrtStaged *S = new IrtStaged();
+    S->resident = rad == nullptr;
+    S->rad_epoch = c->rad_epoch;
+    std::vector<uint64_t> off = {0, rad_n};
+    rc = irt_stage(c, d_rad, rad_n, off, &n_therm, &E_eV, t_ckpt, n_ckpt, false, *S);
+    if (rc) { delete S; return rc; }
+    c->irt_staged = S;
+    return DNA_OK;
+}
+
+extern "C" int dna_irt_finish(dna_ctx *c, uint64_t seed, dna_chem_out *out)
+{
+    if (!c || !out) { set_error("dna_irt_finish: bad arguments"); return DNA_ERR_ARG; }
+    IrtStaged *S = (IrtStaged *)c->irt_staged;
+    if (!S) { set_error("dna_irt_finish: no IRT staged on this context (dna_irt_stage, and no IRT call since)"); return DNA_ERR_ARG; }
+    if (S->resident && S->rad_epoch != c->rad_epoch) {
+        irt_staged_drop(c);
+        set_error("dna_irt_finish: the resident radicals changed since dna_irt_stage");
+        return DNA_ERR_ARG;
+    }
+    DNA_HIP(hipSetDevice(c->device));
+    const int rc = irt_finish(c, *S, seed, out);
+    irt_staged_drop(c);
+    return rc;
 }
 
 static int check_batch_args(const char *fn, const uint64_t *inst_off, uint32_t n_inst, std::vector<uint64_t> &off)
@@ -2709,6 +2825,7 @@ static int plan_build(dna_irt_plan *P)
     dna_ctx *c = P->c;
     hipStream_t st = c->stream;
     plan_release_graph(P);
+    irt_staged_drop(c);                       // the grouping scratch is reused
     int rc = irt_group(c, P->rad, P->G.rad_n, P->off, nullptr, 0, P->G); if (rc) return rc;
     IrtGroup &G = P->G;
     const uint32_t ntr = (uint32_t)G.tr.size();

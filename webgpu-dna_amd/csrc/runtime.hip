@@ -231,6 +231,7 @@ static void ctx_free(dna_ctx *c)
 {
     (void)hipSetDevice(c->device);           // teardown: nothing to report to
     (void)hipStreamSynchronize(c->stream);
+    irt_staged_drop(c);
     void *dev[] = {c->d_tab, c->d_guide, c->d_ctr, c->d_u64, c->d_results, c->d_dose, c->d_rad_raw, c->d_rad_key,
                    c->d_rad, c->d_sec, c->d_prim_nsec, c->d_prim_nrad, c->d_sec_nrad, c->d_energies,
                    c->d_base_p, c->d_cnt, c->d_block, c->d_sec_base, c->irt.scratch, c->irt.events, c->irt.tracks,
@@ -508,6 +509,7 @@ extern "C" int dna_run_transport(dna_ctx *c, const dna_params *p_in, const float
         (void)hipEventElapsedTime(&tt, c->ev[0], c->ev[4]);
         out->t_prim_ms = ta; out->t_sec_ms = tb; out->t_total_ms = tt;
         c->rad_n = rad_n;
+        c->rad_epoch++;
         c->last_n = p.n;
         return DNA_OK;
     }

@@ -72,6 +72,8 @@ def lib():
     L.dna_run_transport.argtypes = [vp, C.POINTER(DnaParams), vp, u32, C.POINTER(DnaTransportOut)]
     L.dna_run_irt.argtypes = [vp, vp, u64, u32, C.c_float, vp, C.c_int, u64, C.POINTER(DnaChemOut)]
     L.dna_run_irt_batch.argtypes = [vp, vp, vp, u32, vp, vp, vp, C.c_int, u64, vp]
+    L.dna_irt_stage.argtypes = [vp, vp, u64, u32, C.c_float, vp, C.c_int]
+    L.dna_irt_finish.argtypes = [vp, u64, C.POINTER(DnaChemOut)]
     L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
                             C.POINTER(DnaScoreOut)]
     L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
@@ -206,6 +208,20 @@ class Context:
         n = rad_n if r is None else r.shape[0]
         _check("dna_run_irt", lib().dna_run_irt(self.h, _ptr(r), n, n_therm, E_eV, _ptr(tc),
                                                 tc.size, seed, C.byref(o)))
+        return o
+
+    def irt_stage(self, rad, rad_n, n_therm, E_eV, t_ckpt):
+        """irt() in two calls (dna_irt_stage / dna_irt_finish): the grouping and preparation
+        now, the IRT kernels at irt_finish(seed).  rad None = the device-resident radicals,
+        which must not change before irt_finish."""
+        r = None if rad is None else np.ascontiguousarray(rad, np.float32).reshape(-1, 4)
+        tc = np.ascontiguousarray(t_ckpt, np.float64)
+        n = rad_n if r is None else r.shape[0]
+        _check("dna_irt_stage", lib().dna_irt_stage(self.h, _ptr(r), n, n_therm, E_eV, _ptr(tc), tc.size))
+
+    def irt_finish(self, seed=1):
+        o = DnaChemOut()
+        _check("dna_irt_finish", lib().dna_irt_finish(self.h, seed, C.byref(o)))
         return o
 
     def irt_batch(self, rad, inst_off, n_therm, E, t_ckpt, seed=1):
