@@ -309,6 +309,11 @@ int dna_xs_probe(struct dna_ctx *ctx, int which, const float *E, const float *r,
  * (queued initial pairs + initial particles, within runs of 4096 tracks), 0 in launch
  * order.  Results do not depend on it. */
 #define DNA_OPT_IRT_ORDER       14
+/* DNA_OPT_IRT_EVBLOCKS: at most this many event blocks per track (1..128, default 128) in
+ * the IRT's event loop, which keeps each block's least event and selects among the blocks:
+ * a block holds 64 events, doubled as often as the track's events need.  Fewer blocks make
+ * a cheaper selection and a dearer refresh.  Results do not depend on it. */
+#define DNA_OPT_IRT_EVBLOCKS    15
 /* DNA_OPT_IRT_CELL: cells of the initial-pair search of the IRT (the spatial hash that
  * stands in for the worker's all-pairs scan, public/irt-worker.js:541-558; the cell
  * semantics of src/shaders/chemistry.wgsl:66-80 with one axis): 0 = autotune together with

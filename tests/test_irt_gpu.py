@@ -189,3 +189,25 @@ def test_irt_track_beyond_lds_alive_bits(ctx):
     o = O.irt(rad, 2, 10000.0, CK, seed=9, n_threads=4)
     assert g.n_tracks == o.n_tracks == 2 and o.n_reacted > 2000
     _same(g, o)
+
+
+@pytest.mark.parametrize("blocks", [1, 3])
+def test_irt_few_event_blocks(ctx, rad10k, blocks):
+    """DNA_OPT_IRT_EVBLOCKS: with 1 or 3 event blocks per track the blocks double many times
+    during the loop (each doubling merges the minima of block pairs); tallies still equal
+    the oracle's."""
+    from dna_amd.abi import DNA_OPT_IRT_EVBLOCKS
+    print(f"irt 10 keV with {blocks} event blocks per track", flush=True)
+    ctx.set_option(DNA_OPT_IRT_EVBLOCKS, blocks)
+    try:
+        assert ctx.get_option(DNA_OPT_IRT_EVBLOCKS) == blocks
+        g = ctx.irt(rad10k, None, 48, 10000.0, CK, seed=4)
+    finally:
+        ctx.set_option(DNA_OPT_IRT_EVBLOCKS, 128)
+    o = O.irt(rad10k, 48, 10000.0, CK, seed=4)
+    _same(g, o)
+    from dna_amd.lib import DnaError
+    for bad in (0, 129):
+        with pytest.raises(DnaError):
+            ctx.set_option(DNA_OPT_IRT_EVBLOCKS, bad)
+    assert ctx.get_option(DNA_OPT_IRT_EVBLOCKS) == 128

@@ -74,6 +74,7 @@ struct dna_ctx {
     int irt_cell = 0;
     int irt_drain = 1;                 // DNA_OPT_IRT_DRAIN: where the initial pairs are drained
     int irt_order = 1;                 // DNA_OPT_IRT_ORDER: loop workgroups heaviest track first
+    int irt_evblocks = 128;            // DNA_OPT_IRT_EVBLOCKS: event blocks per track at most
     int irt_last_block = 0;            // workgroup size of the last IRT launch
     int irt_last_cell = 0;             // cells of the last IRT launch (-1 = none)
     std::map<int, std::pair<int, int>> irt_tuned;   // log2(mean particles per track) -> (size, cells)

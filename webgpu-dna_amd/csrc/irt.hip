@@ -71,6 +71,7 @@ struct IrtArgs {
     uint4 *pend;                  // stage-2 queue, same per-track offset/capacity as events
     float4 *sorted;               // species-sorted copy of the initial particles
     uint32_t lds_slots;           // float4 slots of dynamic LDS for the sorted copy (pairs kernel)
+    uint32_t bm_max;              // event blocks per track at most (1..BM_MAX)
     int drain_at;                 // the initial pairs' drain: 0 in k_irt_loop, 1 in k_irt_pairs,
                                   // 2 in k_irt_drain
     int cell_m;                   // initial-pair hash: 0 = one cell per species, m = cells of
@@ -393,7 +394,7 @@ __global__ void k_bucket(const uint32_t *kept, const uint32_t *S, const uint32_t
 //                latency.
 #define IRT_BLOCK 256                // largest workgroup of k_irt_pairs
 #define IRT_NT (blockDim.x)
-#define BM_MAX 128u                  // event blocks per track (a block holds 64 k events)
+#define BM_MAX 128u                  // event blocks per track at most (a block holds 64 2^k events)
 #define STAGE_MAX 64u                // events staged per reaction (more: blocks rescanned)
 #define AL_LDS_MAX 2048u             // bytes of LDS for the loop kernel's alive bits (16,384 slots)
 #define TSTATE 128                   // u32 per track handed from k_irt_pairs to k_irt_loop:
@@ -1645,6 +1646,34 @@ __device__ __forceinline__ void block_min(const IrtArgs &A, const IrtTrack &T, L
     if (lane == 0) { sh.bm_t[j] = bt; sh.bm_ab[j] = bab; }
 }
 
+// Double the event blocks until bm_max of them cover ne events: block j of the doubled size
+// holds the old blocks 2j and 2j+1, so its minimum is the lesser of theirs (the loop
+// kernel's one wavefront; at most BM_MAX / 2 <= 64 new blocks, one per lane).
+__device__ __forceinline__ void grow_blocks(LoopShared &sh, uint32_t ne, uint32_t bm_max)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    static_assert(BM_MAX <= 128u, "one new block per lane");
+    while ((uint64_t)sh.bs * bm_max < ne) {
+        const uint32_t nb = sh.nblk, nb2 = (nb + 1u) / 2u;
+        double t = 1e300;
+        uint64_t k = ~0ull;
+        if (lane < nb2) {
+            t = sh.bm_t[2 * lane]; k = sh.bm_ab[2 * lane];
+            if (2 * lane + 1 < nb && ev_less(sh.bm_t[2 * lane + 1], sh.bm_ab[2 * lane + 1], t, k)) {
+                t = sh.bm_t[2 * lane + 1]; k = sh.bm_ab[2 * lane + 1];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = lane; j < BM_MAX; j += 64u) {
+            sh.bm_t[j] = j < nb2 ? t : 1e300;
+            sh.bm_ab[j] = j < nb2 ? k : ~0ull;
+        }
+        __syncthreads();
+        if (lane == 0) { sh.bs *= 2u; sh.nblk = nb2; }
+        __syncthreads();
+    }
+}
+
 // The initial pairs' drain as a kernel of its own (DNA_OPT_IRT_DRAIN = 2): one
 // DRAIN_NT-thread workgroup per track, light registers and 4 KB of LDS, between the pairs
 // and the loop kernels; the event count goes to the loop kernel through tstate[21].
@@ -1739,9 +1768,6 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         }
         sh.cnt[0] = st[7]; sh.cnt[1] = st[8]; sh.cnt[2] = st[9]; sh.cnt[3] = 0; sh.cnt[4] = st[10];
         for (int r = 0; r < 9; r++) sh.rxc[r] = 0;
-        // blocks of 64 k events, k the smallest that covers the event capacity with BM_MAX blocks
-        const uint32_t kb = (T.ev_cap + 64u * BM_MAX - 1u) / (64u * BM_MAX);
-        sh.bs = 64u * (kb ? kb : 1u);
     }
     if (lane < 36) {
         const int r = C.rxnMap[lane];
@@ -1782,8 +1808,13 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
     }
     const unsigned long long n_init_ev = sh.n_ev;
     if (lane == 0) {
+        // blocks of bs events, bs the least 64 * 2^k that covers the events so far with
+        // bm_max blocks; it doubles when the events outgrow them (grow_blocks)
         const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap;
-        sh.nblk = (ne + sh.bs - 1) / sh.bs;
+        uint32_t bs = 64u;
+        while ((uint64_t)bs * A.bm_max < ne) bs *= 2u;
+        sh.bs = bs;
+        sh.nblk = (ne + bs - 1) / bs;
     }
     __syncthreads();
     for (uint32_t j = 0; j < sh.nblk; j++) block_min(A, T, sh, al, j);
@@ -1940,7 +1971,9 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
         // merge this reaction's events into the block minima: staged (at most STAGE_MAX
         // events, so at most two blocks), else by rescanning the blocks they went to
         {
-            const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap, e0 = sh.e0, bs = sh.bs;
+            const uint32_t ne = sh.n_ev < T.ev_cap ? sh.n_ev : T.ev_cap, e0 = sh.e0;
+            if ((uint64_t)sh.bs * A.bm_max < ne) grow_blocks(sh, ne, A.bm_max);
+            const uint32_t bs = sh.bs;
             if (ne > e0) {
                 const uint32_t j0 = e0 / bs, j1 = (ne - 1) / bs;
                 if (sh.nt_n > STAGE_MAX) {
@@ -2375,6 +2408,7 @@ static IrtArgs irt_args(const IrtGroup &G, const float4 *d_rad, const IrtBufs &b
     A.tally = b.tally; A.track_tally = b.tt; A.overflow = b.ovf;
     for (int k = 0; k < nc; k++) A.tc[k] = t_ckpt[k];
     A.nc = nc; A.seed = seed; A.seed_ptr = nullptr;
+    A.bm_max = BM_MAX;
     // the sorted copy lives in LDS for tracks of up to 3072 particles (48 KB), in global
     // memory above; the drain's long list reuses that LDS afterwards (at least 128 slots:
     // 512 list entries, twice the largest workgroup, as the small-queue drain needs)
@@ -2555,6 +2589,7 @@ void irt_staged_drop(dna_ctx *c)
 static IrtArgs staged_args(dna_ctx *c, const IrtStaged &S, uint64_t seed)
 {
     IrtArgs A = irt_args(S.G, S.d_rad, S.b, S.part, S.evs, S.tck.data(), S.nc, seed);
+    A.bm_max = (uint32_t)c->irt_evblocks;
     A.drain_at = c->irt_drain;
     if (!c->irt_order) A.order = nullptr;
     if (c->irt_final) {
@@ -2842,6 +2877,7 @@ static int plan_build(dna_irt_plan *P)
     P->d_seed = (uint64_t *)(P->buf + s_bytes + e_bytes + b_bytes);
     rc = irt_prepare(c, G, b); if (rc) return rc;
     IrtArgs A = irt_args(G, P->rad, b, part, evs, P->tck.data(), P->nc, 0);
+    A.bm_max = (uint32_t)c->irt_evblocks;
     A.drain_at = c->irt_drain;
     if (!c->irt_order) A.order = nullptr;
     if (ntr) { rc = irt_pick_launch(c, G, A, b, P->block); if (rc) return rc; }

@@ -177,6 +177,10 @@ extern "C" int dna_set_option(dna_ctx *c, int option, int value)
         if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_IRT_ORDER must be 0 or 1"); return DNA_ERR_ARG; }
         c->irt_order = value;
         return DNA_OK;
+    case DNA_OPT_IRT_EVBLOCKS:
+        if (value < 1 || value > 128) { set_error("dna_set_option: DNA_OPT_IRT_EVBLOCKS must be 1..128"); return DNA_ERR_ARG; }
+        c->irt_evblocks = value;
+        return DNA_OK;
     case DNA_OPT_MLP_FUSED:
         if (value != 0 && value != 1) { set_error("dna_set_option: DNA_OPT_MLP_FUSED must be 0 or 1"); return DNA_ERR_ARG; }
         c->mlp_fused = value;
@@ -217,6 +221,7 @@ extern "C" int dna_get_option(dna_ctx *c, int option, int *value)
     case DNA_OPT_MLP_FUSED: *value = c->mlp_fused; return DNA_OK;
     case DNA_OPT_IRT_DRAIN: *value = c->irt_drain; return DNA_OK;
     case DNA_OPT_IRT_ORDER: *value = c->irt_order; return DNA_OK;
+    case DNA_OPT_IRT_EVBLOCKS: *value = c->irt_evblocks; return DNA_OK;
     case DNA_OPT_WORK_CHUNK: *value = c->work_chunk; return DNA_OK;
     case DNA_OPT_PA_LANES_LAST: *value = c->last_pa_lanes; return DNA_OK;
     case DNA_OPT_RAD_CHUNK_LAST: *value = c->last_rad_chunk; return DNA_OK;

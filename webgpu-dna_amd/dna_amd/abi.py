@@ -12,6 +12,7 @@ DNA_OPT_PA_LANES_LAST, DNA_OPT_RAD_CHUNK_LAST, DNA_OPT_WORK_CHUNK_LAST = 9, 10, 
 DNA_OPT_MLP_FUSED = 12
 DNA_OPT_IRT_DRAIN = 13
 DNA_OPT_IRT_ORDER = 14
+DNA_OPT_IRT_EVBLOCKS = 15
 
 
 class DnaParams(C.Structure):
