@@ -984,7 +984,7 @@ __device__ __forceinline__ void repair_exact(const IrtArgs &A, const IrtTrack &T
             }
             dx -= g[0] * sg; dy -= g[1] * sg; dz -= g[2] * sg;
         }
-        const float r0f = sqrt_bound(dx * dx + dy * dy + dz * dz);
+        const float r0f = sqrt_bound(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         const float M = sg * (6e-3f + 2e-4f * gsum) + 1e-5f * (r0f + sg * gsum) + 1e-6f;
         const float r0lo = r0f - M;
         if (r0lo > c_irt.R_CUT_F || cannot_react(sh.bnd[pc], r0lo, u01_lo(ps, 6u))) return;
@@ -1070,7 +1070,7 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
                 // step is at least the true one: the screen stays conservative
                 const double dt = t_new - (double)bt;
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-                const float r = sqrt_bound(fx * fx + fy * fy + fz * fz) * 0.99999f;
+                const float r = sqrt_bound(fmaf(fz, fz, fmaf(fy, fy, fx * fx))) * 0.99999f;
                 const float sgm = dt > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt)) : 0.f;
                 // first without any draw of the step: each randn() is at most
                 // sqrt(-2 ln(1e-30)) = 11.754 in magnitude, so the step is at most 20.36 sig; a
@@ -1291,7 +1291,8 @@ __device__ __forceinline__ CellWin cell_window(const PairShared &sh, uint32_t sa
 // Initial pairs of the channel blocks (see k_irt_pairs step 2), with the sorted copy in
 // LDS (SG_LDS: DS loads) or in global memory.  A pair is dropped without FP64 work when an
 // FP32 bound decides it: the FP32 squared distance is within 3.1e-7 (relative) of the FP64
-// one (five roundings of non-negative terms), so r2 above R_CUT^2 (1 + 1e-5) or below
+// one (the differences' roundings, doubled by the squares, and the three roundings of the
+// product and the two FMAs, all of non-negative terms), so r2 above R_CUT^2 (1 + 1e-5) or below
 // R_CUT^2 (1 - 1e-5) in FP32 settles the cutoff test, and the bound of cannot_react takes
 // sqrt(r2) (1 - 1e-5) and the draw's 23-bit lower bound.  The exact FP64 distance is
 // computed near the cutoff and for the pairs that are queued.  The cells leave out only
@@ -1308,6 +1309,7 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
     // chunk then run on the scalar unit, not as per-lane copies on the VALU.
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nb = __builtin_amdgcn_readfirstlane(sh.nblk);
+    uint32_t ncand = 0, nsamp = 0;                    // this lane's, per channel block
     const uint32_t nx = __builtin_amdgcn_readfirstlane(sh.nx);
     const int BA[9] = {0, 0, 0, 1, 1, 1, 2, 1, 3}, BB[9] = {0, 1, 2, 1, 2, 3, 2, 4, 5};
     const double RC2 = c_irt.R_CUT2;
@@ -1350,16 +1352,16 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
                 j = w.wlo + jj;
             }
             if (!valid) continue;
-            wcand++;
+            ncand++;
             const float4 pj = Sg[j];
             const float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
-            const float r2f = fx * fx + fy * fy + fz * fz;
+            const float r2f = fmaf(fz, fz, fmaf(fy, fy, fx * fx));
             if (r2f > RC2_hi) continue;
             if (!(r2f < RC2_lo)) {
                 double ddx = (double)pi.x - (double)pj.x, ddy = (double)pi.y - (double)pj.y, ddz = (double)pi.z - (double)pj.z;
                 if (ddx * ddx + ddy * ddy + ddz * ddz > RC2) continue;
             }
-            wsamp++;
+            nsamp++;
             const uint32_t oi = (uint32_t)__float_as_int(pi.w), oj = (uint32_t)__float_as_int(pj.w);
             const uint32_t a = oi < oj ? oi : oj, b = oi < oj ? oj : oi;
             const uint64_t ps = dna_pair_state(tk, a, b);
@@ -1368,6 +1370,7 @@ __device__ __forceinline__ void pair_blocks(const IrtArgs &A, const IrtTrack &T,
             push_pending(A, T, sh, a, b, oi < oj ? sa : sb, oi < oj ? sb : sa, sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
         }
         __syncthreads();                              // cu is rewritten by the next block
+        wcand += ncand; wsamp += nsamp; ncand = 0; nsamp = 0;
     }
 }
 
