@@ -538,6 +538,7 @@ struct LoopShared {
     // p << 31 | sj << 28 | index
     uint32_t rq[4][128];
     uint32_t n_init;                      // particles 0 .. n_init - 1 are initial (born at 0)
+    float sg0[6];                         // per species: the step bound sig of a partner born at 0
     // event selection: blocks of bs events, the minimum live (t, a << 32 | b) of each block
     // (a minimum may have gone stale since), and the events pushed in the current reaction
     uint32_t bs, nblk, e0;
@@ -1067,11 +1068,16 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
                 const uint32_t a = idx < jj ? idx : jj, b = idx < jj ? jj : idx;
                 const uint64_t ps = dna_pair_state(tk, a, b);
                 // t_new - (birth time rounded down) >= the exact step time, so the bound's
-                // step is at least the true one: the screen stays conservative
-                const double dt = t_new - (double)bt;
+                // step is at least the true one: the screen stays conservative.  A partner of
+                // the initial part is born at 0: its step bound is the reaction's, per species
+                double dt = t_new;
+                float sgm = sh.sg0[sj];
+                if (psg & 8u) {
+                    dt = t_new - (double)bt;
+                    sgm = dt > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt)) : 0.f;
+                }
                 float fx = pi.x - pj.x, fy = pi.y - pj.y, fz = pi.z - pj.z;
                 const float r = sqrt_bound(fmaf(fz, fz, fmaf(fy, fy, fx * fx))) * 0.99999f;
-                const float sgm = dt > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[sj] * dt)) : 0.f;
                 // first without any draw of the step: each randn() is at most
                 // sqrt(-2 ln(1e-30)) = 11.754 in magnitude, so the step is at most 20.36 sig; a
                 // partner beyond R_CUT by that much cannot pair, and the draw of the first
@@ -1197,6 +1203,7 @@ __device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t pc = lane >> 1, part = lane & 1u, p = pc / 6u, sj = pc - 6u * (pc / 6u);
     const int np = sh.np;
+    if (lane < 6u) sh.sg0[lane] = t_new > 0 ? sqrt_bound((float)(2.0 * (double)sh.Df[lane] * t_new)) : 0.f;
     uint32_t lo = 0, len = 0, nalive = 0;
     if (lane < 24u && (int)p < np) {
         const uint32_t sp = sh.prod_sp[p];
