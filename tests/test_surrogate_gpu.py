@@ -208,9 +208,11 @@ def test_fused_predict_equals_per_layer_path(ctx, net, mixed):
 
 def test_surrogate_replaces_irt_on_held_out_tracks(ctx):
     """Train on 48k generated tracks; on a held-out batch of 4096 tracks the surrogate
-    chemistry (IRT fallback for uncertain tracks) matches plain IRT's aggregate G(t)
-    within 3% at every (time, species) with G >= 0.05, costs >= 5x less device time,
-    and beats the best input-independent predictor per track."""
+    chemistry (IRT fallback for the 10% most uncertain tracks) matches plain IRT's
+    aggregate G(t) within 3% at every (time, species) with G >= 0.05, costs >= 5x less
+    device time, and beats the best input-independent predictor per track.  The uncertain
+    tracks are mostly the largest (high-energy) ones, so the fallback costs more than its
+    share of tracks: with 20% of them it took 3.2 ms against 16.8 ms of plain IRT."""
     s = Surrogate(ctx, 70, 256, 3, 80, seed=3)
     try:
         print("surrogate: generating 48k training tracks", flush=True)
@@ -218,12 +220,13 @@ def test_surrogate_replaces_irt_on_held_out_tracks(ctx):
         print(f"surrogate: training ({d['t_transport_s']:.2f}s transport, {d['t_irt_s']:.2f}s IRT)", flush=True)
         losses = S.fit(s, d["X"], d["G"], epochs=(16, 8, 4))
         print(f"surrogate: losses {losses}", flush=True)
-        sigma_max = S.calibrate_sigma(s, d["X"], 0.2)
+        sigma_max = S.calibrate_sigma(s, d["X"], 0.1)
         # held-out batch: a different seed stream
-        E = S.log_uniform_energies(4096, 0xBEEF, 1000.0, 20000.0)
-        t = ctx.transport(S.transport_params(4096, seed=0xBEEF), energies=E, copy=0)
+        NH = 4096
+        E = S.log_uniform_energies(NH, 0xBEEF, 1000.0, 20000.0)
+        t = ctx.transport(S.transport_params(NH, seed=0xBEEF), energies=E, copy=0)
         tc = S.checkpoints_16()
-        o_irt, tt = ctx.irt_tracks(None, t["rad_n"], 4096, 4096, 1.0, tc, seed=5)
+        o_irt, tt = ctx.irt_tracks(None, t["rad_n"], NH, NH, 1.0, tc, seed=5)
         G_irt_tr = S.per_track_G(tt, E)
         G_irt = S.aggregate_G(G_irt_tr, E)
         res = S.surrogate_chemistry(ctx, s, t["rad_n"], E, sigma_max, seed=5)
