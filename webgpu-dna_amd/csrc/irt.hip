@@ -532,7 +532,8 @@ struct LoopShared {
     uint16_t wend[6][WB_MAX];
     // re-pairing segments (product p, partner species sj, a list range) flattened
     int      nseg;
-    uint32_t seg_u0[25], seg_lo[24];      // first flattened index, first list entry
+    uint32_t seg_u0[25];                  // first flattened index
+    uint32_t seg_base[24];                // slot of flattened index u: seg_base + u (mod 2^32)
     uint8_t  seg_ps[24];                  // p << 4 | part << 3 | sj (part 1: appended entries)
     // re-pairings that passed the bound: the partner's list entry (x, y, z) and
     // p << 31 | sj << 28 | index
@@ -1045,7 +1046,7 @@ __device__ void pair_products(const IrtArgs &A, const IrtTrack &T, LoopShared &s
     auto fetch = [&](uint32_t u) {
         while (sg + 1 < ns && u >= sh.seg_u0[sg + 1]) sg++;
         ps_n = sh.seg_ps[sg];
-        const size_t slot = (size_t)(ps_n & 7u) * T.cap + sh.seg_lo[sg] + (u - sh.seg_u0[sg]);
+        const uint32_t slot = sh.seg_base[sg] + u;
         pj_n = lps[slot];                              // x, y, z, the particle index
         bt_n = (ps_n & 8u) ? lbr[slot] : 0.f;          // birth time rounded down (appended part)
     };
@@ -1197,7 +1198,7 @@ __device__ __forceinline__ uint32_t hash_cell(float u, float umin, float cinv, u
 // distance at which pair_products' first screen drops it without a draw: leaving it out
 // changes nothing.  Also counts the re-pairings of the worker (every alive partner with a
 // channel, public/irt-worker.js:430-465) and the list slots to scan.
-__device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
+__device__ __forceinline__ void build_segments(LoopShared &sh, double t_new, uint32_t cap)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -1227,7 +1228,10 @@ __device__ __forceinline__ void build_segments(LoopShared &sh, double t_new)
     const uint64_t m = __ballot(len > 0);
     if (len > 0) {
         const uint32_t k = (uint32_t)__popcll(m & below);
-        sh.seg_u0[k] = x - len; sh.seg_lo[k] = lo; sh.seg_ps[k] = (uint8_t)((p << 4) | (part << 3) | sj);
+        // list slot sj * cap + lo + (u - u0); below 2^32, as 6 cap is for particle indices
+        // of 28 bits, so the 32-bit sum with the base taken modulo 2^32 is exact
+        sh.seg_u0[k] = x - len; sh.seg_base[k] = sj * cap + lo - (x - len);
+        sh.seg_ps[k] = (uint8_t)((p << 4) | (part << 3) | sj);
     }
     const uint32_t na = (uint32_t)__builtin_amdgcn_readlane((int)wave_scan_u32(nalive), 63);
     const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
@@ -1968,7 +1972,7 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
                 compact_lists(A, T, sh, al);
                 if (lane == 0) sh.lcomp = 0;
             }
-            build_segments(sh, sh.win_t);
+            build_segments(sh, sh.win_t, T.cap);
             __syncthreads();
             pair_products(A, T, sh, al, tk, sh.win_t);
         }
