@@ -45,6 +45,7 @@ def measured_traffic(kernel):
 # (the event-loop kernel fills every CU's slots with its own one-wavefront tracks).
 IRT_GATE = None
 IRT_STAGED = False
+IRT_OVERLAP = False
 SCORE_FIRST = False
 
 
@@ -54,7 +55,15 @@ def chain_step(ctx, p, dna, seed):
     n_therm = int((t["results"]["esc"] == 0).sum())
     if SCORE_FIRST:
         s = ctx.score(dna, None, t["rad_n"], lcg_state=0x12345678)
-    if IRT_GATE is not None and IRT_STAGED:
+    if IRT_GATE is not None and IRT_OVERLAP:
+        # staged before the gate; the gate covers the IRT until its event loop starts, so the
+        # next chain's IRT fills the GPU as this one's event-loop workgroups finish
+        ctx.irt_stage(None, t["rad_n"], n_therm, E_EV, CK)
+        with IRT_GATE:
+            ctx.irt_launch(seed)
+            ctx.irt_wait(loop_only=True)
+        c = ctx.irt_wait()
+    elif IRT_GATE is not None and IRT_STAGED:
         # group and prepare while another chain's IRT kernels run; the gate covers the kernels
         ctx.irt_stage(None, t["rad_n"], n_therm, E_EV, CK)
         with IRT_GATE:
@@ -285,6 +294,10 @@ def main():
                          "(dna_irt_finish); 0 the gate covers the whole IRT call.  Measured alike "
                          "(204-206k primaries/s): the set-up gap between two IRTs is filled by the "
                          "other chain's scoring and transport either way")
+    ap.add_argument("--irt-overlap", type=int, default=0, choices=(0, 1),
+                    help="with the gate: 1 stages each IRT before the gate and holds the gate only "
+                         "until its event-loop kernel starts (dna_irt_launch / dna_irt_wait), so "
+                         "the next chain's IRT overlaps this one's last event-loop workgroups")
     ap.add_argument("--score-first", type=int, default=0, choices=(0, 1),
                     help="1: score the step's radicals before its IRT (both read the transport's "
                          "radicals only), so the next transport follows the IRT at once")
@@ -317,12 +330,13 @@ def main():
     from concurrent.futures import ThreadPoolExecutor
     from dna_amd import Context, DNATarget, default_params
     K = max(1, args.inflight)
-    global IRT_GATE, IRT_STAGED, SCORE_FIRST
+    global IRT_GATE, IRT_STAGED, SCORE_FIRST, IRT_OVERLAP
     if args.irt_gate and K > 1:
         import threading
         IRT_GATE = threading.Lock()
     IRT_STAGED = bool(args.irt_staged)
     SCORE_FIRST = bool(args.score_first)
+    IRT_OVERLAP = bool(args.irt_overlap)
     ctxs = [Context(0) for _ in range(K)]
     if args.irt_drain >= 0:
         from dna_amd.abi import DNA_OPT_IRT_DRAIN
@@ -410,7 +424,8 @@ def main():
                                "IRT to 1 us (FP64), SSB/DSB on the 21x21 B-DNA fiber grid",
                    "primaries": N_PRIM, "energy_eV": E_EV, "chains_in_flight": K,
                    "irt_gate": IRT_GATE is not None,
-                   "irt_staged": IRT_GATE is not None and IRT_STAGED},
+                   "irt_staged": IRT_GATE is not None and IRT_STAGED,
+                   "irt_overlap": IRT_GATE is not None and IRT_OVERLAP},
         "roofline": roof,
         "breakdown_ms": {"phase_a": round(sum(k_prim) / len(k_prim), 3),
                          "phase_b": round(sum(k_sec) / len(k_sec), 3),

@@ -205,6 +205,17 @@ int dna_irt_stage(struct dna_ctx *ctx, const float *rad, uint64_t rad_n, uint32_
                   float E_eV, const double *t_ckpt, int n_ckpt);
 int dna_irt_finish(struct dna_ctx *ctx, uint64_t seed, dna_chem_out *out);
 
+/* dna_irt_finish in two calls (no counterpart in the reference either): dna_irt_launch
+ * enqueues the staged IRT's kernels on the context's stream and returns; dna_irt_wait with
+ * DNA_IRT_WAIT_LOOP blocks until the kernels before the event loop (the pair enumeration
+ * and its drain) are done, which is when a pipelining caller can launch another batch's IRT
+ * to fill the GPU as this one's event-loop workgroups finish; with DNA_IRT_WAIT_ALL it
+ * blocks until the IRT is done and fills *out as dna_irt_finish does. */
+#define DNA_IRT_WAIT_LOOP 0
+#define DNA_IRT_WAIT_ALL  1
+int dna_irt_launch(struct dna_ctx *ctx, uint64_t seed);
+int dna_irt_wait(struct dna_ctx *ctx, int what, dna_chem_out *out);
+
 /* Many independent IRT instances in one launch: instance i owns radical entries
  * [inst_off[i], inst_off[i+1]) of rad (host pointer, n_inst+1 offsets). */
 int dna_run_irt_batch(struct dna_ctx *ctx, const float *rad, const uint64_t *inst_off,

@@ -76,3 +76,20 @@ def test_staged_empty_input(ctx):
     ctx.irt_stage(np.zeros((0, 4), np.float32), None, 0, 1000.0, CK)
     o = ctx.irt_finish(1)
     assert o.n_tracks == 0 and o.n_events == 0
+
+
+def test_launch_and_wait_equal_one_call(ctx):
+    """dna_irt_launch + dna_irt_wait (event-loop start, then all) give dna_run_irt's results;
+    waiting without a launch, or launching twice, is refused."""
+    p = default_params(10000.0, 24)
+    t = ctx.transport(p)
+    one = ctx.irt(None, t["rad_n"], 24, 10000.0, CK, seed=8)
+    print("stage, launch, wait for the event loop, wait for all", flush=True)
+    ctx.irt_stage(None, t["rad_n"], 24, 10000.0, CK)
+    ctx.irt_launch(8)
+    with pytest.raises(DnaError):
+        ctx.irt_launch(8)
+    assert ctx.irt_wait(loop_only=True) is None
+    assert _key(ctx.irt_wait()) == _key(one)
+    with pytest.raises(DnaError):
+        ctx.irt_wait()

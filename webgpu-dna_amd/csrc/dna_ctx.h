@@ -102,4 +102,5 @@ struct dna_ctx {
     // the count of transport runs that rewrote the resident radicals
     void *irt_staged = nullptr;
     uint64_t rad_epoch = 0;
+    hipEvent_t ev_loop = nullptr;      // recorded before a launched IRT's event-loop kernel
 };

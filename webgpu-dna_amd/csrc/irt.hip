@@ -2443,12 +2443,17 @@ static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStre
 
 // one IRT launch: the pairs kernel, the drain kernel when the drain runs on its own, the
 // loop kernel's workgroup order, the event loop
-static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
+static hipError_t irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int block, hipStream_t st,
+                             hipEvent_t before_loop = nullptr)
 {
     irt_launch_pairs(A, grid, block, st);
     if (A.drain_at == 2) hipLaunchKernelGGL(k_irt_drain, dim3(grid), dim3(DRAIN_NT), 0, st, A);
     if (A.order) hipLaunchKernelGGL(k_irt_order, dim3((grid + ORDER_CHUNK - 1) / ORDER_CHUNK), dim3(ORDER_NT), 0,
                                     st, (const uint32_t *)A.tstate, grid, A.order);
+    if (before_loop) {
+        const hipError_t e = hipEventRecord(before_loop, st);
+        if (e != hipSuccess) return e;
+    }
     // the death-time stores are compiled only into the variants that keep the final state;
     // alive bits in LDS when the largest track's fit in al_lds bytes, else in global memory
     const bool lb = A.al_lds > 0;
@@ -2459,6 +2464,7 @@ static void irt_launch(const IrtGroup &G, const IrtArgs &A, uint32_t grid, int b
         if (lb) hipLaunchKernelGGL((k_irt_loop<false, true>), dim3(grid), dim3(64), (size_t)A.al_lds, st, A);
         else hipLaunchKernelGGL((k_irt_loop<false, false>), dim3(grid), dim3(64), 0, st, A);
     }
+    return hipSuccess;
 }
 
 static void irt_launch_pairs(const IrtArgs &A, uint32_t grid, int block, hipStream_t st)
@@ -2588,12 +2594,20 @@ struct IrtStaged {
     IrtBufs b;
     size_t tally_bytes = 0, tt_bytes = 0;
     int block = 256, cell_m = 0;
+    bool launched = false;              // the kernels are enqueued (irt_launch_staged)
+    uint64_t seed = 0;
+    IrtArgs A;
+    std::vector<unsigned long long> ht;  // read-back: tallies, overflow word
+    std::vector<uint32_t> htt;           // read-back: per-track tallies
 };
 
 namespace dna {
 void irt_staged_drop(dna_ctx *c)
 {
-    delete (IrtStaged *)c->irt_staged;
+    IrtStaged *S = (IrtStaged *)c->irt_staged;
+    // a launched IRT's read-back copies still target S->ht / S->htt: let them land first
+    if (S && S->launched) (void)hipStreamSynchronize(c->stream);   // teardown path: nothing to report to
+    delete S;
     c->irt_staged = nullptr;
 }
 }  // namespace dna
@@ -2656,11 +2670,36 @@ static int irt_stage(dna_ctx *c, const float4 *d_rad, uint64_t rad_n, const std:
     return irt_stage_attempt(c, S);
 }
 
-// The IRT kernels of a staged IRT, the read-back of its tallies and the outputs (a launch
-// whose capacities overflowed is staged again with larger ones and repeated).
-// track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
-static int irt_finish(dna_ctx *c, IrtStaged &S, uint64_t seed, dna_chem_out *outs,
-                      uint32_t *track_tally = nullptr, uint32_t n_sel = 0)
+// The IRT kernels of a staged IRT and the read-back of its tallies, enqueued on the
+// context's stream; c->ev_loop is recorded before the event-loop kernel.
+static int irt_launch_staged(dna_ctx *c, IrtStaged &S, uint64_t seed)
+{
+    hipStream_t st = c->stream;
+    S.seed = seed;
+    S.launched = true;
+    if (S.empty) return DNA_OK;
+    const uint32_t ntr = (uint32_t)S.G.tr.size();
+    S.A = staged_args(c, S, seed);
+    S.A.cell_m = S.cell_m;
+    c->irt_last_block = S.block; c->irt_last_cell = S.A.cell_m ? S.A.cell_m : -1;
+    DNA_HIP(hipEventRecord(c->ev[7], st));
+    DNA_HIP(irt_launch(S.G, S.A, ntr, S.block, st, c->ev_loop));
+    DNA_HIP(hipEventRecord(c->ev[4], st));
+    DNA_HIP(hipGetLastError());
+    // the tallies and the overflow word behind them in one copy
+    S.ht.assign((size_t)S.G.n_inst * TALLY_STRIDE + 1, 0ull);
+    DNA_HIP(hipMemcpyAsync(S.ht.data(), S.b.tally, S.tally_bytes + 8, hipMemcpyDeviceToHost, st));
+    S.htt.assign(S.want_tt ? (size_t)ntr * S.nc * 5 : 0, 0u);
+    if (S.want_tt) DNA_HIP(hipMemcpyAsync(S.htt.data(), S.b.tt, S.tt_bytes, hipMemcpyDeviceToHost, st));
+    DNA_HIP(hipEventRecord(c->ev[6], st));
+    return DNA_OK;
+}
+
+// Wait for a launched IRT and fill the outputs (a launch whose capacities overflowed is
+// staged again with larger ones and repeated).  track_tally (optional): per-primary species
+// counts [n_sel][nc][5] of instance 0.
+static int irt_collect(dna_ctx *c, IrtStaged &S, dna_chem_out *outs, uint32_t *track_tally = nullptr,
+                       uint32_t n_sel = 0)
 {
     hipStream_t st = c->stream;
     const uint32_t n_inst = S.G.n_inst, nc = (uint32_t)S.nc;
@@ -2669,50 +2708,45 @@ static int irt_finish(dna_ctx *c, IrtStaged &S, uint64_t seed, dna_chem_out *out
     for (uint32_t i = 0; i < n_inst; i++) outs[i].n_tracks = 0;
     for (auto &t : S.G.tr) outs[t.inst].n_tracks++;
     if (S.empty) {
-        if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = seed; }
+        if (c->irt_final) { c->final_ready = true; c->final_ntr = 0; c->final_seed = S.seed; }
         return DNA_OK;
     }
     const uint32_t ntr = (uint32_t)S.G.tr.size();
     for (;;) {
-        IrtArgs A = staged_args(c, S, seed);
-        A.cell_m = S.cell_m;
-        c->irt_last_block = S.block; c->irt_last_cell = A.cell_m ? A.cell_m : -1;
-        DNA_HIP(hipEventRecord(c->ev[7], st));
-        irt_launch(S.G, A, ntr, S.block, st);
-        DNA_HIP(hipEventRecord(c->ev[4], st));
-        DNA_HIP(hipGetLastError());
-        // the tallies and the overflow word behind them in one copy
-        std::vector<unsigned long long> ht((size_t)n_inst * TALLY_STRIDE + 1);
-        DNA_HIP(hipMemcpyAsync(ht.data(), S.b.tally, S.tally_bytes + 8, hipMemcpyDeviceToHost, st));
-        std::vector<uint32_t> htt(track_tally ? (size_t)ntr * nc * 5 : 0);
-        if (track_tally) DNA_HIP(hipMemcpyAsync(htt.data(), S.b.tt, S.tt_bytes, hipMemcpyDeviceToHost, st));
-        DNA_HIP(hipEventRecord(c->ev[6], st));
         DNA_HIP(hipStreamSynchronize(st));
-        const uint32_t ovf = (uint32_t)ht[(size_t)n_inst * TALLY_STRIDE];
+        const uint32_t ovf = (uint32_t)S.ht[(size_t)n_inst * TALLY_STRIDE];
         if (ovf) {
             if (ovf & 2u) S.ev_mult *= 4;
             if (++S.attempt >= 4) break;
             int rc = irt_stage_attempt(c, S); if (rc) return rc;
+            rc = irt_launch_staged(c, S, S.seed); if (rc) return rc;
             continue;
         }
         if (track_tally) {
             memset(track_tally, 0, (size_t)n_sel * nc * 5 * 4);
             for (uint32_t j = 0; j < ntr; j++)
                 if (S.G.tr[j].inst == 0 && S.G.tr[j].pid < n_sel)
-                    memcpy(track_tally + (size_t)S.G.tr[j].pid * nc * 5, &htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
+                    memcpy(track_tally + (size_t)S.G.tr[j].pid * nc * 5, &S.htt[(size_t)j * nc * 5], (size_t)nc * 5 * 4);
         }
         float ms = 0, kms = 0;
         (void)hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);     // timings only
         (void)hipEventElapsedTime(&kms, c->ev[7], c->ev[4]);
-        irt_outputs(S.G, ht.data(), S.n_therm.data(), S.E.data(), S.nc, ms, kms, outs);
+        irt_outputs(S.G, S.ht.data(), S.n_therm.data(), S.E.data(), S.nc, ms, kms, outs);
         if (c->irt_final) {
-            c->final_ready = true; c->final_ntr = ntr; c->final_seed = seed; c->final_tracks = S.b.tr;
-            c->final_pos = A.pos; c->final_alive = A.alive; c->final_tb = A.tb; c->final_td = A.tdeath;
+            c->final_ready = true; c->final_ntr = ntr; c->final_seed = S.seed; c->final_tracks = S.b.tr;
+            c->final_pos = S.A.pos; c->final_alive = S.A.alive; c->final_tb = S.A.tb; c->final_td = S.A.tdeath;
         }
         return DNA_OK;
     }
     set_error("dna_run_irt: event/particle capacity kept overflowing");
     return DNA_ERR_CAPACITY;
+}
+
+static int irt_finish(dna_ctx *c, IrtStaged &S, uint64_t seed, dna_chem_out *outs,
+                      uint32_t *track_tally = nullptr, uint32_t n_sel = 0)
+{
+    int rc = irt_launch_staged(c, S, seed); if (rc) return rc;
+    return irt_collect(c, S, outs, track_tally, n_sel);
 }
 
 // track_tally (optional): per-primary species counts [n_sel][nc][5] of instance 0.
@@ -2775,18 +2809,57 @@ extern "C" int dna_irt_stage(dna_ctx *c, const float *rad, uint64_t rad_n, uint3
     return DNA_OK;
 }
 
+// the staged IRT of c, checked for dna_irt_launch / dna_irt_finish (fn names the caller)
+static IrtStaged *staged_for_launch(dna_ctx *c, const char *fn)
+{
+    IrtStaged *S = (IrtStaged *)c->irt_staged;
+    if (!S || S->launched) {
+        set_error(std::string(fn) + ": no IRT staged on this context (dna_irt_stage, and no IRT call since)");
+        return nullptr;
+    }
+    if (S->resident && S->rad_epoch != c->rad_epoch) {
+        irt_staged_drop(c);
+        set_error(std::string(fn) + ": the resident radicals changed since dna_irt_stage");
+        return nullptr;
+    }
+    return S;
+}
+
 extern "C" int dna_irt_finish(dna_ctx *c, uint64_t seed, dna_chem_out *out)
 {
     if (!c || !out) { set_error("dna_irt_finish: bad arguments"); return DNA_ERR_ARG; }
-    IrtStaged *S = (IrtStaged *)c->irt_staged;
-    if (!S) { set_error("dna_irt_finish: no IRT staged on this context (dna_irt_stage, and no IRT call since)"); return DNA_ERR_ARG; }
-    if (S->resident && S->rad_epoch != c->rad_epoch) {
-        irt_staged_drop(c);
-        set_error("dna_irt_finish: the resident radicals changed since dna_irt_stage");
-        return DNA_ERR_ARG;
-    }
+    IrtStaged *S = staged_for_launch(c, "dna_irt_finish");
+    if (!S) return DNA_ERR_ARG;
     DNA_HIP(hipSetDevice(c->device));
     const int rc = irt_finish(c, *S, seed, out);
+    irt_staged_drop(c);
+    return rc;
+}
+
+extern "C" int dna_irt_launch(dna_ctx *c, uint64_t seed)
+{
+    if (!c) { set_error("dna_irt_launch: bad arguments"); return DNA_ERR_ARG; }
+    IrtStaged *S = staged_for_launch(c, "dna_irt_launch");
+    if (!S) return DNA_ERR_ARG;
+    DNA_HIP(hipSetDevice(c->device));
+    const int rc = irt_launch_staged(c, *S, seed);
+    if (rc) irt_staged_drop(c);
+    return rc;
+}
+
+extern "C" int dna_irt_wait(dna_ctx *c, int what, dna_chem_out *out)
+{
+    if (!c || (what != DNA_IRT_WAIT_LOOP && what != DNA_IRT_WAIT_ALL) || (what == DNA_IRT_WAIT_ALL && !out)) {
+        set_error("dna_irt_wait: bad arguments"); return DNA_ERR_ARG;
+    }
+    IrtStaged *S = (IrtStaged *)c->irt_staged;
+    if (!S || !S->launched) { set_error("dna_irt_wait: no IRT launched on this context (dna_irt_launch)"); return DNA_ERR_ARG; }
+    DNA_HIP(hipSetDevice(c->device));
+    if (what == DNA_IRT_WAIT_LOOP) {
+        if (!S->empty) DNA_HIP(hipEventSynchronize(c->ev_loop));
+        return DNA_OK;
+    }
+    const int rc = irt_collect(c, *S, out);
     irt_staged_drop(c);
     return rc;
 }
@@ -2905,7 +2978,7 @@ static int plan_build(dna_irt_plan *P)
     hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, st, b.tally, words);
     hipError_t ce = hipGetLastError();
     if (ce == hipSuccess) ce = hipMemcpyAsync(P->d_seed, P->h_seed, 8, hipMemcpyHostToDevice, st);
-    if (ce == hipSuccess && ntr) { irt_launch(G, A, ntr, P->block, st); ce = hipGetLastError(); }
+    if (ce == hipSuccess && ntr) { ce = irt_launch(G, A, ntr, P->block, st); if (ce == hipSuccess) ce = hipGetLastError(); }
     if (ce == hipSuccess) ce = hipMemcpyAsync(P->h_tally, b.tally, P->tally_bytes + 8, hipMemcpyDeviceToHost, st);
     hipError_t e = hipStreamEndCapture(st, &P->graph);
     if (e == hipSuccess) e = ce;

@@ -85,6 +85,7 @@ extern "C" int dna_ctx_create(int device, const char *table_path, dna_ctx **out)
     DNA_HIP(hipHostMalloc((void **)&c->h_ctr, C_NCTR * sizeof(uint32_t), hipHostMallocDefault));
     DNA_HIP(hipMalloc((void **)&c->d_u64, 64 * sizeof(unsigned long long)));
     for (int i = 0; i < 8; i++) DNA_HIP(hipEventCreate(&c->ev[i]));
+    DNA_HIP(hipEventCreateWithFlags(&c->ev_loop, hipEventDisableTiming));
     {
         int rc = score_scratch_init(device, &c->na_scratch);
         if (rc) { ctx_release(c); return rc; }
@@ -246,6 +247,7 @@ static void ctx_free(dna_ctx *c)
     void *host[] = {c->h_ctr, c->h_results, c->h_dose, c->h_rad, c->sc.h_hits};
     for (void *p : host) if (p) (void)hipHostFree(p);
     for (int i = 0; i < 8; i++) (void)hipEventDestroy(c->ev[i]);
+    if (c->ev_loop) (void)hipEventDestroy(c->ev_loop);
     (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -74,6 +74,8 @@ def lib():
     L.dna_run_irt_batch.argtypes = [vp, vp, vp, u32, vp, vp, vp, C.c_int, u64, vp]
     L.dna_irt_stage.argtypes = [vp, vp, u64, u32, C.c_float, vp, C.c_int]
     L.dna_irt_finish.argtypes = [vp, u64, C.POINTER(DnaChemOut)]
+    L.dna_irt_launch.argtypes = [vp, u64]
+    L.dna_irt_wait.argtypes = [vp, C.c_int, vp]
     L.dna_score.argtypes = [vp, C.POINTER(DnaTarget), vp, u64, C.POINTER(u32), C.c_int,
                             C.POINTER(DnaScoreOut)]
     L.dna_hash_cells.argtypes = [vp, vp, u64, C.c_float, u32, vp, vp, vp]
@@ -222,6 +224,20 @@ class Context:
     def irt_finish(self, seed=1):
         o = DnaChemOut()
         _check("dna_irt_finish", lib().dna_irt_finish(self.h, seed, C.byref(o)))
+        return o
+
+    def irt_launch(self, seed=1):
+        """irt_finish in two calls: the staged IRT's kernels enqueued now."""
+        _check("dna_irt_launch", lib().dna_irt_launch(self.h, seed))
+
+    def irt_wait(self, loop_only=False):
+        """Wait for the launched IRT: until its event loop starts (loop_only, returns None),
+        or until it is done (returns its DnaChemOut)."""
+        if loop_only:
+            _check("dna_irt_wait", lib().dna_irt_wait(self.h, 0, None))
+            return None
+        o = DnaChemOut()
+        _check("dna_irt_wait", lib().dna_irt_wait(self.h, 1, C.byref(o)))
         return o
 
     def irt_batch(self, rad, inst_off, n_therm, E, t_ckpt, seed=1):
