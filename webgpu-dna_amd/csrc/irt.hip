@@ -1891,11 +1891,16 @@ __global__ void __launch_bounds__(64, 4) k_irt_loop(IrtArgs A)
             if (ri == 3 || ri == 4 || ri == 6) sh.cnt[4]++;
             int np = sh.nprod[ri];
             if (np > 0) {
-                double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
-                double wsum = sqDi + sqDj;
-                double mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
-                double my = (sqDj * pa.y + sqDi * pb.y) / wsum;
-                double mz = (sqDj * pa.z + sqDi * pb.z) / wsum;
+                // one product sits at the diffusion-weighted midpoint; two keep the reactants'
+                // positions (no FP64 divisions then)
+                double mx = 0, my = 0, mz = 0;
+                if (np == 1) {
+                    const double sqDi = sh.sqD[si], sqDj = sh.sqD[sj];
+                    const double wsum = sqDi + sqDj;
+                    mx = (sqDj * pa.x + sqDi * pb.x) / wsum;
+                    my = (sqDj * pa.y + sqDi * pb.y) / wsum;
+                    mz = (sqDj * pa.z + sqDi * pb.z) / wsum;
+                }
                 uint32_t nt = sh.n_total;
                 int made = 0;
                 for (int pp = 0; pp < np; pp++) {
